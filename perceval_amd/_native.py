@@ -1,0 +1,87 @@
+"""ctypes binding of libpcvl_hip.so, the C ABI declared in include/pcvl_hip.h.
+
+This is the only way Python reaches the HIP kernels.  torch supplies device
+memory and the current stream; nothing here computes on the CPU.  If the
+library or a GPU is missing, every entry point raises ``NativeUnavailable``:
+there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpcvl_hip.so")
+
+_lock = threading.Lock()
+_lib = None
+
+
+class NativeUnavailable(RuntimeError):
+    """The HIP library is not built or no GPU is visible."""
+
+
+class NativeError(RuntimeError):
+    """A C-ABI entry point returned a negative status."""
+
+
+c_int = ctypes.c_int
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_u64 = ctypes.c_uint64
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+c_size_t = ctypes.c_size_t
+P = ctypes.POINTER
+
+# name -> (restype, argtypes); must match include/pcvl_hip.h
+SIGNATURES = {
+    "pcvl_last_error": (ctypes.c_char_p, []),
+    "pcvl_abi_version": (c_int, []),
+    "pcvl_perm_plan_create": (c_int, [P(c_i32), P(c_i64), c_i64, c_int, P(c_void_p)]),
+    "pcvl_perm_plan_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pcvl_perm_plan_destroy": (c_int, [c_void_p]),
+    "pcvl_perm_plan_info": (c_int, [c_void_p, P(c_i64), P(c_int), P(c_int)]),
+    "pcvl_perm_batch": (c_int, [c_void_p, P(c_i32), P(c_i64), c_i64, c_void_p, c_void_p]),
+}
+
+
+def load(require_gpu: bool = True):
+    """Load the library (once).  Raises NativeUnavailable if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise NativeUnavailable(
+                    f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+                    f"or `make -C perceval_amd/csrc`")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    if require_gpu:
+        import torch
+        if not torch.cuda.is_available():
+            raise NativeUnavailable("no GPU visible: the simulation core runs only on the GPU")
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = _lib.pcvl_last_error().decode(errors="replace") if _lib is not None else ""
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise NativeError(f"{what} failed ({rc}): {msg}")
+
+
+def stream_handle(stream=None) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def ptr(t) -> int:
+    return int(t.data_ptr())

@@ -1,0 +1,100 @@
+"""Batched complex permanents on the GPU (host side of pcvl_perm_*).
+
+Replaces ``xq.permanent_cx`` (perceval/backends/_naive.py:70-71).  A batch is a
+packed complex128 device tensor holding row-major matrices back to back; the
+orders may differ from matrix to matrix (0..28) and are handled in one
+persistent launch.  ``PermanentPlan`` keeps the launch plan for a batch shape so
+repeated batches (sampler acceptance, Naive all_prob over a fixed Fock space)
+pay the host-side scheduling once.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+MAX_ORDER = 28
+
+
+class PermanentPlan:
+    """Launch plan for a fixed list of matrix orders."""
+
+    def __init__(self, orders, max_chunk_log2: int = 0):
+        lib = _native.load()
+        self.orders = np.ascontiguousarray(np.asarray(orders, dtype=np.int32).ravel())
+        if self.orders.size and (self.orders.min() < 0 or self.orders.max() > MAX_ORDER):
+            raise ValueError(f"matrix orders must lie in [0, {MAX_ORDER}]")
+        sizes = self.orders.astype(np.int64) ** 2
+        self.offsets = np.zeros(len(self.orders), dtype=np.int64)
+        if len(sizes) > 1:
+            self.offsets[1:] = np.cumsum(sizes)[:-1]
+        self.total_elems = int(sizes.sum())
+        self._plan = ctypes.c_void_p()
+        _native.check(lib.pcvl_perm_plan_create(
+            self.orders.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            self.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            len(self.orders), int(max_chunk_log2), ctypes.byref(self._plan)), "pcvl_perm_plan_create")
+        self._lib = lib
+
+    def info(self) -> dict:
+        nitems, grid, single = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        _native.check(self._lib.pcvl_perm_plan_info(self._plan, ctypes.byref(nitems), ctypes.byref(grid),
+                                                    ctypes.byref(single)), "pcvl_perm_plan_info")
+        return {"items": nitems.value, "grid": grid.value, "single_order": single.value}
+
+    def __call__(self, packed, out=None, stream=None):
+        """Permanents of the packed complex128 device tensor ``packed``."""
+        import torch
+        if packed.dtype != torch.complex128 or not packed.is_cuda:
+            raise TypeError("packed matrices must be a complex128 CUDA tensor")
+        if not packed.is_contiguous() or packed.numel() < self.total_elems:
+            raise ValueError("packed tensor is not contiguous or too small for the plan")
+        if out is None:
+            out = torch.empty(len(self.orders), dtype=torch.complex128, device=packed.device)
+        elif out.dtype != torch.complex128 or out.numel() < len(self.orders) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous complex128 tensor of the batch length")
+        _native.check(self._lib.pcvl_perm_plan_execute(self._plan, _native.ptr(packed), _native.ptr(out),
+                                                       _native.stream_handle(stream)), "pcvl_perm_plan_execute")
+        return out
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            _native.check(self._lib.pcvl_perm_plan_destroy(self._plan), "pcvl_perm_plan_destroy")
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pack(mats) -> tuple[np.ndarray, np.ndarray]:
+    """Pack a list of square complex matrices: (orders int32, flat complex128)."""
+    orders = np.array([np.shape(m)[0] for m in mats], dtype=np.int32)
+    flat = np.concatenate([np.ascontiguousarray(m, dtype=np.complex128).ravel() for m in mats]) \
+        if len(mats) else np.zeros(0, dtype=np.complex128)
+    return orders, flat
+
+
+def permanents(mats, device="cuda") -> np.ndarray:
+    """Permanents of a list of square complex matrices, computed on the GPU."""
+    import torch
+    orders, flat = pack(mats)
+    if len(orders) == 0:
+        return np.zeros(0, dtype=np.complex128)
+    plan = PermanentPlan(orders)
+    try:
+        d = torch.from_numpy(flat).to(device) if flat.size else torch.zeros(1, dtype=torch.complex128,
+                                                                              device=device)
+        out = plan(d)
+        return out.cpu().numpy()
+    finally:
+        plan.close()
+
+
+def permanent(M) -> complex:
+    """Permanent of one square complex matrix, computed on the GPU."""
+    return complex(permanents([np.asarray(M, dtype=np.complex128)])[0])

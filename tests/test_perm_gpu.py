@@ -1,0 +1,115 @@
+"""GPU parity of the batched Glynn permanent kernel against the CPU oracle.
+
+Bar (north star): relative error <= 1e-10 against a complex128 CPU oracle; the
+oracle here is the long-double Glynn of oracle/pcvl_oracle.c, itself pinned in
+tests/test_oracle.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from perceval_amd.utils.workloads import haar_unitary, submatrix_batch
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-10
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def _say(msg):
+    print(f"[perm-gpu] {msg}", flush=True)
+
+
+def _gpu_perms(mats):
+    from perceval_amd.permanent import permanents
+    _say(f"GPU batch of {len(mats)} (orders {sorted(set(int(np.shape(m)[0]) for m in mats))})")
+    out = permanents(mats)
+    _say("GPU batch done")
+    return out
+
+
+def _oracle(mats):
+    _say(f"oracle batch of {len(mats)}")
+    out = oracle.perm_batch(mats, precise=True)
+    _say("oracle batch done")
+    return out
+
+
+def test_small_orders_vs_brute(gpu):
+    rng = np.random.RandomState(1)
+    mats = [rng.randn(n, n) + 1j * rng.randn(n, n) for n in range(0, 9) for _ in range(3)]
+    got = _gpu_perms(mats)
+    for M, g in zip(mats, got):
+        ref = oracle.perm_brute(M) if M.shape[0] <= 7 else oracle.perm(M)
+        assert _rel(g, ref) < 1e-12, (M.shape, g, ref)
+
+
+@pytest.mark.parametrize("n", list(range(9, 25)))
+def test_wave_orders_vs_oracle(gpu, n):
+    U = haar_unitary(64, seed=100 + n)
+    rng = np.random.RandomState(n)
+    mats = [U[np.ix_(rng.choice(64, n, replace=False), rng.choice(64, n, replace=False))] for _ in range(2)]
+    got = _gpu_perms(mats)
+    ref = _oracle(mats)
+    for g, r in zip(got, ref):
+        assert _rel(g, r) < REL_TOL, (n, g, r)
+
+
+def test_mixed_batch_config_shape(gpu):
+    # Same generator as the batched-permanent workload (n uniform in 12..24 from
+    # one 64x64 Haar unitary), 4096 matrices; every matrix of order <= 16 and a
+    # seeded sample of the larger ones are checked against the oracle.
+    _, orders, mats = submatrix_batch(4096, 12, 24, seed=7)
+    got = _gpu_perms(mats)
+    small = [i for i, n in enumerate(orders) if n <= 16]
+    rng = np.random.RandomState(3)
+    large = [i for i, n in enumerate(orders) if n > 16]
+    sample = small + sorted(rng.choice(large, size=24, replace=False).tolist())
+    ref = _oracle([mats[i] for i in sample])
+    errs = [_rel(got[i], r) for i, r in zip(sample, ref)]
+    assert max(errs) < REL_TOL, max(errs)
+
+
+def test_order20_batch_and_replay(gpu):
+    import torch
+    from perceval_amd.permanent import PermanentPlan, pack
+    _, orders, mats = submatrix_batch(512, 20, 20, seed=11)
+    o, flat = pack(mats)
+    plan = PermanentPlan(o)
+    d = torch.from_numpy(flat).cuda()
+    a = plan(d).cpu().numpy()
+    b = plan(d).cpu().numpy()
+    plan.close()
+    assert np.array_equal(a.view(np.float64), b.view(np.float64)), "replay is not bit-reproducible"
+    ref = _oracle(mats[:16])
+    for g, r in zip(a[:16], ref):
+        assert _rel(g, r) < REL_TOL
+
+
+def test_identities_order20(gpu):
+    n = 20
+    rng = np.random.RandomState(5)
+    A = haar_unitary(n, seed=9)
+    d1 = np.exp(1j * rng.uniform(0, 2 * np.pi, n)) * rng.uniform(0.5, 1.5, n)
+    d2 = np.exp(1j * rng.uniform(0, 2 * np.pi, n)) * rng.uniform(0.5, 1.5, n)
+    P = np.eye(n)[rng.permutation(n)]
+    Q = np.eye(n)[rng.permutation(n)]
+    mats = [np.eye(n, dtype=complex), np.ones((n, n), dtype=complex) / 2, A, A.T, P @ A @ Q,
+            np.diag(d1) @ A @ np.diag(d2)]
+    got = _gpu_perms(mats)
+    assert _rel(got[0], 1.0) < 1e-14
+    assert _rel(got[1], math.factorial(n) / 2 ** n) < REL_TOL   # all-ones: maximal cancellation
+    assert _rel(got[3], got[2]) < REL_TOL
+    assert _rel(got[4], got[2]) < REL_TOL
+    assert _rel(got[5], np.prod(d1) * np.prod(d2) * got[2]) < REL_TOL
+
+
+def test_rejects_order_above_max(gpu):
+    from perceval_amd.permanent import PermanentPlan
+    with pytest.raises(ValueError):
+        PermanentPlan([29])
