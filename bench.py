@@ -72,7 +72,7 @@ def cpu_baseline_perm(mats, budget_s: float = 8.0):
         k = min(len(mats), k * 2)
     return {"value": done / dt, "unit": "perms/s", "cores": threads, "kind": "port",
             "sample": f"{done} of the n=20 matrices, oracle chunked-double Glynn (oracle/pcvl_oracle.c), "
-                      f"{threads} OpenMP threads, {dt:.2f} s wall"}
+                      f"{threads} POSIX threads, {dt:.2f} s wall"}
 
 
 def main():

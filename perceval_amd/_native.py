@@ -44,6 +44,16 @@ SIGNATURES = {
     "pcvl_perm_plan_destroy": (c_int, [c_void_p]),
     "pcvl_perm_plan_info": (c_int, [c_void_p, P(c_i64), P(c_int), P(c_int)]),
     "pcvl_perm_batch": (c_int, [c_void_p, P(c_i32), P(c_i64), c_i64, c_void_p, c_void_p]),
+    "pcvl_fock_count": (c_i64, [c_int, c_int]),
+    "pcvl_fock_rank": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
+    "pcvl_fock_unrank": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
+    "pcvl_slos_plan_create": (c_int, [c_int, P(c_i32), P(c_void_p)]),
+    "pcvl_slos_plan_info": (c_int, [c_void_p, P(c_int), P(c_i64), P(c_size_t), P(c_i32)]),
+    "pcvl_slos_plan_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "pcvl_slos_plan_destroy": (c_int, [c_void_p]),
+    "pcvl_cc_workspace_bytes": (c_int, [c_int, c_i64, P(c_size_t)]),
+    "pcvl_cc_samples": (c_int, [c_void_p, c_int, P(c_i32), c_i64, c_u64, c_u64, c_void_p, c_void_p, c_size_t,
+                                c_void_p]),
 }
 
 

@@ -1,0 +1,152 @@
+"""SLOS backend (perceval/backends/_slos.py:105-223) on the GPU.
+
+Each distinct input state owns a deployed path (a SlosPlan: photon-addition
+order from _Path._decompose, :61-86, and the Fock layers).  A new circuit with
+the same number of modes keeps the deployed paths and only recomputes the
+coefficients (:130-141).  all_prob returns the final layer's probabilities in
+FSArray order (:205-214); prob_amplitude looks a state up by its rank (:187-193).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ..slos import COEFS, PROBS, SlosPlan
+from ..utils.states import BSDistribution, FockState, StateVector, fock_space
+from ._abstract_backends import AStrongSimulationBackend, _as_circuit
+
+
+def fock_rank_host(state) -> int:
+    """rank(s) = sum_{q<m-1} C(R_{q+1} + m-q-2, m-q-1), R_q = photons in modes >= q."""
+    s = list(state)
+    m = len(s)
+    R = sum(s)
+    r = 0
+    for q in range(m - 1):
+        R -= s[q]
+        r += math.comb(R + m - q - 2, m - q - 1)
+    return r
+
+
+class _Layer:
+    """Marker for one deployed Fock layer (m modes, k photons)."""
+
+    __slots__ = ("m", "k", "count")
+
+    def __init__(self, m: int, k: int):
+        self.m = m
+        self.k = k
+        self.count = math.comb(m + k - 1, k)
+
+
+class SLOSBackend(AStrongSimulationBackend):
+    def __init__(self, mask=None, use_symbolic=False):
+        super().__init__()
+        if use_symbolic:
+            raise NotImplementedError("symbolic SLOS is not supported by this simulation core")
+        self._symb = False
+        self._reset()
+        if mask is not None:
+            self.set_mask(mask)
+
+    @property
+    def name(self) -> str:
+        return "SLOS"
+
+    def _reset(self):
+        self._fsms: list = [[]]
+        self._state_mapping: dict = {}
+        self._results: dict = {}
+        self.clear_iterator_cache()
+
+    def set_circuit(self, circuit):
+        circuit = _as_circuit(circuit)
+        previous = self._circuit
+        assert not circuit.requires_polarization, "Circuit must not contain polarized components"
+        self._input_state = None
+        self._circuit = circuit
+        self._umat = circuit.compute_unitary()
+        self._results = {}
+        if not (self._state_mapping and previous is not None and previous.m == circuit.m):
+            self._reset()
+
+    def set_input_state(self, input_state):
+        super().set_input_state(input_state)
+        self.preprocess([self._input_state])
+
+    def clear_mask(self):
+        super().clear_mask()
+        self._reset()
+
+    def _deploy(self, input_list):
+        m = self._circuit.m
+        for input_state in input_list:
+            for k in range(len(self._fsms), input_state.n + 1):
+                self._fsms.append(_Layer(m, k))
+
+    def preprocess(self, input_list) -> bool:
+        input_list = [FockState(s) for s in input_list]
+        new = [s for s in input_list if s not in self._state_mapping]
+        if not new:
+            return False
+        self._deploy(new)
+        for s in new:
+            self._state_mapping[s] = SlosPlan(list(s))
+        return True
+
+    # ------------------------------------------------------------ results
+    def _compute(self, istate, mode):
+        key = (istate, mode)
+        if key not in self._results:
+            plan = self._state_mapping[istate]
+            self._results[key] = plan.run(self._umat, mode).cpu().numpy()
+        return self._results[key]
+
+    def _masked_indices(self, istate):
+        rows = fock_space(istate.m, istate.n)
+        if self._mask is None:
+            return None
+        return np.array([i for i, r in enumerate(rows) if self._mask.match(FockState(r))], dtype=np.int64)
+
+    def prob_amplitude(self, output_state) -> complex:
+        output_state = FockState(output_state)
+        if self._input_state.n != output_state.n:
+            return complex(0)
+        idx = fock_rank_host(output_state)
+        coefs = self._compute(self._input_state, COEFS)
+        assert 0 <= idx < len(coefs)
+        return complex(coefs[idx] * math.sqrt(output_state.prodnfact() / self._input_state.prodnfact()))
+
+    def all_prob(self, input_state=None):
+        """SLOS specific signature, to enhance optimization in some computations"""
+        if input_state is not None:
+            self.set_input_state(input_state)
+        probs = self._compute(self._input_state, PROBS)
+        sel = self._masked_indices(self._input_state)
+        return (probs if sel is None else probs[sel]).tolist()
+
+    def all_prob_tensor(self, input_state=None):
+        """Probabilities of every output state as a CUDA float64 tensor (no host copy)."""
+        if input_state is not None:
+            self.set_input_state(input_state)
+        return self._state_mapping[self._input_state].run(self._umat, PROBS)
+
+    def prob_distribution(self) -> BSDistribution:
+        bsd = BSDistribution()
+        for s, p in zip(self._get_iterator(self._input_state), self.all_prob()):
+            bsd.add(s, p)
+        return bsd
+
+    def evolve(self) -> StateVector:
+        istate = self._input_state
+        coefs = self._compute(istate, COEFS)
+        rows = fock_space(istate.m, istate.n)
+        res = StateVector()
+        pin = istate.prodnfact()
+        for r, c in zip(rows, coefs):
+            s = FockState(r)
+            if self._mask is not None and not self._mask.match(s):
+                continue
+            res += s * complex(c * math.sqrt(s.prodnfact() / pin))
+        return res

@@ -1,0 +1,349 @@
+// Exact boson sampling: algorithm A of Clifford & Clifford (2017), batched
+// over samples.  Replaces xq.Clifford2017.samples
+// (perceval/backends/_clifford2017.py:52-57; docs/source/backends.rst:108-116).
+//
+// For an input with photons in modes cols[0..n-1] let A = U[:, cols] (m x n).
+// A sample permutes the columns of A uniformly, then for k = 1..n draws the
+// output mode r_k with probability proportional to
+//     w_i = | sum_{l<k} A[i, perm[l]] * perm(B^{(-l)}) |^2,
+//     B = A[r_1..r_{k-1}, perm[0..k-1]]  ((k-1) x k),
+// where B^{(-l)} drops column l.  All k sub-permanents come from one Glynn
+// pass over the rows of B: per Gray-code term, the k column sums c_l are
+// shared and prod_{l' != l} c_l' is the product of a prefix and a suffix.
+//
+// Mapping: one launch per photon step k, specialised on k so that each step
+// gets its own register allocation; within a launch one wavefront owns one
+// sample (persistent over samples).  The 2^{k-2} Gray-code terms are split
+// over the 32 lane pairs; partial sums of the k sub-permanents are reduced
+// across the wave, then lane i computes the weight of candidate row i and a
+// wave prefix sum draws r_k.  Per-sample state (permuted columns, drawn modes)
+// lives in a caller-provided workspace.  Random numbers
+// are the counter-based uniforms of common.h: stream = sample index, counters
+// 0..n-2 for the column permutation (Fisher-Yates, i = n-1 down to 1) and
+// n-1+k-1 for the k-th photon, so a run is fixed by its seed.
+#include "common.h"
+#include "../../include/pcvl_hip.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace pcvl {
+
+constexpr int CC_MAX_N = 24;
+constexpr int CC_MAX_M = 256;   // multiple of 64
+
+struct CCShared {
+    cd B[(CC_MAX_N - 1) * CC_MAX_N];     // B[i*k + l] = 2 * U[r_i, cols[perm[l]]]
+    cd sub[CC_MAX_N];
+    int pc[CC_MAX_N];                    // cols[perm[l]]: input mode of permuted column l
+    int r[CC_MAX_N];
+};
+
+// Partial sums of the k sub-permanents over this lane pair's chunk of
+// Gray-code terms (B holds 2*A rows; rows = k-1 >= 1).  The k columns are cut
+// into halves L = [0, h) and H = [h, k) with h = k/2; the even lane of a pair
+// owns L, the odd lane owns H, each padded to hmax = k - h columns (a padding
+// column has column sum 1 and never flips).  Per term each lane forms the
+// prefix products of its half and the half total T; the partners swap T, and
+// prod_{l' != l} c_l' = prefix(l) * suffix(l) * T_partner comes out of one
+// backward pass.  acc[j] is this lane's partial for column c0 + j.
+template <int K>
+__device__ __forceinline__ void subperm_pair(const cd *__restrict__ B, int lane, cd *acc)
+{
+    constexpr int rows = K - 1;
+    constexpr int h = K / 2;
+    constexpr int hmax = K - h;
+    constexpr int total_log2 = rows - 1;                  // 2^{rows-1} terms
+    constexpr int c = total_log2 > 5 ? total_log2 - 5 : 0; // 32 pairs
+    constexpr int active_pairs = total_log2 > 5 ? 32 : (1 << total_log2);
+    const int half = lane & 1;
+    const int pair = lane >> 1;
+    const int c0 = half ? h : 0;
+    const int ncol = half ? hmax : h;    // columns this lane really owns
+#pragma unroll
+    for (int j = 0; j < hmax; j++) acc[j] = {0.0, 0.0};
+    if (pair >= active_pairs) return;
+    const uint32_t steps = 1u << c;
+    const uint32_t base = (uint32_t)pair << c;
+    const uint32_t g0 = base ^ (base >> 1);
+
+    // For odd K the even lane's last column is padding: it reads a valid
+    // column of B (column h) but scales it by 0 and starts at 1, so the
+    // code has no lane-divergent branches.
+    const double pad = (hmax - 1 < ncol) ? 1.0 : 0.0;
+    cd cs[hmax];
+#pragma unroll
+    for (int j = 0; j < hmax; j++) {
+        const double f = (j == hmax - 1) ? pad : 1.0;
+        const cd b = B[c0 + j];
+        cs[j] = {fma(0.5 * f, b.r, 1.0 - f), 0.5 * f * b.i};
+    }
+    for (int i = 1; i < rows; i++) {
+        const double d = ((g0 >> (i - 1)) & 1) ? -0.5 : 0.5;
+#pragma unroll
+        for (int j = 0; j < hmax; j++) {
+            const double dj = (j == hmax - 1) ? d * pad : d;
+            const cd b = B[i * K + c0 + j];
+            cs[j].r = fma(dj, b.r, cs[j].r);
+            cs[j].i = fma(dj, b.i, cs[j].i);
+        }
+    }
+    for (uint32_t u = 0; u < steps; u++) {
+        const uint32_t t = base + u;
+        if (u) {
+            const int b = __builtin_ctz(t);
+            const double s = (((t >> b) ^ (t >> (b + 1))) & 1) ? -1.0 : 1.0;
+            const double sl = s * pad;
+            const cd *row = B + (b + 1) * K + c0;
+#pragma unroll
+            for (int j = 0; j < hmax; j++) {
+                const cd a = row[j];
+                const double sj = (j == hmax - 1) ? sl : s;
+                cs[j].r = fma(sj, a.r, cs[j].r);
+                cs[j].i = fma(sj, a.i, cs[j].i);
+            }
+        }
+        cd P[hmax];
+        P[0] = {1.0, 0.0};
+        if constexpr (hmax > 1) P[1] = cs[0];
+#pragma unroll
+        for (int j = 2; j < hmax; j++) P[j] = cmul(P[j - 1], cs[j - 1]);
+        const cd T = (hmax == 1) ? cs[0] : cmul(P[hmax - 1], cs[hmax - 1]);
+        cd Tp;
+        Tp.r = __shfl_xor(T.r, 1, 64);
+        Tp.i = __shfl_xor(T.i, 1, 64);
+        cd S = (t & 1) ? cd{-Tp.r, -Tp.i} : Tp;
+#pragma unroll
+        for (int j = hmax - 1; j >= 1; j--) {
+            cfma_acc(acc[j], P[j], S);
+            S = cmul(S, cs[j]);
+        }
+        acc[0].r += S.r;
+        acc[0].i += S.i;
+        // Keep the next column-sum update below this term's products (no
+        // overlap of two generations of cs; see perm_glynn.hip).
+#pragma unroll
+        for (int j = 0; j < hmax; j++)
+            asm volatile("" : "+v"(cs[j].r), "+v"(cs[j].i) : "v"(acc[0].r), "v"(acc[0].i));
+    }
+}
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the lanes of one parity (the 32 pairs).
+__device__ __forceinline__ double pair_sum_d(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 2; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int K>
+__device__ __forceinline__ void subperms(CCShared &sh, int lane)
+{
+    constexpr int h = K / 2;
+    constexpr int hmax = K - h;
+    cd acc[hmax];
+    subperm_pair<K>(sh.B, lane, acc);
+    constexpr int rows = K - 1;
+    const double scale = ldexp(1.0, -(rows - 1));
+#pragma unroll
+    for (int j = 0; j < hmax; j++) {
+        const double re = pair_sum_d(acc[j].r), im = pair_sum_d(acc[j].i);
+        if (lane == 0 && j < h) sh.sub[j] = {re * scale, im * scale};
+        if (lane == 1) sh.sub[h + j] = {re * scale, im * scale};
+    }
+}
+
+// Column permutation of each sample (Fisher-Yates, counters 0..n-2):
+// pc[s*n + l] = input mode of permuted column l.
+__global__ void cc_init_kernel(const int32_t *__restrict__ cols, int n, int64_t count, uint64_t seed, uint64_t first,
+                               int32_t *__restrict__ pc)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    int perm[CC_MAX_N];
+    for (int l = 0; l < n; l++) perm[l] = l;
+    uint64_t ctr = 0;
+    for (int i = n - 1; i >= 1; i--) {
+        const double u = uniform01(seed, first + (uint64_t)s, ctr++);
+        int j = (int)(u * (i + 1));
+        if (j > i) j = i;
+        const int t = perm[i]; perm[i] = perm[j]; perm[j] = t;
+    }
+    for (int l = 0; l < n; l++) pc[s * n + l] = cols[perm[l]];
+}
+
+// Step k (= K) of every sample: one wavefront per sample, persistent.
+// r[s*n + i] holds the output mode of photon i (i < k-1 already drawn).
+template <int K>
+__global__ void __launch_bounds__(64) cc_step_kernel(const cd *__restrict__ U, int m, int n, int64_t count,
+                                                     uint64_t seed, uint64_t first, const int32_t *__restrict__ pc_all,
+                                                     int32_t *__restrict__ r_all)
+{
+    __shared__ CCShared sh;
+    const int lane = threadIdx.x;
+    constexpr int CH = CC_MAX_M / 64;
+    for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+        const uint64_t stream = first + (uint64_t)s;
+        if (lane < K) sh.pc[lane] = pc_all[s * n + lane];
+        if (lane < K - 1) sh.r[lane] = r_all[s * n + lane];
+        __syncthreads();
+        if constexpr (K == 1) {
+            if (lane == 0) sh.sub[0] = {1.0, 0.0};
+        } else {
+            // B = 2 * A[r_1..r_{k-1}, perm[0..k-1]] with A = U[:, cols]
+            for (int e = lane; e < (K - 1) * K; e += 64) {
+                const int i = e / K, l = e % K;
+                const cd a = U[(size_t)sh.r[i] * m + sh.pc[l]];
+                sh.B[e] = {2.0 * a.r, 2.0 * a.i};
+            }
+            __syncthreads();
+            subperms<K>(sh, lane);
+        }
+        __syncthreads();
+        // weight of every candidate row i (lane i of each 64-row chunk), then
+        // an inclusive scan over rows in order draws r_k
+        double wv[CH];
+        double total = 0.0;
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const int i = c * 64 + lane;
+            double w = 0.0;
+            if (c * 64 < m && i < m) {
+                cd a = {0.0, 0.0};
+#pragma unroll
+                for (int l = 0; l < K; l++) cfma_acc(a, U[(size_t)i * m + sh.pc[l]], sh.sub[l]);
+                w = a.r * a.r + a.i * a.i;
+            }
+            wv[c] = w;
+            total += wave_sum_d(w);
+        }
+        const double target = uniform01(seed, stream, (uint64_t)(n - 1) + (uint64_t)(K - 1)) * total;
+        int pick = m - 1;
+        double base = 0.0;
+        bool found = false;
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            if (!found && c * 64 < m) {
+                const int i = c * 64 + lane;
+                double incl = wv[c];
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const double v = __shfl_up(incl, off, 64);
+                    if (lane >= off) incl += v;
+                }
+                const bool hit = (i < m) && (target < base + incl);
+                const unsigned long long mask = __ballot(hit);
+                if (mask) {
+                    pick = c * 64 + (int)__builtin_ctzll(mask);
+                    found = true;
+                }
+                base += __shfl(incl, 63, 64);
+            }
+        }
+        if (lane == 0) r_all[s * n + (K - 1)] = pick;
+        __syncthreads();
+    }
+}
+
+// Occupation vector of each sample from its drawn modes.
+__global__ void cc_finish_kernel(const int32_t *__restrict__ r_all, int m, int n, int64_t count,
+                                 int32_t *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count * m) return;
+    const int64_t s = e / m;
+    const int i = (int)(e % m);
+    int v = 0;
+    for (int k = 0; k < n; k++) v += (r_all[s * n + k] == i) ? 1 : 0;
+    out[e] = v;
+}
+
+template <int K>
+static int launch_step(int k, const cd *U, int m, int n, int64_t count, uint64_t seed, uint64_t first,
+                       const int32_t *pc, int32_t *r, int cus, hipStream_t st)
+{
+    if constexpr (K > CC_MAX_N) {
+        set_error("internal: photon step out of range");
+        return ERR_ARG;
+    } else {
+        if (k != K) return launch_step<K + 1>(k, U, m, n, count, seed, first, pc, r, cus, st);
+        int per_cu = 0;
+        PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&cc_step_kernel<K>), 64, 0));
+        if (per_cu < 1) per_cu = 1;
+        const int grid = (int)std::min<int64_t>(count, (int64_t)cus * per_cu);
+        hipLaunchKernelGGL((cc_step_kernel<K>), dim3(grid), dim3(64), 0, st, U, m, n, count, seed, first, pc, r);
+        PCVL_HIP_CHECK(hipGetLastError());
+        return OK;
+    }
+}
+
+} // namespace pcvl
+
+using namespace pcvl;
+
+extern "C" int pcvl_cc_workspace_bytes(int n, int64_t count, size_t *bytes)
+{
+    PCVL_ARG_CHECK(bytes != nullptr, "bytes is null");
+    PCVL_ARG_CHECK(n >= 0 && n <= CC_MAX_N && count >= 0, "n outside [0, 24] or count < 0");
+    *bytes = sizeof(int32_t) * ((size_t)2 * (size_t)count * (size_t)std::max(n, 1) + (size_t)CC_MAX_N);
+    return OK;
+}
+
+extern "C" int pcvl_cc_samples(const double *d_U, int m, const int32_t *h_in_occ, int64_t count, uint64_t seed,
+                               uint64_t first_sample, int32_t *d_out, void *d_work, size_t work_bytes,
+                               hipStream_t stream)
+{
+    PCVL_ARG_CHECK(m >= 1 && m <= CC_MAX_M, "mode count outside [1, 256]");
+    PCVL_ARG_CHECK(h_in_occ != nullptr, "input state is null");
+    PCVL_ARG_CHECK(count >= 0, "count < 0");
+    std::vector<int32_t> cols;
+    for (int i = 0; i < m; i++) {
+        PCVL_ARG_CHECK(h_in_occ[i] >= 0, "negative occupation");
+        for (int v = 0; v < h_in_occ[i]; v++) cols.push_back(i);
+    }
+    PCVL_ARG_CHECK((int)cols.size() <= CC_MAX_N, "more than 24 photons");
+    if (count == 0) return OK;
+    PCVL_ARG_CHECK(d_U && d_out, "null device buffer");
+    const int n = (int)cols.size();
+    size_t need = 0;
+    pcvl_cc_workspace_bytes(n, count, &need);
+    PCVL_ARG_CHECK(d_work != nullptr && work_bytes >= need, "workspace missing or smaller than pcvl_cc_workspace_bytes");
+    if (n == 0) {
+        const int64_t tot = count * m;
+        hipLaunchKernelGGL(cc_finish_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
+                           static_cast<const int32_t *>(d_work), m, 0, count, d_out);
+        PCVL_HIP_CHECK(hipGetLastError());
+        return OK;
+    }
+    // workspace: pc (count*n) | r (count*n) | cols (CC_MAX_N)
+    int32_t *pc = static_cast<int32_t *>(d_work);
+    int32_t *r = pc + (size_t)count * n;
+    int32_t *d_cols = r + (size_t)count * n;
+    PCVL_HIP_CHECK(hipMemcpyAsync(d_cols, cols.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
+    const cd *U = reinterpret_cast<const cd *>(d_U);
+    int dev = 0, cus = 0;
+    PCVL_HIP_CHECK(hipGetDevice(&dev));
+    PCVL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(cc_init_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, d_cols, n, count,
+                       seed, first_sample, pc);
+    PCVL_HIP_CHECK(hipGetLastError());
+    for (int k = 1; k <= n; k++) {
+        int rc = launch_step<1>(k, U, m, n, count, seed, first_sample, pc, r, cus, stream);
+        if (rc != OK) return rc;
+    }
+    const int64_t tot = count * m;
+    hipLaunchKernelGGL(cc_finish_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, r, m, n, count,
+                       d_out);
+    PCVL_HIP_CHECK(hipGetLastError());
+    // the host column list copied above must outlive the copy
+    PCVL_HIP_CHECK(hipStreamSynchronize(stream));
+    return OK;
+}

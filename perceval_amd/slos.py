@@ -1,0 +1,109 @@
+"""SLOS strong simulation and Fock-space ranking on the GPU (host side of
+pcvl_slos_* and pcvl_fock_*).
+
+A ``SlosPlan`` corresponds to one deployed SLOS path of the reference
+(_SLOSBackend._deploy/_Path, perceval/backends/_slos.py:61-102,151-185): it
+fixes the input state (hence the photon-addition order) and the Fock layers;
+``run`` recomputes the final layer for a new unitary, which is what the
+reference's set_circuit fast path does (_slos.py:130-141).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+COEFS = 0
+PROBS = 1
+
+
+class SlosPlan:
+    def __init__(self, in_occ):
+        lib = _native.load()
+        self.in_occ = np.ascontiguousarray(np.asarray(list(in_occ), dtype=np.int32))
+        self.m = int(self.in_occ.size)
+        self._plan = ctypes.c_void_p()
+        _native.check(lib.pcvl_slos_plan_create(self.m, self.in_occ.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                ctypes.byref(self._plan)), "pcvl_slos_plan_create")
+        self._lib = lib
+        n, count, work = ctypes.c_int(), ctypes.c_int64(), ctypes.c_size_t()
+        path = np.zeros(max(1, int(self.in_occ.sum())), dtype=np.int32)
+        _native.check(lib.pcvl_slos_plan_info(self._plan, ctypes.byref(n), ctypes.byref(count), ctypes.byref(work),
+                                              path.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                      "pcvl_slos_plan_info")
+        self.n = n.value
+        self.count = count.value
+        self.work_bytes = work.value
+        self.path = path[:self.n].tolist()
+        self._work = None
+
+    def _workspace(self, device):
+        import torch
+        if self._work is None or self._work.device != torch.device(device):
+            self._work = torch.empty(max(16, self.work_bytes), dtype=torch.uint8, device=device)
+        return self._work
+
+    def run(self, U, mode: int = PROBS, out=None, stream=None):
+        """Final layer for unitary U (m x m complex128, numpy or CUDA tensor):
+        complex coefficients (mode COEFS) or probabilities (mode PROBS), as a
+        CUDA tensor in FSArray order."""
+        import torch
+        if isinstance(U, np.ndarray):
+            dU = torch.from_numpy(np.ascontiguousarray(U, dtype=np.complex128)).cuda()
+        else:
+            dU = U.to(dtype=torch.complex128).contiguous()
+        if tuple(dU.shape) != (self.m, self.m):
+            raise ValueError(f"unitary must be {self.m} x {self.m}")
+        dtype = torch.complex128 if mode == COEFS else torch.float64
+        if out is None:
+            out = torch.empty(self.count, dtype=dtype, device=dU.device)
+        elif out.dtype != dtype or out.numel() < self.count or not out.is_contiguous():
+            raise ValueError("out has the wrong dtype, size or layout")
+        work = self._workspace(dU.device)
+        _native.check(self._lib.pcvl_slos_plan_execute(self._plan, _native.ptr(dU), mode, _native.ptr(out),
+                                                       _native.ptr(work), work.numel(),
+                                                       _native.stream_handle(stream)), "pcvl_slos_plan_execute")
+        return out
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            _native.check(self._lib.pcvl_slos_plan_destroy(self._plan), "pcvl_slos_plan_destroy")
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fock_count(m: int, n: int) -> int:
+    return int(_native.load(require_gpu=False).pcvl_fock_count(m, n))
+
+
+def fock_rank(occ, nmax: int | None = None) -> np.ndarray:
+    """Ranks (FSArray.find) of occupation rows, computed on the GPU."""
+    import torch
+    occ = np.ascontiguousarray(np.atleast_2d(np.asarray(occ, dtype=np.int32)))
+    count, m = occ.shape
+    nmax = int(occ.sum(axis=1).max()) if nmax is None and count else (nmax or 0)
+    lib = _native.load()
+    d_occ = torch.from_numpy(occ).cuda()
+    d_out = torch.empty(count, dtype=torch.int64, device="cuda")
+    _native.check(lib.pcvl_fock_rank(_native.ptr(d_occ), count, m, nmax, _native.ptr(d_out),
+                                     _native.stream_handle()), "pcvl_fock_rank")
+    return d_out.cpu().numpy()
+
+
+def fock_unrank(ranks, m: int, n: int) -> np.ndarray:
+    """Occupation rows of the given ranks in the n-photon, m-mode space (GPU)."""
+    import torch
+    ranks = np.ascontiguousarray(np.asarray(ranks, dtype=np.int64).ravel())
+    lib = _native.load()
+    d_r = torch.from_numpy(ranks).cuda()
+    d_out = torch.empty((len(ranks), m), dtype=torch.int32, device="cuda")
+    _native.check(lib.pcvl_fock_unrank(_native.ptr(d_r), len(ranks), m, n, _native.ptr(d_out),
+                                       _native.stream_handle()), "pcvl_fock_unrank")
+    return d_out.cpu().numpy()

@@ -1,0 +1,44 @@
+"""Fock-space masks (exqalibur FSMask semantics as exercised by
+tests/utils/test_mask.py:33-45 and AStrongSimulationBackend.set_mask,
+perceval/backends/_abstract_backends.py:103-146).
+
+A mask is a string with one character per mode: a digit fixes the photon count
+of that mode, a space or '*' accepts any count.  Several masks are OR-ed.  On
+`at_least_modes` a digit is a lower bound instead of an exact count.  With
+allow_partial=True a state may hold fewer photons than a digit requires (it
+can still be completed by photons added later).
+"""
+from __future__ import annotations
+
+
+class FSMask:
+    def __init__(self, m: int, n: int, masks, at_least_modes=None):
+        if isinstance(masks, str):
+            masks = [masks]
+        self.m = m
+        self.n = n
+        self.at_least = set(at_least_modes or [])
+        self.conditions = []
+        for mk in masks:
+            mk = mk.replace("*", " ")
+            if len(mk) != m:
+                raise ValueError("mask length must equal the number of modes")
+            self.conditions.append([None if ch == " " else int(ch) for ch in mk])
+
+    def _match_one(self, cond, state, allow_partial: bool) -> bool:
+        for i, c in enumerate(cond):
+            if c is None:
+                continue
+            v = state[i]
+            if i in self.at_least:
+                if v < c and not allow_partial:
+                    return False
+            elif allow_partial:
+                if v > c:
+                    return False
+            elif v != c:
+                return False
+        return True
+
+    def match(self, state, allow_partial: bool = False) -> bool:
+        return any(self._match_one(c, state, allow_partial) for c in self.conditions)

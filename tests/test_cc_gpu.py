@@ -1,0 +1,92 @@
+"""GPU parity of the Clifford & Clifford sampler.
+
+The GPU and the oracle draw from the same counter-based uniforms, so for the
+same seed they produce the same samples except where a uniform falls within
+rounding distance of a CDF boundary; the statistical checks then compare with
+exact distributions (small sizes) and closed-form moments (m=60, n=20)."""
+import itertools
+import math
+from collections import Counter
+
+import numpy as np
+import pytest
+
+import oracle
+from perceval_amd.utils.workloads import haar_unitary
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[cc-gpu] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("m,inp", [(8, [1, 1, 1, 1, 0, 0, 0, 0]), (6, [2, 0, 1, 0, 0, 1]), (3, [0, 0, 1])])
+def test_same_seed_same_samples(gpu, m, inp):
+    from perceval_amd.sampling import cc_samples
+    U = haar_unitary(m, seed=m)
+    g = cc_samples(U, inp, 2000, seed=99)
+    c = oracle.cc_samples(U, inp, 2000, seed=99)
+    assert (g.sum(axis=1) == sum(inp)).all()
+    same = np.mean(np.all(g == c, axis=1))
+    assert same >= 0.999, same
+
+
+def test_same_seed_same_samples_m60_n20(gpu):
+    from perceval_amd.sampling import cc_samples
+    U = haar_unitary(60, seed=60)
+    inp = [1] * 20 + [0] * 40
+    _say("GPU 48 samples")
+    g = cc_samples(U, inp, 48, seed=5, first_sample=1000)
+    _say("oracle 48 samples")
+    c = oracle.cc_samples(U, inp, 48, seed=5, first_sample=1000)
+    _say("done")
+    assert np.mean(np.all(g == c, axis=1)) >= 0.97
+
+
+@pytest.mark.parametrize("inp", [[1, 1, 0, 1, 0], [2, 1, 0, 0]])
+def test_distribution_chi_square(gpu, inp):
+    from perceval_amd.sampling import cc_samples
+    U = haar_unitary(len(inp), seed=13)
+    exact = oracle.distribution_exact(U, inp)
+    N = 50000
+    s = cc_samples(U, inp, N, seed=17)
+    cnt = Counter(map(tuple, s.tolist()))
+    chi2, dof = 0.0, -1
+    for st, p in exact.items():
+        if p * N >= 5:
+            chi2 += (cnt.get(st, 0) - p * N) ** 2 / (p * N)
+            dof += 1
+    z = ((chi2 / dof) ** (1 / 3) - (1 - 2 / (9 * dof))) / math.sqrt(2 / (9 * dof))
+    assert z < 3.7, (chi2, dof)
+
+
+def test_moments_m60_n20(gpu):
+    from perceval_amd.sampling import cc_samples
+    m = 60
+    U = haar_unitary(m, seed=600)
+    inp = [1] * 20 + [0] * 40
+    N = 4000
+    _say(f"GPU {N} samples m=60 n=20")
+    s = cc_samples(U, inp, N, seed=11).astype(float)
+    _say("done")
+    assert (s.sum(axis=1) == 20).all()
+    mean = (np.abs(U[:, :20]) ** 2).sum(axis=1)
+    se = np.sqrt(np.maximum(s.var(axis=0), 1e-4) / N)
+    assert np.all(np.abs(s.mean(axis=0) - mean) < 5 * se)
+    # pair moments on a few mode pairs
+    rng = np.random.RandomState(0)
+    for j, k in rng.choice(m, size=(8, 2), replace=False):
+        exp = sum(abs(U[j, i] * U[k, i2] + U[j, i2] * U[k, i]) ** 2 for i, i2 in itertools.combinations(range(20), 2))
+        x = s[:, j] * s[:, k]
+        assert abs(x.mean() - exp) < 5 * max(x.std(), 1e-3) / math.sqrt(N)
+
+
+def test_deterministic_replay(gpu):
+    from perceval_amd.sampling import cc_samples
+    U = haar_unitary(10, seed=1)
+    a = cc_samples(U, [1] * 5 + [0] * 5, 500, seed=3)
+    b = cc_samples(U, [1] * 5 + [0] * 5, 500, seed=3)
+    c = cc_samples(U, [1] * 5 + [0] * 5, 500, seed=4)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, c)

@@ -1,0 +1,146 @@
+"""Host-side logic that needs no GPU: Fock types, circuit unitaries, masks,
+the backend registry and error conventions, and the C-ABI library surface."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import perceval_amd as pcvl
+from perceval_amd import (BS, BSDistribution, BackendFactory, BasicState, Circuit, FSMask, PERM, PS, StateVector,
+                          Unitary, clements_mesh)
+from perceval_amd.utils.states import fock_space
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_fockstate_basics():
+    s = BasicState("|2,0,1>")
+    assert s.m == 3 and s.n == 3 and list(s) == [2, 0, 1] and str(s) == "|2,0,1>"
+    assert s.prodnfact() == 2
+    assert BasicState([2, 0, 1]) == s and hash(BasicState([2, 0, 1])) == hash(s)
+    assert (BasicState([1]) * BasicState([0, 2])) == BasicState([1, 0, 2])
+    with pytest.raises(ValueError):
+        BasicState([1, -1])
+
+
+def test_statevector_and_distribution():
+    sv = 0.5 * StateVector([1, 0]) + 0.5j * StateVector([0, 1])
+    assert sv[BasicState([1, 0])] == 0.5 and sv[[0, 1]] == 0.5j
+    sv2 = StateVector(sv).normalize()
+    assert sv2.norm() == pytest.approx(1.0)
+    d = BSDistribution()
+    d.add(BasicState([1, 0]), 0.25)
+    d.add([1, 0], 0.25)
+    assert d[BasicState([1, 0])] == 0.5 and d[[0, 1]] == 0.0
+
+
+def test_fock_space_order_pinned():
+    # tests/utils/test_statevector.py:430-438
+    got = [list(r) for n in range(3) for r in fock_space(3, n).tolist()]
+    assert got == [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [2, 0, 0], [1, 1, 0], [1, 0, 1], [0, 2, 0],
+                   [0, 1, 1], [0, 0, 2]]
+    assert len(list(pcvl.allstate_iterator(BasicState([1, 0, 0, 0])))) == 4
+
+
+def test_mask_semantics():
+    # tests/utils/test_mask.py:33-45
+    mask = FSMask(6, 4, ["    11"])
+    assert mask.match(BasicState("|0,0,1,1,1,1>"))
+    assert not mask.match(BasicState("|0,0,1,1,1,0>"), False)
+    assert mask.match(BasicState("|0,0,1,1,1,0>"), True)
+    mask = FSMask(6, 4, ["   011", "   110"])
+    assert mask.match(BasicState("|0,0,1,0,1,1>"))
+    assert mask.match(BasicState("|0,0,1,1,1,0>"))
+    assert not mask.match(BasicState("|0,0,1,1,1,1>"))
+
+
+def test_bs_conventions_match_reference_formulas():
+    th, tl, bl, tr, br = 1.1, 0.3, -0.4, 0.7, 0.2
+    c, s = math.cos(th / 2), math.sin(th / 2)
+    U = BS(th, tl, bl, tr, br).compute_unitary()
+    exp = np.array([[np.exp(1j * (tl + tr)) * c, 1j * np.exp(1j * (tr + bl)) * s],
+                    [1j * np.exp(1j * (tl + br)) * s, np.exp(1j * (bl + br)) * c]])
+    assert np.allclose(U, exp)
+    assert np.allclose(BS.H().compute_unitary(), np.array([[1, 1], [1, -1]]) / math.sqrt(2))
+    assert np.allclose(BS.Ry().compute_unitary(), np.array([[1, -1], [1, 1]]) / math.sqrt(2))
+    assert np.allclose(PS(0.5).compute_unitary(), [[np.exp(0.5j)]])
+    assert np.allclose(PERM([2, 0, 1]).compute_unitary()[:, 0], [0, 0, 1])
+
+
+def test_circuit_composition_left_multiplies():
+    a, b = BS.H(), BS(0.3)
+    c = Circuit(2) // a // b
+    assert np.allclose(c.compute_unitary(), b.compute_unitary() @ a.compute_unitary())
+    c3 = Circuit(3) // (1, PS(0.2)) // (1, BS.H())
+    E1 = np.eye(3, dtype=complex)
+    E1[1, 1] = np.exp(0.2j)
+    E2 = np.eye(3, dtype=complex)
+    E2[1:, 1:] = BS.H().compute_unitary()
+    assert np.allclose(c3.compute_unitary(), E2 @ E1)
+
+
+def test_clements_mesh_shape_and_unitarity():
+    mesh = clements_mesh(12)
+    params = mesh.get_parameters()
+    assert len(params) == 132
+    rng = np.random.RandomState(0)
+    for p in params:
+        p.set_value(rng.uniform(0, 2 * math.pi))
+    U = mesh.compute_unitary()
+    assert np.allclose(U @ U.conj().T, np.eye(12), atol=1e-12)
+
+
+def test_backend_registry_names():
+    assert sorted(BackendFactory.list()) == ["CliffordClifford2017", "Naive", "SLOS"]
+    with pytest.warns(UserWarning):
+        b = BackendFactory.get_backend("NoSuchBackend")
+    assert b.name == "SLOS"
+
+
+@pytest.mark.parametrize("backend_name", ["SLOS", "Naive", "CliffordClifford2017"])
+def test_backend_wrong_size(backend_name):
+    # tests/backends/test_backends.py:106-113
+    backend = BackendFactory.get_backend(backend_name)
+    with pytest.raises(AssertionError):
+        backend.set_circuit(Circuit(2))
+        backend.set_input_state(BasicState([1, 1, 1]))
+
+
+@pytest.mark.parametrize("backend_name", ["SLOS", "Naive", "CliffordClifford2017"])
+def test_input_before_circuit_asserts(backend_name):
+    with pytest.raises(AssertionError):
+        BackendFactory.get_backend(backend_name).set_input_state(BasicState([1, 0]))
+
+
+def _declared_symbols():
+    with open(os.path.join(ROOT, "include", "pcvl_hip.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(pcvl_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from perceval_amd import _native
+    lib = _native.load(require_gpu=False)
+    declared = _declared_symbols()
+    assert len(declared) >= 10
+    for name in declared:
+        assert hasattr(lib, name), name
+        assert name in _native.SIGNATURES, f"{name} has no ctypes signature"
+    assert lib.pcvl_abi_version() >= 100
+
+
+def test_no_gpu_means_loud_failure():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from perceval_amd import _native
+    from perceval_amd.permanent import permanents
+    with pytest.raises(_native.NativeUnavailable):
+        permanents([np.eye(3)])
+    b = BackendFactory.get_backend("SLOS")
+    b.set_circuit(BS.H())
+    with pytest.raises(_native.NativeUnavailable):
+        b.set_input_state(BasicState([1, 1]))

@@ -1,0 +1,128 @@
+"""GPU parity of the SLOS kernels and the Fock ranking against the oracle.
+
+Bars: Fock ranks/states bit-exact; SLOS probabilities within 1e-12 absolute of
+the CPU oracle; at the full m=24, n=10 size (92,561,040 outputs), the sum rule
+and permanents of sampled outputs."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from perceval_amd.utils.workloads import haar_unitary
+
+pytestmark = pytest.mark.gpu
+
+ABS_TOL = 1e-12
+
+
+def _say(msg):
+    print(f"[slos-gpu] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("m,n", [(1, 0), (1, 3), (2, 5), (3, 2), (5, 4), (8, 4), (12, 6), (24, 3)])
+def test_fock_rank_unrank_exact(gpu, m, n):
+    from perceval_amd.slos import fock_rank, fock_unrank
+    states = oracle.fock_enumerate(m, n)
+    ranks = fock_rank(states, nmax=n)
+    assert np.array_equal(ranks, np.arange(len(states)))
+    back = fock_unrank(np.arange(len(states)), m, n)
+    assert np.array_equal(back, states)
+
+
+def test_fock_rank_unrank_m24_n10_sampled(gpu):
+    from perceval_amd.slos import fock_rank, fock_unrank
+    total = oracle.fock_count(24, 10)
+    rng = np.random.RandomState(0)
+    ranks = np.concatenate([rng.randint(0, total, 4096), [0, 1, total - 2, total - 1]]).astype(np.int64)
+    states = fock_unrank(ranks, 24, 10)
+    ref = np.array([oracle.fock_unrank(int(r), 24, 10) for r in ranks], dtype=np.int32)
+    assert np.array_equal(states, ref)
+    assert np.array_equal(fock_rank(states, nmax=10), ranks)
+    # invalid inputs are flagged, not wrapped
+    bad = fock_unrank(np.array([-1, total], dtype=np.int64), 24, 10)
+    assert (bad == -1).all()
+
+
+@pytest.mark.parametrize("inp", [[1, 1, 1, 1, 0, 0, 0, 0], [2, 0, 1, 0, 0, 1, 0, 0], [0, 0, 0, 0, 0, 0, 0, 4]])
+def test_slos_config1_vs_oracle(gpu, inp):
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan
+    U = haar_unitary(8, seed=42)
+    plan = SlosPlan(inp)
+    assert plan.path == oracle.slos_path(inp)
+    probs = plan.run(U, PROBS).cpu().numpy()
+    coefs = plan.run(U, COEFS).cpu().numpy()
+    ref_c = oracle.slos_coefs(U, inp)
+    ref_p = oracle.slos_probs(U, inp)
+    assert len(probs) == 330
+    assert np.max(np.abs(probs - ref_p)) < ABS_TOL
+    assert np.max(np.abs(coefs - ref_c)) < ABS_TOL
+    plan.close()
+
+
+def test_slos_reference_unitary_u_random_8(gpu):
+    import os
+    from perceval_amd.slos import PROBS, SlosPlan
+    U = np.load(os.path.join(os.path.dirname(__file__), "golden", "u_random_8.npy"))
+    inp = [1, 1, 1, 1, 0, 0, 0, 0]
+    p = SlosPlan(inp).run(U, PROBS).cpu().numpy()
+    exact = np.array(list(oracle.distribution_exact(U, inp).values()))
+    assert np.max(np.abs(p - exact)) < ABS_TOL
+
+
+def test_slos_m12_n6_full_vs_oracle(gpu):
+    from perceval_amd.slos import PROBS, SlosPlan
+    U = haar_unitary(12, seed=7)
+    inp = [1] * 6 + [0] * 6
+    p = SlosPlan(inp).run(U, PROBS).cpu().numpy()
+    ref = oracle.slos_probs(U, inp, "gather")
+    assert len(p) == 12376
+    assert np.max(np.abs(p - ref)) < ABS_TOL
+
+
+def test_slos_m24_n10_full_size(gpu):
+    import torch
+    from perceval_amd.slos import PROBS, SlosPlan
+    U = haar_unitary(24, seed=24)
+    inp = [1] * 10 + [0] * 14
+    plan = SlosPlan(inp)
+    _say(f"plan: {plan.count} outputs, workspace {plan.work_bytes / 2**20:.0f} MiB")
+    p = plan.run(U, PROBS)
+    torch.cuda.synchronize()
+    _say("all_prob done")
+    assert p.numel() == 92561040
+    total = float(p.sum().item())
+    assert abs(total - 1.0) < 1e-10
+    assert float(p.min().item()) >= 0.0
+    rng = np.random.RandomState(1)
+    idx = np.concatenate([[0, 1, p.numel() - 1], rng.randint(0, p.numel(), 61)])
+    got = p[torch.from_numpy(idx).cuda()].cpu().numpy()
+    for i, g in zip(idx, got):
+        s = oracle.fock_unrank(int(i), 24, 10)
+        M = oracle.submatrix(U, inp, s)
+        ref = abs(oracle.perm(M)) ** 2 / oracle.prodnfact(s)
+        assert abs(g - ref) < ABS_TOL, (i, s, g, ref)
+    # the largest output probabilities agree with permanents too
+    top = torch.topk(p, 8).indices.cpu().numpy()
+    for i in top:
+        s = oracle.fock_unrank(int(i), 24, 10)
+        ref = abs(oracle.perm(oracle.submatrix(U, inp, s))) ** 2 / oracle.prodnfact(s)
+        assert abs(float(p[int(i)].item()) - ref) < ABS_TOL
+    plan.close()
+
+
+def test_slos_vacuum_and_one_photon(gpu):
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan
+    U = haar_unitary(5, seed=3)
+    p0 = SlosPlan([0] * 5).run(U, PROBS).cpu().numpy()
+    assert p0.tolist() == [1.0]
+    c1 = SlosPlan([0, 0, 1, 0, 0]).run(U, COEFS).cpu().numpy()
+    assert np.allclose(c1, U[:, 2], atol=0, rtol=0)
+
+
+def test_slos_rejects_bad_inputs(gpu):
+    from perceval_amd.slos import SlosPlan
+    with pytest.raises(ValueError):
+        SlosPlan([-1, 2])
+    with pytest.raises(ValueError):
+        SlosPlan([1] * 65)
