@@ -136,6 +136,104 @@ int pcvl_cc_samples(const double *d_U, int m, const int32_t *h_in_occ, int64_t c
                     uint64_t first_sample, int32_t *d_out, void *d_work, size_t work_bytes,
                     hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Phase gradients -- no counterpart in the reference, which differentiates
+ * numerically by repeated SLOSBackend.set_circuit/all_prob calls
+ * (benchmark/benchmark_QML-DE-solver.py:140-142; perceval/backends/_slos.py:130-141).
+ * For each of nsets phase vectors (d_phases: nsets x nparams) of one mesh:
+ *   loss = sum over targets of P(s), P(s) = |perm(U[s, in])|^2 / (prod s! prod t!),
+ *   grad = d loss / d phases  (nsets x nparams),
+ *   probs = P(s) per target (nsets x ntargets; d_probs may be NULL).
+ * The mesh program lists nops components in circuit order (U = L_K ... L_1):
+ *   h_kind[p] = 0: phase shifter on mode h_mode[p], phase = phases[h_param[p]],
+ *                  or h_const[8p] when h_param[p] = -1;
+ *   h_kind[p] = 1: fixed 2x2 unitary on modes (h_mode[p], h_mode[p]+1),
+ *                  h_const[8p..8p+7] row-major interleaved complex.
+ * h_targets: ntargets rows of m int32.  m <= 32, 1..8 photons, <= 64 targets.
+ * Synchronises `stream`.
+ * ------------------------------------------------------------------------- */
+int pcvl_mesh_adjoint(const int32_t *h_kind, const int32_t *h_mode, const int32_t *h_param,
+                      const double *h_const, int nops, int m, const double *d_phases, int nparams,
+                      int nsets, const int32_t *h_in_occ, const int32_t *h_targets, int ntargets,
+                      double *d_loss, double *d_grad, double *d_probs, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned Metropolis independence sampler -- no counterpart in the reference
+ * (whose sampler is xq.Clifford2017, perceval/backends/_clifford2017.py:52-57).
+ * A proposal network (bf16 weights, fp32 arithmetic) proposes output states
+ * as non-decreasing mode sequences; FP64 permanents give the exact target
+ * probabilities, so the chain's stationary law is the boson-sampling law.
+ * All buffers are caller-owned device memory; matrices are row-major.
+ * ------------------------------------------------------------------------- */
+
+/* Y[i][j] = relu?(sum_k A(i,k) B(k,j) + bias[j]) (+ Y[i][j] when accumulate),
+ * A(i,k) = d_A[i*sai + k*sak], B(k,j) = d_B[k*sbk + j*sbj]; B is bf16 when
+ * b_is_bf16, else fp32.  d_bias may be NULL. */
+int pcvl_mlp_gemm(const float *d_A, int64_t sai, int64_t sak, const void *d_B, int b_is_bf16,
+                  int64_t sbk, int64_t sbj, float *d_Y, int64_t ldy, int M, int N, int K,
+                  const float *d_bias, int relu, int accumulate, hipStream_t stream);
+
+/* dH[e] = 0 where H[e] <= 0 (ReLU backward, in place). */
+int pcvl_mlp_relu_bwd(float *d_dH, const float *d_H, int64_t count, hipStream_t stream);
+
+/* d_db[j] = sum_i dY[i][j] for an M x N matrix. */
+int pcvl_mlp_colsum(const float *d_dY, int M, int N, float *d_db, hipStream_t stream);
+
+/* Masked softmax cross-entropy per row (modes below d_lo[i] masked):
+ * loss[i] = -log q(target[i]); d_dlogits (may be NULL) = (q - onehot) * scale. */
+int pcvl_mlp_ce(const float *d_logits, int M, int m, const int32_t *d_target, const int32_t *d_lo,
+                float scale, float *d_loss, float *d_dlogits, hipStream_t stream);
+
+/* One Adam step on fp32 master weights (step counts from 1); refreshes the
+ * bf16 copy when d_wbf16 is not NULL. */
+int pcvl_mlp_adam(float *d_w, const float *d_g, float *d_m1, float *d_m2, uint16_t *d_wbf16,
+                  int64_t count, float lr, float beta1, float beta2, float eps, int step,
+                  hipStream_t stream);
+
+/* bf16 (round to nearest even) copy of fp32 weights. */
+int pcvl_mlp_to_bf16(const float *d_w, uint16_t *d_wbf16, int64_t count, hipStream_t stream);
+
+/* Inputs of step p for B sequences (B x n int32): counts of the first p modes
+ * and a one-hot of p (d_x: B x (m+n) fp32); d_lo[b] = seq[b][p-1] (0 at p=0). */
+int pcvl_mis_features(const int32_t *d_seq, int B, int n, int m, int p, float *d_x, int32_t *d_lo,
+                      hipStream_t stream);
+
+/* Teacher-forcing inputs of every (sequence, step) pair, row b*n+p, with
+ * d_lo and the target mode seq[b][p]. */
+int pcvl_mis_features_all(const int32_t *d_seq, int B, int n, int m, float *d_x, int32_t *d_lo,
+                          int32_t *d_target, hipStream_t stream);
+
+/* Draws step p of every sequence from its masked softmax (uniform stream
+ * (seed, first + b), counter p) and accumulates log q in d_logq. */
+int pcvl_mis_sample_step(const float *d_logits, int B, int m, int n, int p, const int32_t *d_lo,
+                         uint64_t seed, uint64_t first, int32_t *d_seq, double *d_logq,
+                         hipStream_t stream);
+
+/* n x n submatrices U[seq[b], cols] of B proposals (complex, packed). */
+int pcvl_mis_submatrices(const double *d_U, int m, const int32_t *d_cols, int n, const int32_t *d_seq,
+                         int64_t B, double *d_out, hipStream_t stream);
+
+/* log p of B proposals from their permanents (complex) and sequences. */
+int pcvl_mis_logp(const double *d_perms, const int32_t *d_seq, int n, int64_t B, double log_prod_in,
+                  double *d_logp, hipStream_t stream);
+
+/* Accept/reject scan over B proposals with log weights log p - log q,
+ * starting from a state of log weight logw0; d_state[t] = proposal index held
+ * after step t (-1: the initial state); *d_accepted = accepted moves. */
+int pcvl_mis_chain(const double *d_logw, int64_t B, double logw0, uint64_t seed, uint64_t first,
+                   int64_t *d_state, int64_t *d_accepted, hipStream_t stream);
+
+/* Occupation counts (B x m int32) of sorted mode sequences (B x n int32). */
+int pcvl_mis_seq_to_occ(const int32_t *d_seq, int64_t B, int n, int m, int32_t *d_occ, hipStream_t stream);
+
+/* Sorted mode sequences (B x n) of occupation vectors (B x m); a row whose
+ * photon number differs from n becomes -1s. */
+int pcvl_mis_occ_to_seq(const int32_t *d_occ, int64_t B, int m, int n, int32_t *d_seq, hipStream_t stream);
+
+/* Chain output: d_out[t] = proposal d_state[t] of d_prop, or d_x0 for -1. */
+int pcvl_mis_gather(const int32_t *d_prop, const int64_t *d_state, int64_t B, int n, const int32_t *d_x0,
+                    int32_t *d_out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,27 @@ SIGNATURES = {
     "pcvl_slos_plan_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pcvl_slos_plan_destroy": (c_int, [c_void_p]),
     "pcvl_cc_workspace_bytes": (c_int, [c_int, c_i64, P(c_size_t)]),
+    "pcvl_mlp_gemm": (c_int, [c_void_p, c_i64, c_i64, c_void_p, c_int, c_i64, c_i64, c_void_p, c_i64, c_int, c_int,
+                              c_int, c_void_p, c_int, c_int, c_void_p]),
+    "pcvl_mlp_relu_bwd": (c_int, [c_void_p, c_void_p, c_i64, c_void_p]),
+    "pcvl_mlp_colsum": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "pcvl_mlp_ce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, ctypes.c_float, c_void_p, c_void_p,
+                            c_void_p]),
+    "pcvl_mlp_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_float, ctypes.c_float, c_int, c_void_p]),
+    "pcvl_mlp_to_bf16": (c_int, [c_void_p, c_void_p, c_i64, c_void_p]),
+    "pcvl_mis_features": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "pcvl_mis_features_all": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pcvl_mis_sample_step": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_u64, c_u64, c_void_p,
+                                     c_void_p, c_void_p]),
+    "pcvl_mis_submatrices": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_i64, c_void_p, c_void_p]),
+    "pcvl_mis_logp": (c_int, [c_void_p, c_void_p, c_int, c_i64, c_double, c_void_p, c_void_p]),
+    "pcvl_mis_chain": (c_int, [c_void_p, c_i64, c_double, c_u64, c_u64, c_void_p, c_void_p, c_void_p]),
+    "pcvl_mis_seq_to_occ": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
+    "pcvl_mis_occ_to_seq": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
+    "pcvl_mis_gather": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p]),
+    "pcvl_mesh_adjoint": (c_int, [P(c_i32), P(c_i32), P(c_i32), P(c_double), c_int, c_int, c_void_p, c_int, c_int,
+                                  P(c_i32), P(c_i32), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pcvl_cc_samples": (c_int, [c_void_p, c_int, P(c_i32), c_i64, c_u64, c_u64, c_void_p, c_void_p, c_size_t,
                                 c_void_p]),
 }

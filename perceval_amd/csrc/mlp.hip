@@ -1,0 +1,314 @@
+// Autoregressive proposal network for the Metropolis independence sampler:
+// forward, backward and optimiser kernels (bf16 weights, fp32 arithmetic).
+//
+// No counterpart in the reference.  The model proposes an output state of n
+// photons over m modes as a non-decreasing sequence of modes s_0 <= ... <=
+// s_{n-1}; step p sees the counts of the modes drawn so far and a one-hot of
+// p (D = m + n inputs), and emits logits over the m modes, masked below
+// s_{p-1}.  The sequence is unique per Fock state, so q(state) is exact.
+//   h = relu(x W1^T + b1)  (hidden H),  logits = h W2^T + b2.
+// Matrices are row-major; W1 is H x D and W2 is m x H, stored as bf16 for the
+// products with fp32 master copies for the optimiser.
+#include "common.h"
+#include "../../include/pcvl_hip.h"
+
+namespace pcvl {
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t b)
+{
+    return __uint_as_float(((uint32_t)b) << 16);
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+template <typename T> __device__ __forceinline__ float ld(const T *p, size_t i);
+template <> __device__ __forceinline__ float ld<float>(const float *p, size_t i) { return p[i]; }
+template <> __device__ __forceinline__ float ld<uint16_t>(const uint16_t *p, size_t i) { return bf16_to_f32(p[i]); }
+
+// Y[i][j] = act(alpha * sum_k A(i,k) B(k,j) + bias[j]) (+ Y[i][j] if accumulate)
+// with A(i,k) = A[i*sai + k*sak], B(k,j) = B[k*sbk + j*sbj].  64 x 64 output
+// tile per 256-thread block, 16-deep K slices staged through LDS, 4 x 4
+// outputs per thread.
+constexpr int GT = 64, GK = 16;
+
+template <typename TB>
+__global__ void __launch_bounds__(256) gemm_kernel(const float *__restrict__ A, int64_t sai, int64_t sak,
+                                                   const TB *__restrict__ B, int64_t sbk, int64_t sbj,
+                                                   float *__restrict__ Y, int64_t ldy, int M, int N, int K,
+                                                   const float *__restrict__ bias, int relu, int accumulate)
+{
+    __shared__ float As[GK][GT + 1];
+    __shared__ float Bs[GK][GT + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
+    float acc[4][4] = {};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        for (int e = threadIdx.x; e < GK * GT; e += 256) {
+            const int kk = e / GT, ii = e % GT;
+            const int i = i0 + ii, k = k0 + kk;
+            As[kk][ii] = (i < M && k < K) ? A[i * sai + k * sak] : 0.0f;
+            const int j = j0 + ii;
+            Bs[kk][ii] = (j < N && k < K) ? ld<TB>(B, (size_t)(k * sbk + j * sbj)) : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; kk++) {
+            float a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) a[r] = As[kk][ty + 16 * r];
+#pragma unroll
+            for (int c = 0; c < 4; c++) b[c] = Bs[kk][tx + 16 * c];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[r][c] = fmaf(a[r], b[c], acc[r][c]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = i0 + ty + 16 * r;
+        if (i >= M) continue;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int j = j0 + tx + 16 * c;
+            if (j >= N) continue;
+            float v = acc[r][c];
+            if (bias) v += bias[j];
+            if (accumulate) v += Y[(size_t)i * ldy + j];
+            if (relu) v = v > 0.0f ? v : 0.0f;
+            Y[(size_t)i * ldy + j] = v;
+        }
+    }
+}
+
+// dH *= (H > 0)
+__global__ void relu_bwd_kernel(float *__restrict__ dH, const float *__restrict__ H, int64_t count)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count && !(H[e] > 0.0f)) dH[e] = 0.0f;
+}
+
+// db[j] = sum_i dY[i][j]  (one block per column chunk, fixed order)
+__global__ void colsum_kernel(const float *__restrict__ dY, int M, int N, float *__restrict__ db)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    float s = 0.0f;
+    for (int i = 0; i < M; i++) s += dY[(size_t)i * N + j];
+    db[j] = s;
+}
+
+// Masked softmax cross-entropy of row i with target t[i], modes < lo[i]
+// masked: loss[i] = -log q(t | row); dlogits = (softmax - onehot) * scale.
+__global__ void ce_kernel(const float *__restrict__ logits, int M, int m, const int32_t *__restrict__ target,
+                          const int32_t *__restrict__ lo, float scale, float *__restrict__ loss,
+                          float *__restrict__ dlogits)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const float *z = logits + (size_t)i * m;
+    const int l0 = lo[i];
+    float mx = -3.0e38f;
+    for (int j = l0; j < m; j++) mx = fmaxf(mx, z[j]);
+    float se = 0.0f;
+    for (int j = l0; j < m; j++) se += __expf(z[j] - mx);
+    const float lse = mx + __logf(se);
+    loss[i] = lse - z[target[i]];
+    if (dlogits) {
+        float *d = dlogits + (size_t)i * m;
+        for (int j = 0; j < m; j++) {
+            const float p = (j >= l0) ? __expf(z[j] - lse) : 0.0f;
+            d[j] = (p - (j == target[i] ? 1.0f : 0.0f)) * scale;
+        }
+    }
+}
+
+// Adam on fp32 master weights; writes the bf16 copy used by the products.
+__global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m1,
+                            float *__restrict__ m2, uint16_t *__restrict__ wb, int64_t count, float lr, float b1,
+                            float b2, float eps, float c1, float c2)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    const float gg = g[e];
+    const float a = b1 * m1[e] + (1.0f - b1) * gg;
+    const float b = b2 * m2[e] + (1.0f - b2) * gg * gg;
+    m1[e] = a;
+    m2[e] = b;
+    const float v = w[e] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+    w[e] = v;
+    if (wb) wb[e] = f32_to_bf16(v);
+}
+
+__global__ void to_bf16_kernel(const float *__restrict__ w, uint16_t *__restrict__ wb, int64_t count)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) wb[e] = f32_to_bf16(w[e]);
+}
+
+// Features of step p for each sequence row b: x = [counts of seq[b][0..p-1] | onehot(p)],
+// and the mask bound lo[b] = seq[b][p-1] (0 at p = 0).
+__global__ void features_kernel(const int32_t *__restrict__ seq, int B, int n, int m, int p,
+                                float *__restrict__ x, int32_t *__restrict__ lo)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int D = m + n;
+    float *xb = x + (size_t)b * D;
+    for (int j = 0; j < D; j++) xb[j] = 0.0f;
+    for (int q = 0; q < p; q++) xb[seq[(size_t)b * n + q]] += 1.0f;
+    xb[m + p] = 1.0f;
+    lo[b] = p > 0 ? seq[(size_t)b * n + p - 1] : 0;
+}
+
+// Teacher-forcing features for all (row, step) pairs of a batch of sorted
+// sequences: row b*n + p.  Also writes the targets seq[b][p].
+__global__ void features_all_kernel(const int32_t *__restrict__ seq, int B, int n, int m, float *__restrict__ x,
+                                    int32_t *__restrict__ lo, int32_t *__restrict__ target)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (int64_t)B * n) return;
+    const int b = (int)(r / n), p = (int)(r % n);
+    const int D = m + n;
+    float *xr = x + (size_t)r * D;
+    for (int j = 0; j < D; j++) xr[j] = 0.0f;
+    for (int q = 0; q < p; q++) xr[seq[(size_t)b * n + q]] += 1.0f;
+    xr[m + p] = 1.0f;
+    lo[r] = p > 0 ? seq[(size_t)b * n + p - 1] : 0;
+    target[r] = seq[(size_t)b * n + p];
+}
+
+// Draw step p of every row from the masked softmax of its logits; the uniform
+// is the counter-based stream (seed, first + b, p).  Accumulates log q.
+__global__ void sample_step_kernel(const float *__restrict__ logits, int B, int m, int n, int p,
+                                   const int32_t *__restrict__ lo, uint64_t seed, uint64_t first,
+                                   int32_t *__restrict__ seq, double *__restrict__ logq)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const float *z = logits + (size_t)b * m;
+    const int l0 = lo[b];
+    float mx = -3.0e38f;
+    for (int j = l0; j < m; j++) mx = fmaxf(mx, z[j]);
+    double se = 0.0;
+    for (int j = l0; j < m; j++) se += exp((double)(z[j] - mx));
+    const double target = uniform01(seed, first + (uint64_t)b, (uint64_t)p) * se;
+    double cum = 0.0;
+    int pick = m - 1;
+    for (int j = l0; j < m; j++) {
+        cum += exp((double)(z[j] - mx));
+        if (target < cum) { pick = j; break; }
+    }
+    seq[(size_t)b * n + p] = pick;
+    const double lp = (double)(z[pick] - mx) - log(se);
+    logq[b] = (p == 0 ? 0.0 : logq[b]) + lp;
+}
+
+} // namespace pcvl
+
+using namespace pcvl;
+
+static int grid1(int64_t count) { return (int)((count + 255) / 256); }
+
+extern "C" int pcvl_mlp_gemm(const float *d_A, int64_t sai, int64_t sak, const void *d_B, int b_is_bf16,
+                             int64_t sbk, int64_t sbj, float *d_Y, int64_t ldy, int M, int N, int K,
+                             const float *d_bias, int relu, int accumulate, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(M >= 0 && N >= 0 && K >= 0, "negative GEMM size");
+    if (M == 0 || N == 0) return OK;
+    PCVL_ARG_CHECK(d_A && d_B && d_Y, "null device buffer");
+    dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+    if (b_is_bf16) {
+        hipLaunchKernelGGL(gemm_kernel<uint16_t>, grid, dim3(256), 0, stream, d_A, sai, sak,
+                           static_cast<const uint16_t *>(d_B), sbk, sbj, d_Y, ldy, M, N, K, d_bias, relu, accumulate);
+    } else {
+        hipLaunchKernelGGL(gemm_kernel<float>, grid, dim3(256), 0, stream, d_A, sai, sak,
+                           static_cast<const float *>(d_B), sbk, sbj, d_Y, ldy, M, N, K, d_bias, relu, accumulate);
+    }
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mlp_relu_bwd(float *d_dH, const float *d_H, int64_t count, hipStream_t stream)
+{
+    if (count <= 0) return OK;
+    hipLaunchKernelGGL(relu_bwd_kernel, dim3(grid1(count)), dim3(256), 0, stream, d_dH, d_H, count);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mlp_colsum(const float *d_dY, int M, int N, float *d_db, hipStream_t stream)
+{
+    if (N <= 0) return OK;
+    hipLaunchKernelGGL(colsum_kernel, dim3(grid1(N)), dim3(256), 0, stream, d_dY, M, N, d_db);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mlp_ce(const float *d_logits, int M, int m, const int32_t *d_target, const int32_t *d_lo,
+                           float scale, float *d_loss, float *d_dlogits, hipStream_t stream)
+{
+    if (M <= 0) return OK;
+    hipLaunchKernelGGL(ce_kernel, dim3(grid1(M)), dim3(256), 0, stream, d_logits, M, m, d_target, d_lo, scale, d_loss,
+                       d_dlogits);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mlp_adam(float *d_w, const float *d_g, float *d_m1, float *d_m2, uint16_t *d_wbf16, int64_t count,
+                             float lr, float beta1, float beta2, float eps, int step, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(step >= 1, "Adam step counts from 1");
+    if (count <= 0) return OK;
+    const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(grid1(count)), dim3(256), 0, stream, d_w, d_g, d_m1, d_m2, d_wbf16, count, lr,
+                       beta1, beta2, eps, c1, c2);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mlp_to_bf16(const float *d_w, uint16_t *d_wbf16, int64_t count, hipStream_t stream)
+{
+    if (count <= 0) return OK;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(grid1(count)), dim3(256), 0, stream, d_w, d_wbf16, count);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mis_features(const int32_t *d_seq, int B, int n, int m, int p, float *d_x, int32_t *d_lo,
+                                 hipStream_t stream)
+{
+    PCVL_ARG_CHECK(p >= 0 && p < n, "step outside [0, n)");
+    if (B <= 0) return OK;
+    hipLaunchKernelGGL(features_kernel, dim3(grid1(B)), dim3(256), 0, stream, d_seq, B, n, m, p, d_x, d_lo);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mis_features_all(const int32_t *d_seq, int B, int n, int m, float *d_x, int32_t *d_lo,
+                                     int32_t *d_target, hipStream_t stream)
+{
+    if (B <= 0) return OK;
+    hipLaunchKernelGGL(features_all_kernel, dim3(grid1((int64_t)B * n)), dim3(256), 0, stream, d_seq, B, n, m, d_x,
+                       d_lo, d_target);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mis_sample_step(const float *d_logits, int B, int m, int n, int p, const int32_t *d_lo,
+                                    uint64_t seed, uint64_t first, int32_t *d_seq, double *d_logq, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(p >= 0 && p < n, "step outside [0, n)");
+    if (B <= 0) return OK;
+    hipLaunchKernelGGL(sample_step_kernel, dim3(grid1(B)), dim3(256), 0, stream, d_logits, B, m, n, p, d_lo, seed,
+                       first, d_seq, d_logq);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}

@@ -1,0 +1,175 @@
+"""Learned Metropolis independence sampler: the proposal network's kernels
+against the NumPy oracle (oracle/mlp.py), exactness of q and p, the chain scan
+against a host restatement, and the sampler's output law against exact
+distributions and plain Clifford-Clifford samples."""
+import math
+from collections import Counter
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp as mlp_oracle
+from perceval_amd.utils.workloads import haar_unitary
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[mis-gpu] {msg}", flush=True)
+
+
+def _seqs_from_cc(U, inp, count, seed):
+    s = oracle.cc_samples(U, inp, count, seed=seed)
+    return np.array([[j for j, v in enumerate(row) for _ in range(v)] for row in s], dtype=np.int32)
+
+
+def test_network_gradients_vs_oracle(gpu):
+    import torch
+    from perceval_amd.mis import ProposalNet
+    m, n = 10, 4
+    U = haar_unitary(m, seed=2)
+    seqs = _seqs_from_cc(U, [1] * n + [0] * (m - n), 64, seed=1)
+    net = ProposalNet(m, n, hidden=32, seed=3)
+    # non-trivial biases and second-layer weights
+    g = torch.Generator().manual_seed(5)
+    net.b1.copy_(torch.randn(32, generator=g).float().cuda() * 0.1)
+    net.W2.copy_(torch.randn(m, 32, generator=g).float().cuda() * 0.3)
+    net.b2.copy_(torch.randn(m, generator=g).float().cuda() * 0.1)
+    net.refresh_bf16()
+    d = torch.from_numpy(seqs).cuda()
+    x, lo, tgt = net.teacher_forcing(d)
+    loss, grads = net.loss_and_grads(x, lo, tgt, 1.0 / 64)
+    X, LO, TG = mlp_oracle.features(seqs, m)
+    assert np.array_equal(x.cpu().numpy(), X) and np.array_equal(lo.cpu().numpy(), LO)
+    ref_loss, ref_grads = mlp_oracle.loss_and_grads(*(p.double().cpu().numpy() for p in net.params), X, LO, TG,
+                                                    1.0 / 64)
+    assert np.allclose(loss.cpu().numpy(), ref_loss, rtol=1e-5, atol=1e-5)
+    for gg, rg in zip(grads, ref_grads):
+        gg = gg.double().cpu().numpy()
+        assert np.max(np.abs(gg - rg)) <= 1e-5 * max(1.0, np.max(np.abs(rg))), np.max(np.abs(gg - rg))
+
+
+def test_adam_step_vs_oracle(gpu):
+    import torch
+    from perceval_amd import _native
+    lib = _native.load()
+    rng = np.random.RandomState(0)
+    w = rng.randn(1000).astype(np.float32)
+    g = rng.randn(1000).astype(np.float32)
+    dw, dg = torch.from_numpy(w).cuda(), torch.from_numpy(g).cuda()
+    m1, m2 = torch.zeros_like(dw), torch.zeros_like(dw)
+    wb = torch.empty(1000, dtype=torch.bfloat16, device="cuda")
+    rw, rm1, rm2 = w.astype(np.float64), np.zeros(1000), np.zeros(1000)
+    for step in (1, 2, 3):
+        _native.check(lib.pcvl_mlp_adam(_native.ptr(dw), _native.ptr(dg), _native.ptr(m1), _native.ptr(m2),
+                                        _native.ptr(wb), 1000, 1e-2, 0.9, 0.999, 1e-8, step,
+                                        _native.stream_handle()), "adam")
+        rw, rm1, rm2 = mlp_oracle.adam(rw, g.astype(np.float64), rm1, rm2, 1e-2, 0.9, 0.999, 1e-8, step)
+    assert np.allclose(dw.cpu().numpy(), rw, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(wb.float().cpu().numpy(), mlp_oracle.bf16_round(dw.cpu().numpy()).astype(np.float32))
+
+
+def test_sampling_gives_exact_log_q(gpu):
+    from perceval_amd.mis import ProposalNet
+    m, n = 12, 5
+    net = ProposalNet(m, n, hidden=64, seed=1)
+    seq, logq = net.sample(512, seed=9)
+    s = seq.cpu().numpy()
+    assert (np.diff(s, axis=1) >= 0).all() and s.min() >= 0 and s.max() < m
+    assert np.allclose(logq.cpu().numpy(), net.log_q(seq).cpu().numpy(), atol=2e-4)
+    # q sums to one over the whole space (m=12, n=5: 4368 states)
+    import torch
+    states = oracle.fock_enumerate(m, n)
+    seqs = np.array([[j for j, v in enumerate(r) for _ in range(v)] for r in states], dtype=np.int32)
+    tot = torch.exp(net.log_q(torch.from_numpy(seqs).cuda())).sum().item()
+    assert abs(tot - 1.0) < 1e-4
+
+
+def test_log_p_vs_oracle(gpu):
+    import torch
+    from perceval_amd.mis import LearnedMIS, ProposalNet
+    m, n = 12, 5
+    U = haar_unitary(m, seed=4)
+    inp = [1] * n + [0] * (m - n)
+    net = ProposalNet(m, n, hidden=16)
+    mis = LearnedMIS(U, inp, net, chunk=256)
+    states = oracle.fock_enumerate(m, n)[::17]
+    seqs = np.array([[j for j, v in enumerate(r) for _ in range(v)] for r in states], dtype=np.int32)
+    lp = mis.log_p(torch.from_numpy(seqs).cuda()).cpu().numpy()
+    for s, v in zip(states, lp):
+        ref = abs(oracle.perm(oracle.submatrix(U, inp, s))) ** 2 / oracle.prodnfact(s)
+        assert abs(math.exp(v) - ref) <= 1e-12 + 1e-10 * ref
+    mis.close()
+
+
+def test_chain_scan_vs_host_restatement(gpu):
+    import torch
+    from perceval_amd import _native
+    lib = _native.load()
+    rng = np.random.RandomState(3)
+    logw = rng.randn(5000) * 2
+    d = torch.from_numpy(logw).cuda()
+    state = torch.empty(5000, dtype=torch.int64, device="cuda")
+    acc = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _native.check(lib.pcvl_mis_chain(_native.ptr(d), 5000, 0.3, 77, 10, _native.ptr(state), _native.ptr(acc),
+                                     _native.stream_handle()), "chain")
+    cur, idx, ref, nacc = 0.3, -1, [], 0
+    for t in range(5000):
+        dlt = logw[t] - cur
+        if dlt >= 0 or math.log(oracle.uniform(77, 10 + t, 0)) < dlt:
+            cur, idx, nacc = logw[t], t, nacc + 1
+        ref.append(idx)
+    assert np.array_equal(state.cpu().numpy(), np.array(ref)) and acc.item() == nacc
+
+
+@pytest.mark.parametrize("trained", [False, True])
+def test_chain_law_small(gpu, trained):
+    import torch
+    from perceval_amd.mis import LearnedMIS, ProposalNet, seq_to_occ
+    m, n = 5, 3
+    U = haar_unitary(m, seed=31)
+    inp = [1, 1, 1, 0, 0]
+    net = ProposalNet(m, n, hidden=32, seed=2)
+    if trained:
+        net.train(torch.from_numpy(_seqs_from_cc(U, inp, 20000, seed=4)).cuda(), epochs=3, batch=1024)
+    mis = LearnedMIS(U, inp, net, chunk=4096)
+    x0 = torch.from_numpy(_seqs_from_cc(U, inp, 1, seed=8)[0]).cuda()
+    res = mis.run(60000, seed=5, x0_seq=x0)
+    occ = seq_to_occ(res["seq"], m).cpu().numpy()
+    exact = oracle.distribution_exact(U, inp)
+    c = Counter(map(tuple, occ.tolist()))
+    tvd = 0.5 * sum(abs(c.get(s, 0) / len(occ) - p) for s, p in exact.items())
+    _say(f"trained={trained} acceptance={res['acceptance']:.3f} tvd={tvd:.4f}")
+    assert tvd < 0.02
+    mis.close()
+
+
+def test_learned_proposal_beats_untrained_m20_n6(gpu):
+    import torch
+    from perceval_amd.mis import LearnedMIS, ProposalNet, marginal_tvds, seq_to_occ
+    from perceval_amd.sampling import cc_samples
+    m, n = 20, 6
+    U = haar_unitary(m, seed=77)
+    inp = [1] * n + [0] * (m - n)
+    train = cc_samples(U, inp, 30000, seed=1)
+    ref = cc_samples(U, inp, 30000, seed=2)
+    from perceval_amd.mis import occ_to_seq
+    tseq = occ_to_seq(torch.from_numpy(train).cuda(), n)
+    x0 = tseq[0]
+    accs = {}
+    for trained in (False, True):
+        net = ProposalNet(m, n, hidden=128, seed=0)
+        if trained:
+            hist = net.train(tseq, epochs=4, batch=2048)
+            _say(f"train nll {hist}")
+        mis = LearnedMIS(U, inp, net, chunk=8192)
+        res = mis.run(30000, seed=11, x0_seq=x0)
+        occ = seq_to_occ(res["seq"], m).cpu().numpy()
+        tv = marginal_tvds(occ, ref)
+        accs[trained] = res["acceptance"]
+        _say(f"trained={trained} acceptance={res['acceptance']:.3f} {tv}")
+        mis.close()
+        if trained:
+            assert tv["tvd1_mean"] < 0.02
+    assert accs[True] > accs[False]
