@@ -115,17 +115,24 @@ __global__ void ce_kernel(const float *__restrict__ logits, int M, int m, const 
     if (i >= M) return;
     const float *z = logits + (size_t)i * m;
     const int l0 = lo[i];
+    const int t = target[i];
+    if (t < l0 || t >= m || l0 < 0) {   // target outside the allowed modes: q = 0
+        loss[i] = __int_as_float(0x7f800000);
+        if (dlogits)
+            for (int j = 0; j < m; j++) dlogits[(size_t)i * m + j] = 0.0f;
+        return;
+    }
     float mx = -3.0e38f;
     for (int j = l0; j < m; j++) mx = fmaxf(mx, z[j]);
     float se = 0.0f;
     for (int j = l0; j < m; j++) se += __expf(z[j] - mx);
     const float lse = mx + __logf(se);
-    loss[i] = lse - z[target[i]];
+    loss[i] = lse - z[t];
     if (dlogits) {
         float *d = dlogits + (size_t)i * m;
         for (int j = 0; j < m; j++) {
             const float p = (j >= l0) ? __expf(z[j] - lse) : 0.0f;
-            d[j] = (p - (j == target[i] ? 1.0f : 0.0f)) * scale;
+            d[j] = (p - (j == t ? 1.0f : 0.0f)) * scale;
         }
     }
 }
@@ -163,9 +170,13 @@ __global__ void features_kernel(const int32_t *__restrict__ seq, int B, int n, i
     const int D = m + n;
     float *xb = x + (size_t)b * D;
     for (int j = 0; j < D; j++) xb[j] = 0.0f;
-    for (int q = 0; q < p; q++) xb[seq[(size_t)b * n + q]] += 1.0f;
+    for (int q = 0; q < p; q++) {
+        const int j = seq[(size_t)b * n + q];
+        if (j >= 0 && j < m) xb[j] += 1.0f;
+    }
     xb[m + p] = 1.0f;
-    lo[b] = p > 0 ? seq[(size_t)b * n + p - 1] : 0;
+    const int l = p > 0 ? seq[(size_t)b * n + p - 1] : 0;
+    lo[b] = (l >= 0 && l < m) ? l : 0;
 }
 
 // Teacher-forcing features for all (row, step) pairs of a batch of sorted
@@ -179,10 +190,14 @@ __global__ void features_all_kernel(const int32_t *__restrict__ seq, int B, int 
     const int D = m + n;
     float *xr = x + (size_t)r * D;
     for (int j = 0; j < D; j++) xr[j] = 0.0f;
-    for (int q = 0; q < p; q++) xr[seq[(size_t)b * n + q]] += 1.0f;
+    for (int q = 0; q < p; q++) {
+        const int j = seq[(size_t)b * n + q];
+        if (j >= 0 && j < m) xr[j] += 1.0f;
+    }
     xr[m + p] = 1.0f;
-    lo[r] = p > 0 ? seq[(size_t)b * n + p - 1] : 0;
-    target[r] = seq[(size_t)b * n + p];
+    const int l = p > 0 ? seq[(size_t)b * n + p - 1] : 0;
+    lo[r] = (l >= 0 && l < m) ? l : 0;
+    target[r] = seq[(size_t)b * n + p];   // validated by ce_kernel
 }
 
 // Draw step p of every row from the masked softmax of its logits; the uniform
