@@ -1,20 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X simulation core.
 
-Headline (one JSON line on stdout): batched complex permanents at n = 20 --
-4096 n x n submatrices of a seeded 64 x 64 Haar unitary, one persistent launch
-per step, inputs resident in HBM -- in permanents per second, with the FP64
-roofline of the permanent kernel and the CPU oracle timed on this host.
+Prints one JSON line.  Headline (`value`): batched complex permanents at
+n = 20 -- 4096 n x n submatrices (rows and columns drawn without replacement)
+of a seeded 64 x 64 Haar unitary, one persistent launch per step, inputs
+resident in HBM -- in permanents per second, with the FP64 roofline of the
+permanent kernel and the CPU oracle timed on this host.  The other measured
+configurations are reported beside it:
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+  mixed_n12_24  4096 permanents, n uniform in 12..24, one launch
+  slos_m24_n10  SLOS all_prob, Haar 24x24, 10 photons, 92,561,040 outputs
+                (achieved algorithmic HBM GB/s)
+  gradient      adjoint phase gradients, Clements 12 modes (132 phases),
+                6 photons, 256 parameter sets, 32 targets
+  sampling      m = 60, n = 20: plain GPU Clifford-Clifford samples/s, the
+                learned Metropolis independence sampler's samples/s, and the
+                1- and 2-mode marginal TVD between the two sample sets
 
-A step is one pass of the hot path over the batch.  Only one GPU is used;
---gpus above 1 exits with a message.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--sections ...]
+
+A step is one pass of the permanent hot path over the batch.  Only one GPU
+is used; --gpus above 1 exits with a message.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,6 +39,7 @@ if ROOT not in sys.path:
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 vector, FMA = 2 flops
 HBM_PEAK_GBS = 8000.0
+SECTIONS = ("perm", "mixed", "slos", "gradient", "sampling")
 
 
 def glynn_flops(n: int) -> float:
@@ -35,44 +48,233 @@ def glynn_flops(n: int) -> float:
     return float(2 ** (n - 1) * (8 * n - 4))
 
 
+def slos_bytes(m: int, n: int) -> float:
+    """Algorithmic HBM bytes of fused SLOS all_prob: every layer k < n is
+    written once and read once as parent (16 B complex), the last layer reads
+    layer n-1 and writes 8-byte probabilities."""
+    N = [math.comb(m + k - 1, k) for k in range(n + 1)]
+    b = sum(16.0 * (N[k - 1] + N[k]) for k in range(1, n))
+    return b + 16.0 * N[n - 1] + 8.0 * N[n]
+
+
 def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+
+
+class Timer:
+    """HIP events on torch's current stream, which is the stream every
+    entry point of this library is launched on."""
+
+    def __init__(self):
+        import torch
+        self.a = torch.cuda.Event(enable_timing=True)
+        self.b = torch.cuda.Event(enable_timing=True)
+
+    def __enter__(self):
+        import torch
+        torch.cuda.synchronize()
+        self.a.record()
+        return self
+
+    def __exit__(self, *exc):
+        import torch
+        self.b.record()
+        torch.cuda.synchronize()
+        self.ms = self.a.elapsed_time(self.b)
+
+
 def time_plan(plan, d_mats, out, steps: int, warmup: int):
-    import torch
     for _ in range(warmup):
         plan(d_mats, out)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(steps):
-        plan(d_mats, out)
-    stop.record()
-    torch.cuda.synchronize()
-    return start.elapsed_time(stop) / steps   # ms per step
+    with Timer() as t:
+        for _ in range(steps):
+            plan(d_mats, out)
+    return t.ms / steps
 
 
-def cpu_baseline_perm(mats, budget_s: float = 8.0):
-    """Oracle (chunked double Glynn, OpenMP) on a growing sample of the same
-    matrices until about budget_s of wall time has been spent."""
-    import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    k, spent, done = 16, 0.0, 0
-    while True:
-        sample = mats[:k]
+# ------------------------------------------------------------------ sections
+
+def bench_perm(args, res):
+    import torch
+    from perceval_amd.permanent import PermanentPlan, pack
+    from perceval_amd.utils.workloads import submatrix_batch
+    n = args.order
+    log(f"[bench] perm: {args.batch} x n={n} submatrices of a 64x64 Haar unitary")
+    _, orders, mats = submatrix_batch(args.batch, n, n, seed=2024)
+    o, flat = pack(mats)
+    d_mats = torch.from_numpy(flat).cuda()
+    out = torch.empty(len(o), dtype=torch.complex128, device="cuda")
+    plan = PermanentPlan(o, max_chunk_log2=args.chunk_log2, variant=args.variant)
+    log(f"[bench] plan {plan.info()} variant {args.variant}; warmup {args.warmup}, steps {args.steps}")
+    ms = time_plan(plan, d_mats, out, args.steps, args.warmup)
+    plan.close()
+    achieved = len(o) * glynn_flops(n) / (ms * 1e-3) / 1e12
+    res["perm"] = {"ms": ms, "perms_per_s": len(o) / (ms * 1e-3), "tflops": achieved}
+    log(f"[bench] perm: {ms:.3f} ms/step, {len(o) / (ms * 1e-3):.1f} perms/s, {achieved:.2f} TFLOP/s "
+        f"({100 * achieved / FP64_PEAK_TFLOPS:.1f}% of FP64 peak)")
+    if not args.no_cpu:
+        import oracle
+        th = cpu_threads()
+        k, dt = 16, 0.0
+        while True:
+            t0 = time.perf_counter()
+            oracle.perm_batch(mats[:k], precise=False, threads=th)
+            dt = time.perf_counter() - t0
+            if dt >= args.cpu_budget or k >= len(mats):
+                break
+            k = min(len(mats), k * 2)
+        res["perm_cpu"] = {"value": k / dt, "unit": "perms/s", "cores": th, "kind": "port",
+                           "sample": f"{k} of the {len(mats)} order-{n} matrices, oracle chunked-double Glynn "
+                                     f"(oracle/pcvl_oracle.c), {th} POSIX threads, {dt:.2f} s wall"}
+        log(f"[bench] perm CPU: {k / dt:.1f} perms/s on {th} threads")
+
+
+def bench_mixed(args, res):
+    import torch
+    from perceval_amd.permanent import PermanentPlan, pack
+    from perceval_amd.utils.workloads import submatrix_batch
+    _, morders, mmats = submatrix_batch(4096, 12, 24, seed=2025)
+    mo, mflat = pack(mmats)
+    md = torch.from_numpy(mflat).cuda()
+    mout = torch.empty(len(mo), dtype=torch.complex128, device="cuda")
+    mplan = PermanentPlan(mo, max_chunk_log2=args.chunk_log2)
+    mms = time_plan(mplan, md, mout, max(3, args.steps // 4), 1)
+    mplan.close()
+    mflops = sum(glynn_flops(int(k)) for k in mo)
+    res["mixed_n12_24"] = {"perms_per_s": len(mo) / (mms * 1e-3), "ms_per_step": mms,
+                           "tflops": mflops / (mms * 1e-3) / 1e12,
+                           "frac_fp64_peak": mflops / (mms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    log(f"[bench] mixed n=12..24: {mms:.3f} ms/step, {res['mixed_n12_24']['tflops']:.2f} TFLOP/s")
+
+
+def bench_slos(args, res):
+    import torch
+    from perceval_amd.slos import PROBS, SlosPlan
+    from perceval_amd.utils.workloads import haar_unitary
+    m, n = 24, 10
+    U = haar_unitary(m, seed=24)
+    inp = [1] * n + [0] * (m - n)
+    plan = SlosPlan(inp)
+    dU = torch.from_numpy(U).cuda()
+    out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
+    plan.run(dU, PROBS, out)
+    reps = 5
+    with Timer() as t:
+        for _ in range(reps):
+            plan.run(dU, PROBS, out)
+    ms = t.ms / reps
+    total = float(out.sum().item())
+    plan.close()
+    gbytes = slos_bytes(m, n) / 1e9
+    res["slos_m24_n10"] = {"ms": ms, "outputs": plan.count, "algorithmic_GB": gbytes,
+                           "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,
+                           "sum_probs": total}
+    log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}")
+    if not args.no_cpu:
+        import oracle
+        th = cpu_threads()
         t0 = time.perf_counter()
-        oracle.perm_batch(sample, precise=False, threads=threads)
+        oracle.slos_coefs(U, inp, "gather", threads=th)
         dt = time.perf_counter() - t0
-        spent += dt
-        done = k
-        if dt >= budget_s or k >= len(mats) or spent > 2 * budget_s:
-            break
-        k = min(len(mats), k * 2)
-    return {"value": done / dt, "unit": "perms/s", "cores": threads, "kind": "port",
-            "sample": f"{done} of the n=20 matrices, oracle chunked-double Glynn (oracle/pcvl_oracle.c), "
-                      f"{threads} POSIX threads, {dt:.2f} s wall"}
+        res["slos_cpu"] = {"value": dt * 1e3, "unit": "ms per all_prob (coefficients)", "cores": th, "kind": "port",
+                           "sample": f"full m=24 n=10 layer recursion, oracle gather form, {th} POSIX threads"}
+        log(f"[bench] SLOS CPU: {dt:.2f} s on {th} threads")
+
+
+def bench_gradient(args, res):
+    import torch
+    from perceval_amd.components import clements_mesh
+    from perceval_amd.gradient import MeshProgram, mesh_adjoint
+    from perceval_amd.utils.states import fock_space
+    prog = MeshProgram(clements_mesh(12))
+    rng = np.random.RandomState(4)
+    phases = torch.from_numpy(rng.uniform(0, 2 * math.pi, size=(256, prog.nparams))).cuda()
+    states = fock_space(12, 6)
+    targets = states[np.sort(rng.choice(len(states), 32, replace=False))]
+    inp = [1] * 6 + [0] * 6
+    mesh_adjoint(prog, phases, inp, targets)
+    reps = 5
+    with Timer() as t:
+        for _ in range(reps):
+            mesh_adjoint(prog, phases, inp, targets)
+    ms = t.ms / reps
+    res["gradient"] = {"ms_per_batch": ms, "sets_per_s": 256 / (ms * 1e-3)}
+    log(f"[bench] gradient: {ms:.3f} ms per 256 sets, {256 / (ms * 1e-3):.0f} sets/s")
+    if not args.no_cpu:
+        import oracle
+        ph = phases[:2].cpu().numpy()
+
+        def loss(p):
+            U = prog.unitary(p)
+            return sum(abs(oracle.perm(oracle.submatrix(U, inp, s))) ** 2 / oracle.prodnfact(s) for s in targets)
+        t0 = time.perf_counter()
+        for b in range(len(ph)):
+            for k in range(prog.nparams):
+                a = ph[b].copy(); a[k] += 1e-6
+                c = ph[b].copy(); c[k] -= 1e-6
+                _ = (loss(a) - loss(c)) / 2e-6
+        dt = time.perf_counter() - t0
+        res["gradient_cpu"] = {"value": len(ph) / dt, "unit": "sets/s", "cores": 1, "kind": "port",
+                               "sample": "2 parameter sets, central differences over 132 phases with oracle "
+                                         "permanents, 1 thread"}
+        log(f"[bench] gradient CPU (FD): {len(ph) / dt:.2f} sets/s")
+
+
+def bench_sampling(args, res):
+    import torch
+    from perceval_amd.mis import LearnedMIS, ProposalNet, marginal_tvds, occ_to_seq, seq_to_occ
+    from perceval_amd.sampling import cc_samples_device
+    from perceval_amd.utils.workloads import haar_unitary
+    m, n, N = 60, 20, args.samples
+    U = haar_unitary(m, seed=60)
+    inp = [1] * n + [0] * (m - n)
+    dU = torch.from_numpy(U).cuda()
+    cc_samples_device(dU, inp, 256, seed=99)                      # warm-up (code objects, allocator)
+    with Timer() as t:
+        train = cc_samples_device(dU, inp, N, seed=1)
+    cc_ms = t.ms
+    log(f"[bench] plain CC: {N} samples in {cc_ms:.1f} ms = {N / (cc_ms * 1e-3):.0f} samples/s")
+    ref = cc_samples_device(dU, inp, N, seed=2)                   # independent judge set
+    net = ProposalNet(m, n, hidden=256, seed=0)
+    tseq = occ_to_seq(train, n)
+    t0 = time.perf_counter()
+    hist = net.train(tseq, epochs=args.epochs, batch=2048, lr=2e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"[bench] proposal training: {train_s:.2f} s, mean -log q per epoch {['%.3f' % h for h in hist]}")
+    mis = LearnedMIS(U, inp, net, chunk=16384)
+    x0 = occ_to_seq(cc_samples_device(dU, inp, 1, seed=7), n)[0]
+    mis.run(4096, seed=3, x0_seq=x0)                              # warm-up
+    with Timer() as t:
+        out = mis.run(N, seed=5, x0_seq=x0)
+    mis_ms = t.ms
+    acc = out["acceptance"]
+    mis.close()
+    occ_mis = seq_to_occ(out["seq"], m).cpu().numpy()
+    tv = marginal_tvds(occ_mis, ref.cpu().numpy())
+    tv_cc = marginal_tvds(train.cpu().numpy(), ref.cpu().numpy())  # noise floor of two independent CC sets
+    res["sampling_m60_n20"] = {
+        "samples": N, "cc_samples_per_s": N / (cc_ms * 1e-3), "mis_samples_per_s": N / (mis_ms * 1e-3),
+        "mis_vs_cc": (N / (mis_ms * 1e-3)) / (N / (cc_ms * 1e-3)), "mis_acceptance": acc,
+        "tvd_mis_vs_cc": tv, "tvd_cc_vs_cc": tv_cc, "train_seconds": train_s,
+        "train_nll": [float(h) for h in hist]}
+    log(f"[bench] MIS: {N} samples in {mis_ms:.1f} ms = {N / (mis_ms * 1e-3):.0f} samples/s "
+        f"({(N / (mis_ms * 1e-3)) / (N / (cc_ms * 1e-3)):.2f}x plain CC), acceptance {acc:.3f}, TVD {tv}, "
+        f"CC-vs-CC floor {tv_cc}")
+    if not args.no_cpu:
+        import oracle
+        th = cpu_threads()
+        k = max(th, 64)
+        t0 = time.perf_counter()
+        oracle.cc_samples(U, inp, k, seed=1, threads=th)
+        dt = time.perf_counter() - t0
+        res["sampling_cpu"] = {"value": k / dt, "unit": "samples/s", "cores": th, "kind": "port",
+                               "sample": f"{k} samples m=60 n=20, oracle Clifford-Clifford, {th} POSIX threads"}
+        log(f"[bench] CC CPU: {k / dt:.1f} samples/s on {th} threads")
 
 
 def main():
@@ -82,77 +284,54 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--order", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--chunk-log2", type=int, default=0, help="permanent lane chunk (0 = default)")
+    ap.add_argument("--variant", type=int, default=0, help="permanent kernel variant (0 = by order)")
+    ap.add_argument("--samples", type=int, default=100000)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=5.0, help="seconds of CPU baseline per section")
+    ap.add_argument("--sections", default=",".join(SECTIONS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baselines")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-order batch")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
-              f"the batch is embarrassingly parallel and would shard by matrices, see DESIGN.md",
+              f"every workload here is a batch of independent items and would shard by items across GPUs",
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    from perceval_amd.permanent import PermanentPlan, pack
-    from perceval_amd.utils.workloads import submatrix_batch
+    sections = [s for s in args.sections.split(",") if s]
+    if args.no_mixed and "mixed" in sections:
+        sections.remove("mixed")
+    res = {}
+    for s in sections:
+        {"perm": bench_perm, "mixed": bench_mixed, "slos": bench_slos, "gradient": bench_gradient,
+         "sampling": bench_sampling}[s](args, res)
 
-    n = args.order
-    log(f"[bench] building {args.batch} x n={n} submatrices of a 64x64 Haar unitary")
-    _, orders, mats = submatrix_batch(args.batch, n, n, seed=2024)
-    o, flat = pack(mats)
-    d_mats = torch.from_numpy(flat).cuda()
-    out = torch.empty(len(o), dtype=torch.complex128, device="cuda")
-    plan = PermanentPlan(o)
-    log(f"[bench] plan {plan.info()}; warmup {args.warmup}, steps {args.steps}")
-    ms = time_plan(plan, d_mats, out, args.steps, args.warmup)
-    perms_per_s = len(o) / (ms * 1e-3)
-    flops = len(o) * glynn_flops(n)
-    achieved = flops / (ms * 1e-3) / 1e12
-    log(f"[bench] {ms:.3f} ms/step, {perms_per_s:.1f} perms/s, {achieved:.2f} TFLOP/s")
-
-    extras = {}
-    if not args.no_mixed:
-        _, morders, mmats = submatrix_batch(4096, 12, 24, seed=2025)
-        mo, mflat = pack(mmats)
-        md = torch.from_numpy(mflat).cuda()
-        mout = torch.empty(len(mo), dtype=torch.complex128, device="cuda")
-        mplan = PermanentPlan(mo)
-        mms = time_plan(mplan, md, mout, max(3, args.steps // 4), 1)
-        mflops = sum(glynn_flops(int(k)) for k in mo)
-        extras["mixed_n12_24"] = {"perms_per_s": len(mo) / (mms * 1e-3), "ms_per_step": mms,
-                                  "tflops": mflops / (mms * 1e-3) / 1e12,
-                                  "frac_fp64_peak": mflops / (mms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
-        log(f"[bench] mixed n=12..24: {mms:.3f} ms/step, {extras['mixed_n12_24']['tflops']:.2f} TFLOP/s")
-        mplan.close()
-
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] CPU oracle baseline")
-        cpu = cpu_baseline_perm(mats)
-        log(f"[bench] CPU {cpu['value']:.1f} perms/s on {cpu['cores']} threads")
-
-    plan.close()
+    perm = res.get("perm", {"ms": float("nan"), "perms_per_s": float("nan"), "tflops": float("nan")})
     line = {
         "metric": "perms/s at n=20 (%FP64 peak); samples/s m=60 n=20 learned-MIS vs plain CC",
-        "value": perms_per_s,
+        "value": perm["perms_per_s"],
         "unit": "perms/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": perm["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128 (fp64)",
-        "data": "synthetic: seeded Haar 64x64 unitary, n x n submatrices (rows/cols without replacement)",
-        "config": {"workload": f"batched Glynn permanents, {len(o)} matrices of order {n}",
-                   "batch": len(o), "order": n, "parallelism": "single GPU"},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
-        "cpu_baseline": cpu,
+        "data": "synthetic: seeded Haar unitaries; n x n submatrices (rows/cols without replacement) of a 64x64 one",
+        "config": {"workload": f"batched Glynn permanents, {args.batch} matrices of order {args.order}",
+                   "batch": args.batch, "order": args.order, "parallelism": "single GPU"},
+        "roofline": {"bound": "fp64-valu", "achieved": perm["tflops"], "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": perm["tflops"] / FP64_PEAK_TFLOPS, "traffic": None},
+        "cpu_baseline": res.get("perm_cpu"),
     }
-    line.update(extras)
+    for k, v in res.items():
+        if k not in ("perm", "perm_cpu"):
+            line[k] = v
     print(json.dumps(line), flush=True)
 
 

@@ -50,6 +50,13 @@ typedef struct pcvl_perm_plan_s *pcvl_perm_plan;
 int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
                           int max_chunk_log2, pcvl_perm_plan *out);
 
+/* As pcvl_perm_plan_create, with the kernel variant of single-order batches:
+ * 0 picks by order, 1 keeps Gray-code row 1 in VGPRs, 2 reads it from LDS
+ * (fewer VGPRs, more resident waves).  Mixed-order batches ignore it.  Used
+ * by the autotuner. */
+int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                             int max_chunk_log2, int variant, pcvl_perm_plan *out);
+
 /* Enqueue the permanents of the batch on `stream`.  Allocates nothing, so it
  * may be captured in a HIP graph and replayed; replays of one plan must not
  * overlap in time. */
@@ -212,6 +219,12 @@ int pcvl_mis_sample_step(const float *d_logits, int B, int m, int n, int p, cons
 /* n x n submatrices U[seq[b], cols] of B proposals (complex, packed). */
 int pcvl_mis_submatrices(const double *d_U, int m, const int32_t *d_cols, int n, const int32_t *d_seq,
                          int64_t B, double *d_out, hipStream_t stream);
+
+/* Permanents of U[seq[b], cols] for B sorted mode sequences (B x n int32, n <= 28),
+ * with repeated rows summed by multiplicity (a group of k equal rows costs
+ * k+1 Gray terms instead of 2^k).  d_out: B complex. */
+int pcvl_mis_permanents(const double *d_U, int m, const int32_t *d_cols, int n, const int32_t *d_seq,
+                        int64_t B, double *d_out, hipStream_t stream);
 
 /* log p of B proposals from their permanents (complex) and sequences. */
 int pcvl_mis_logp(const double *d_perms, const int32_t *d_seq, int n, int64_t B, double log_prod_in,

@@ -40,6 +40,7 @@ SIGNATURES = {
     "pcvl_last_error": (ctypes.c_char_p, []),
     "pcvl_abi_version": (c_int, []),
     "pcvl_perm_plan_create": (c_int, [P(c_i32), P(c_i64), c_i64, c_int, P(c_void_p)]),
+    "pcvl_perm_plan_create_ex": (c_int, [P(c_i32), P(c_i64), c_i64, c_int, c_int, P(c_void_p)]),
     "pcvl_perm_plan_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "pcvl_perm_plan_destroy": (c_int, [c_void_p]),
     "pcvl_perm_plan_info": (c_int, [c_void_p, P(c_i64), P(c_int), P(c_int)]),
@@ -66,6 +67,7 @@ SIGNATURES = {
     "pcvl_mis_sample_step": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_u64, c_u64, c_void_p,
                                      c_void_p, c_void_p]),
     "pcvl_mis_submatrices": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_i64, c_void_p, c_void_p]),
+    "pcvl_mis_permanents": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_i64, c_void_p, c_void_p]),
     "pcvl_mis_logp": (c_int, [c_void_p, c_void_p, c_int, c_i64, c_double, c_void_p, c_void_p]),
     "pcvl_mis_chain": (c_int, [c_void_p, c_i64, c_double, c_u64, c_u64, c_void_p, c_void_p, c_void_p]),
     "pcvl_mis_seq_to_occ": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),

@@ -13,6 +13,8 @@
 #include "common.h"
 #include "../../include/pcvl_hip.h"
 
+#include <algorithm>
+
 namespace pcvl {
 
 // Packs the n x n submatrix U[seq[b], cols] of every proposal b (row r =
@@ -124,9 +126,193 @@ __global__ void mis_gather_kernel(const int32_t *__restrict__ prop, const int64_
     out[e] = k < 0 ? x0[r] : prop[k * n + r];
 }
 
+
+// ------------------------------------------------------------------------
+// Permanents of proposals with repeated rows (multiset Glynn).
+//
+// A proposal s (sorted modes) selects rows U[s_r, cols]; output photons in the
+// same mode give identical rows.  In Glynn's sum the k copies of a row enter
+// the column sums only through their sign total k - 2t (t = copies with sign
+// -1), with weight C(k, t) (-1)^t, so a group of k copies needs k+1 terms
+// instead of 2^k.  Rows are ordered [fixed row | u unique rows | G groups];
+// the unique rows are walked in Gray-code order, the group digits form an
+// outer mixed-radix index o with weight W_o = prod_g C(k_g, t_g) (-1)^{t_g}:
+//     perm = 2 sum_o W_o sum_{t < 2^u} (-1)^t prod_c h_c(o, t),
+//     h_c = (1/2) (a_fc + sum_unique delta_i a_ic + sum_g (k_g - 2 t_g) a_gc).
+// (Half column sums let a flip add +-a directly; prod c = 2^n prod h.)
+// One wavefront per proposal; lanes take (o, inner chunk) units.
+constexpr int MISP_MAX_N = 28;
+
+struct MispShared {
+    cd A[MISP_MAX_N * MISP_MAX_N];   // rows of the fixed / unique / group modes
+    int kg[MISP_MAX_N];              // multiplicity of each group
+    int rmode[MISP_MAX_N];           // mode of each stored row
+    int gmode[MISP_MAX_N], gcnt[MISP_MAX_N];   // distinct modes of the sequence and their counts
+    int nu, ng;                      // unique rows (besides the fixed one), groups
+};
+
+template <int N>
+__device__ __forceinline__ void misp_prod_acc(cd &acc, const cd *h, bool neg)
+{
+    cd p = h[0];
+#pragma unroll
+    for (int c = 1; c < N; c++) p = cmul(p, h[c]);
+    if (neg) { acc.r -= p.r; acc.i -= p.i; } else { acc.r += p.r; acc.i += p.i; }
+}
+
+template <int N>
+__global__ void __launch_bounds__(64) mis_perm_kernel(const cd *__restrict__ U, int m, const int32_t *__restrict__ cols,
+                                                      const int32_t *__restrict__ seq, int64_t B, cd *__restrict__ out)
+{
+    __shared__ MispShared sh;
+    const int lane = threadIdx.x;
+    for (int64_t b = blockIdx.x; b < B; b += gridDim.x) {
+        if (lane == 0) {
+            // groups of equal modes in the sorted sequence (worked in LDS so
+            // the per-proposal bookkeeping holds no VGPR arrays)
+            int R = 0;
+            for (int r = 0; r < N; r++) {
+                int j = seq[b * N + r];
+                j = (j >= 0 && j < m) ? j : 0;
+                if (R > 0 && sh.gmode[R - 1] == j) sh.gcnt[R - 1]++;
+                else { sh.gmode[R] = j; sh.gcnt[R] = 1; R++; }
+            }
+            int fixed = -1;
+            for (int g = 0; g < R; g++) if (sh.gcnt[g] == 1) { fixed = g; break; }
+            int nrow = 0, nu = 0, ng = 0;
+            if (fixed >= 0) {
+                sh.rmode[nrow++] = sh.gmode[fixed];
+                for (int g = 0; g < R; g++)
+                    if (g != fixed && sh.gcnt[g] == 1) { sh.rmode[nrow++] = sh.gmode[g]; nu++; }
+            } else {
+                sh.rmode[nrow++] = sh.gmode[0];
+                sh.gcnt[0] -= 1;   // one copy of group 0 is the fixed row
+            }
+            for (int g = 0; g < R; g++) {
+                if (g == fixed || sh.gcnt[g] < 1) continue;
+                if (fixed >= 0 && sh.gcnt[g] == 1) continue;   // already a unique row
+                sh.rmode[nrow++] = sh.gmode[g];
+                sh.kg[ng++] = sh.gcnt[g];
+            }
+            sh.nu = nu;
+            sh.ng = ng;
+        }
+        __syncthreads();
+        const int nu = sh.nu, ng = sh.ng, nrow = 1 + nu + ng;
+        // rows: A[r][c] = U[mode_r, cols[c]]
+        for (int e = lane; e < nrow * N; e += 64) {
+            const int r = e / N, c = e % N;
+            sh.A[e] = U[(size_t)sh.rmode[r] * m + cols[c]];
+        }
+        __syncthreads();
+        // outer size and inner chunking
+        uint64_t O = 1;
+        for (int g = 0; g < ng; g++) O *= (uint64_t)(sh.kg[g] + 1);
+        const int inner_log2 = nu;
+        int c = inner_log2;                          // steps per unit = 2^c
+        while (c > 0 && (O << (inner_log2 - c)) < 64) c--;
+        if (c > 16) c = 16;
+        const uint64_t nchunks = 1ull << (inner_log2 - c);
+        const uint64_t units = O * nchunks;
+        cd acc = {0.0, 0.0};
+        for (uint64_t unit = lane; unit < units; unit += 64) {
+            uint64_t o = unit / nchunks;
+            const uint64_t chunk = unit % nchunks;
+            const uint64_t base = chunk << c;
+            const uint64_t g0 = base ^ (base >> 1);
+            cd h[N];
+#pragma unroll
+            for (int cc = 0; cc < N; cc++) h[cc] = {0.5 * sh.A[cc].r, 0.5 * sh.A[cc].i};
+            for (int i = 1; i <= nu; i++) {
+                const double d = ((g0 >> (i - 1)) & 1) ? -0.5 : 0.5;
+#pragma unroll
+                for (int cc = 0; cc < N; cc++) {
+                    h[cc].r = fma(d, sh.A[i * N + cc].r, h[cc].r);
+                    h[cc].i = fma(d, sh.A[i * N + cc].i, h[cc].i);
+                }
+            }
+            // group digits of o (mixed radix k_g + 1) and the weight W_o
+            double W = 1.0;
+            for (int g = 0; g < ng; g++) {
+                const int k = sh.kg[g];
+                const int t = (int)(o % (uint64_t)(k + 1));
+                o /= (uint64_t)(k + 1);
+                double binom = 1.0;
+                for (int i = 1; i <= t; i++) binom = binom * (double)(k - t + i) / (double)i;
+                W *= (t & 1) ? -binom : binom;
+                const double f = 0.5 * (double)(k - 2 * t);
+#pragma unroll
+                for (int cc = 0; cc < N; cc++) {
+                    h[cc].r = fma(f, sh.A[(1 + nu + g) * N + cc].r, h[cc].r);
+                    h[cc].i = fma(f, sh.A[(1 + nu + g) * N + cc].i, h[cc].i);
+                }
+            }
+            cd part = {0.0, 0.0};
+            const uint64_t steps = 1ull << c;
+            for (uint64_t u = 0; u < steps; u++) {
+                const uint64_t t = base + u;
+                if (u) {
+                    const int bb = __builtin_ctzll(t);
+                    const double sg = (((t >> bb) ^ (t >> (bb + 1))) & 1) ? -1.0 : 1.0;
+                    const cd *row = sh.A + (bb + 1) * N;
+#pragma unroll
+                    for (int cc = 0; cc < N; cc++) {
+                        h[cc].r = fma(sg, row[cc].r, h[cc].r);
+                        h[cc].i = fma(sg, row[cc].i, h[cc].i);
+                    }
+                }
+                misp_prod_acc<N>(part, h, (t & 1) != 0);
+#pragma unroll
+                for (int cc = 0; cc < N; cc++)
+                    asm volatile("" : "+v"(h[cc].r), "+v"(h[cc].i) : "v"(part.r), "v"(part.i));
+            }
+            acc.r = fma(W, part.r, acc.r);
+            acc.i = fma(W, part.i, acc.i);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            acc.r += __shfl_xor(acc.r, off, 64);
+            acc.i += __shfl_xor(acc.i, off, 64);
+        }
+        if (lane == 0) out[b] = {2.0 * acc.r, 2.0 * acc.i};
+        __syncthreads();
+    }
+}
+
+template <int N>
+static int launch_mis_perm(int n, const cd *U, int m, const int32_t *cols, const int32_t *seq, int64_t B, cd *out,
+                           hipStream_t st)
+{
+    if constexpr (N > MISP_MAX_N) {
+        set_error("photon number above 28");
+        return ERR_ARG;
+    } else {
+        if (n != N) return launch_mis_perm<N + 1>(n, U, m, cols, seq, B, out, st);
+        int dev = 0, cus = 0, per_cu = 0;
+        PCVL_HIP_CHECK(hipGetDevice(&dev));
+        PCVL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&mis_perm_kernel<N>), 64, 0));
+        if (per_cu < 1) per_cu = 1;
+        const int grid = (int)std::min<int64_t>(B, (int64_t)cus * per_cu);
+        hipLaunchKernelGGL((mis_perm_kernel<N>), dim3(grid), dim3(64), 0, st, U, m, cols, seq, B, out);
+        PCVL_HIP_CHECK(hipGetLastError());
+        return OK;
+    }
+}
+
 } // namespace pcvl
 
 using namespace pcvl;
+
+extern "C" int pcvl_mis_permanents(const double *d_U, int m, const int32_t *d_cols, int n, const int32_t *d_seq,
+                                   int64_t B, double *d_out, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(m >= 1 && n >= 1 && n <= MISP_MAX_N && B >= 0, "bad sizes (1 <= n <= 28)");
+    if (B == 0) return OK;
+    PCVL_ARG_CHECK(d_U && d_cols && d_seq && d_out, "null device buffer");
+    return launch_mis_perm<1>(n, reinterpret_cast<const cd *>(d_U), m, d_cols, d_seq, B, reinterpret_cast<cd *>(d_out),
+                              stream);
+}
 
 extern "C" int pcvl_mis_seq_to_occ(const int32_t *d_seq, int64_t B, int n, int m, int32_t *d_occ, hipStream_t stream)
 {

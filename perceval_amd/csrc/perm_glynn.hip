@@ -99,7 +99,7 @@ __device__ __forceinline__ void flip_lds(cd *cs, const cd *__restrict__ row, dou
 
 // One wave item: 64 lanes x 2^c steps of one order-N matrix.  sA is this
 // wave's LDS slice holding 2*A (row-major).
-template <int N>
+template <int N, int R1 = 0>
 __device__ __forceinline__ cd glynn_wave_item(const double *__restrict__ mats, const PermItem &it, cd *sA)
 {
     const int lane = threadIdx.x & 63;
@@ -133,7 +133,10 @@ __device__ __forceinline__ cd glynn_wave_item(const double *__restrict__ mats, c
 #pragma unroll
     for (int j = 0; j < N; j++) cs[j] = {0.5 * cs[j].r, 0.5 * cs[j].i};
 
-    constexpr bool kRow1InRegs = (N <= 22);
+    // Row 1 flips every other step: keep it in VGPRs (R1 = 1), read it from
+    // LDS like the other rows (R1 = 2, fewer VGPRs and more waves), or pick
+    // by order (R1 = 0).
+    constexpr bool kRow1InRegs = (R1 == 1) || (R1 == 0 && N <= 22);
     cd r1[kRow1InRegs ? N : 1];
     if constexpr (kRow1InRegs) {
 #pragma unroll
@@ -193,21 +196,21 @@ __device__ __forceinline__ cd wave_sum(cd v)
     return v;
 }
 
-template <int NLO, int NHI>
+template <int NLO, int NHI, int R1>
 __device__ __forceinline__ cd dispatch_item(const double *__restrict__ mats, const PermItem &it, cd *sA)
 {
     if constexpr (NLO > NHI) {
         return {0.0, 0.0};
     } else {
-        if (it.n == NLO) return glynn_wave_item<NLO>(mats, it, sA);
-        return dispatch_item<NLO + 1, NHI>(mats, it, sA);
+        if (it.n == NLO) return glynn_wave_item<NLO, R1>(mats, it, sA);
+        return dispatch_item<NLO + 1, NHI, R1>(mats, it, sA);
     }
 }
 
 // Persistent wave kernel: one wavefront per workgroup, items pulled from a
 // cost-sorted queue with an atomic cursor.  The cursor is returned to zero by
 // perm_finalize so the same plan can be replayed (also inside a HIP graph).
-template <int NLO, int NHI>
+template <int NLO, int NHI, int R1 = 0>
 __global__ void __launch_bounds__(64, 2) perm_wave_kernel(const double *__restrict__ mats,
                                                        const PermItem *__restrict__ items, int64_t nitems,
                                                        unsigned long long *cursor, cd *__restrict__ partials)
@@ -237,7 +240,7 @@ __global__ void __launch_bounds__(64, 2) perm_wave_kernel(const double *__restri
             it.c = __builtin_amdgcn_readfirstlane(raw.c);
         }
         if (it.n < NLO || it.n > NHI || it.c < 2 || it.c > it.n - 7) continue;
-        cd v = dispatch_item<NLO, NHI>(mats, it, sA);
+        cd v = dispatch_item<NLO, NHI, R1>(mats, it, sA);
         v = wave_sum(v);
         if (threadIdx.x == 0) partials[it.slot] = v;
     }
@@ -326,6 +329,7 @@ struct PermPlanImpl {
     int64_t count = 0;
     int64_t nitems = 0, nwave_mats = 0, nsmall = 0;
     int single_n = 0;          // order shared by every wave item, or 0 if mixed
+    int variant = 0;           // 0 auto, 1 row 1 in VGPRs, 2 row 1 in LDS (single-order plans)
     int grid = 0;
     PermItem *d_items = nullptr;
     unsigned long long *d_cursor = nullptr;
@@ -347,10 +351,10 @@ static int pick_chunk_log2(int n, int max_c)
     return c;
 }
 
-template <int NLO, int NHI>
+template <int NLO, int NHI, int R1>
 static int launch_wave(PermPlanImpl *p, const double *mats, hipStream_t st)
 {
-    hipLaunchKernelGGL((perm_wave_kernel<NLO, NHI>), dim3(p->grid), dim3(64), 0, st, mats, p->d_items,
+    hipLaunchKernelGGL((perm_wave_kernel<NLO, NHI, R1>), dim3(p->grid), dim3(64), 0, st, mats, p->d_items,
                        p->nitems, p->d_cursor, p->d_partials);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
@@ -363,19 +367,27 @@ static int launch_single(PermPlanImpl *p, const double *mats, hipStream_t st)
         set_error("internal: order out of range");
         return ERR_ARG;
     } else {
-        if (p->single_n == N) return launch_wave<N, N>(p, mats, st);
+        if (p->single_n == N) {
+            if (p->variant == 1) return launch_wave<N, N, 1>(p, mats, st);
+            if (p->variant == 2) return launch_wave<N, N, 2>(p, mats, st);
+            return launch_wave<N, N, 0>(p, mats, st);
+        }
         return launch_single<N + 1>(p, mats, st);
     }
 }
 
 template <int N>
-static const void *single_kernel_ptr(int n)
+static const void *single_kernel_ptr(int n, int variant)
 {
     if constexpr (N > PERM_WAVE_MAX) {
         return nullptr;
     } else {
-        if (n == N) return reinterpret_cast<const void *>(&perm_wave_kernel<N, N>);
-        return single_kernel_ptr<N + 1>(n);
+        if (n == N) {
+            if (variant == 1) return reinterpret_cast<const void *>(&perm_wave_kernel<N, N, 1>);
+            if (variant == 2) return reinterpret_cast<const void *>(&perm_wave_kernel<N, N, 2>);
+            return reinterpret_cast<const void *>(&perm_wave_kernel<N, N, 0>);
+        }
+        return single_kernel_ptr<N + 1>(n, variant);
     }
 }
 
@@ -398,10 +410,11 @@ struct pcvl_perm_plan_s {
     PermPlanImpl impl;
 };
 
-extern "C" int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
-                                     int max_chunk_log2, pcvl_perm_plan *out)
+extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                                        int max_chunk_log2, int variant, pcvl_perm_plan *out)
 {
     PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
+    PCVL_ARG_CHECK(variant >= 0 && variant <= 2, "variant outside [0, 2]");
     PCVL_ARG_CHECK(count >= 0, "count < 0");
     PCVL_ARG_CHECK(count == 0 || (h_n && h_offsets), "null order/offset array");
     if (max_chunk_log2 <= 0) max_chunk_log2 = 13;
@@ -450,6 +463,7 @@ extern "C" int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offset
     p.nwave_mats = (int64_t)wave_idx.size();
     p.nsmall = (int64_t)small_idx.size();
     p.single_n = (nmin == nmax) ? nmax : 0;
+    p.variant = p.single_n ? variant : 0;
 
     // One device block: items | cursor | partials | wave arrays | small arrays.
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -501,8 +515,8 @@ extern "C" int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offset
     }
 
     if (p.nitems > 0) {
-        const void *kp = p.single_n ? single_kernel_ptr<PERM_WAVE_MIN>(p.single_n)
-                                    : reinterpret_cast<const void *>(&perm_wave_kernel<PERM_WAVE_MIN, PERM_WAVE_MAX>);
+        const void *kp = p.single_n ? single_kernel_ptr<PERM_WAVE_MIN>(p.single_n, p.variant)
+                                    : reinterpret_cast<const void *>(&perm_wave_kernel<PERM_WAVE_MIN, PERM_WAVE_MAX, 0>);
         int rw = 0;
         int rc = resident_waves(kp, &rw);
         if (rc != OK) { (void)hipFree(p.d_block); delete plan; return rc; }
@@ -510,6 +524,12 @@ extern "C" int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offset
     }
     *out = plan;
     return OK;
+}
+
+extern "C" int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                                     int max_chunk_log2, pcvl_perm_plan *out)
+{
+    return pcvl_perm_plan_create_ex(h_n, h_offsets, count, max_chunk_log2, 0, out);
 }
 
 extern "C" int pcvl_perm_plan_execute(pcvl_perm_plan plan, const double *d_mats, double *d_out, hipStream_t stream)
@@ -527,7 +547,7 @@ extern "C" int pcvl_perm_plan_execute(pcvl_perm_plan plan, const double *d_mats,
     }
     if (p.nwave_mats > 0) {
         int rc = p.single_n ? launch_single<PERM_WAVE_MIN>(&p, d_mats, stream)
-                            : launch_wave<PERM_WAVE_MIN, PERM_WAVE_MAX>(&p, d_mats, stream);
+                            : launch_wave<PERM_WAVE_MIN, PERM_WAVE_MAX, 0>(&p, d_mats, stream);
         if (rc != OK) return rc;
         int blocks = (int)((p.nwave_mats + 255) / 256);
         hipLaunchKernelGGL(perm_finalize_kernel, dim3(blocks), dim3(256), 0, stream, p.d_wave_idx,

@@ -188,7 +188,7 @@ class LearnedMIS:
     """Independence Metropolis chain with a ProposalNet proposal and FP64
     permanent targets, for a fixed unitary and single-photon input."""
 
-    def __init__(self, U, in_occ, net: ProposalNet, chunk: int = 16384):
+    def __init__(self, U, in_occ, net: ProposalNet, chunk: int = 16384, multiset: bool = True):
         import torch
         self.in_occ = list(in_occ)
         self.m = len(self.in_occ)
@@ -200,9 +200,20 @@ class LearnedMIS:
         self.cols = torch.tensor(cols, dtype=torch.int32, device="cuda")
         self.log_prod_in = float(sum(math.lgamma(v + 1) for v in self.in_occ))
         self.chunk = chunk
-        self._plan = PermanentPlan(np.full(chunk, self.n, dtype=np.int32))
-        self._mats = torch.empty(chunk * self.n * self.n, dtype=torch.complex128, device="cuda")
+        self.multiset = multiset
         self._perms = torch.empty(chunk, dtype=torch.complex128, device="cuda")
+        if not multiset:
+            self._plan = PermanentPlan(np.full(chunk, self.n, dtype=np.int32))
+            self._mats = torch.empty(chunk * self.n * self.n, dtype=torch.complex128, device="cuda")
+
+    def permanents(self, seqs):
+        """perm(U[seq, cols]) of sorted sequences (B x n), complex128 CUDA tensor."""
+        import torch
+        B = seqs.shape[0]
+        out = torch.empty(B, dtype=torch.complex128, device="cuda")
+        _native.check(_lib().pcvl_mis_permanents(_p(self.U), self.m, _p(self.cols), self.n, _p(seqs.contiguous()),
+                                                 B, _p(out), _s()), "mis_permanents")
+        return out
 
     def log_p(self, seqs):
         """Exact log p of sorted sequences (B x n), by FP64 permanents."""
@@ -213,11 +224,16 @@ class LearnedMIS:
         for i0 in range(0, B, self.chunk):
             sb = seqs[i0:i0 + self.chunk].contiguous()
             b = sb.shape[0]
-            _native.check(lib.pcvl_mis_submatrices(_p(self.U), self.m, _p(self.cols), self.n, _p(sb), b,
-                                                   _p(self._mats), _s()), "submatrices")
-            if b < self.chunk:   # the plan covers a full chunk; pad with the last matrix
-                self._mats[b * self.n * self.n:].zero_()
-            self._plan(self._mats, self._perms)
+            if self.multiset:
+                # repeated rows summed by multiplicity (multiset Glynn)
+                _native.check(lib.pcvl_mis_permanents(_p(self.U), self.m, _p(self.cols), self.n, _p(sb), b,
+                                                      _p(self._perms), _s()), "mis_permanents")
+            else:
+                _native.check(lib.pcvl_mis_submatrices(_p(self.U), self.m, _p(self.cols), self.n, _p(sb), b,
+                                                       _p(self._mats), _s()), "submatrices")
+                if b < self.chunk:   # the plan covers a full chunk; zero the unused matrices
+                    self._mats[b * self.n * self.n:].zero_()
+                self._plan(self._mats, self._perms)
             _native.check(lib.pcvl_mis_logp(_p(self._perms), _p(sb), self.n, b, self.log_prod_in, _p(out[i0:i0 + b]),
                                             _s()), "logp")
         return out
@@ -242,7 +258,8 @@ class LearnedMIS:
         return {"seq": out, "acceptance": float(acc.item()) / max(1, count), "logw": logw, "proposals": props}
 
     def close(self):
-        self._plan.close()
+        if not self.multiset:
+            self._plan.close()
 
 
 # --------------------------------------------------------------- diagnostics

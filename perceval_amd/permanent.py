@@ -21,7 +21,7 @@ MAX_ORDER = 28
 class PermanentPlan:
     """Launch plan for a fixed list of matrix orders."""
 
-    def __init__(self, orders, max_chunk_log2: int = 0):
+    def __init__(self, orders, max_chunk_log2: int = 0, variant: int = 0):
         lib = _native.load()
         self.orders = np.ascontiguousarray(np.asarray(orders, dtype=np.int32).ravel())
         if self.orders.size and (self.orders.min() < 0 or self.orders.max() > MAX_ORDER):
@@ -32,10 +32,11 @@ class PermanentPlan:
             self.offsets[1:] = np.cumsum(sizes)[:-1]
         self.total_elems = int(sizes.sum())
         self._plan = ctypes.c_void_p()
-        _native.check(lib.pcvl_perm_plan_create(
+        _native.check(lib.pcvl_perm_plan_create_ex(
             self.orders.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
             self.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-            len(self.orders), int(max_chunk_log2), ctypes.byref(self._plan)), "pcvl_perm_plan_create")
+            len(self.orders), int(max_chunk_log2), int(variant), ctypes.byref(self._plan)),
+            "pcvl_perm_plan_create_ex")
         self._lib = lib
 
     def info(self) -> dict:

@@ -1,0 +1,61 @@
+"""HIP-graph capture of plan executions and the permanent autotuner."""
+import numpy as np
+import pytest
+
+import oracle
+from perceval_amd.utils.workloads import submatrix_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_graph_replays_match_eager(gpu):
+    import torch
+    from perceval_amd.graphs import graphed_permanents
+    from perceval_amd.permanent import PermanentPlan, pack
+    _, orders, mats = submatrix_batch(256, 9, 18, seed=3)
+    o, flat = pack(mats)
+    plan = PermanentPlan(o)
+    d = torch.from_numpy(flat).cuda()
+    eager = plan(d).clone()
+    out = torch.zeros_like(eager)
+    g = graphed_permanents(plan, d, out)
+    for rep in range(4):
+        out.zero_()
+        _ = torch.randn(1000, device="cuda").sum().item()   # unrelated eager work between replays
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), f"replay {rep} differs"
+    ref = oracle.perm_batch(mats[:8], precise=True)
+    assert np.max(np.abs(eager[:8].cpu().numpy() - ref) / np.abs(ref)) < 1e-10
+    plan.close()
+
+
+def test_graph_slos_replays(gpu):
+    import torch
+    from perceval_amd.graphs import GraphedCall
+    from perceval_amd.slos import PROBS, SlosPlan
+    from perceval_amd.utils.workloads import haar_unitary
+    plan = SlosPlan([1, 1, 1, 0, 0, 0, 0])
+    dU = torch.from_numpy(haar_unitary(7, seed=1)).cuda()
+    out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
+    eager = plan.run(dU, PROBS).clone()
+    plan._workspace(dU.device)
+    g = GraphedCall(lambda: plan.run(dU, PROBS, out))
+    for _ in range(3):
+        out.fill_(-1)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager)
+
+
+def test_autotune_picks_a_measured_config(gpu, tmp_path, monkeypatch):
+    import perceval_amd.autotune as at
+    monkeypatch.setattr(at, "_CACHE_PATH", str(tmp_path / "tune.json"))
+    res = at.tune_permanent(16, batch=512, reps=2)
+    assert res["variant"] in at.VARIANTS and res["chunk_log2"] <= 9
+    assert res["ms"] == min(t["ms"] for t in res["trials"])
+    again = at.tune_permanent(16, batch=512)
+    assert again["variant"] == res["variant"] and again["chunk_log2"] == res["chunk_log2"]
+    plan = at.tuned_plan([16] * 512)
+    assert plan.info()["single_order"] == 16
+    plan.close()

@@ -103,6 +103,27 @@ def test_log_p_vs_oracle(gpu):
     mis.close()
 
 
+@pytest.mark.parametrize("m,n", [(6, 5), (12, 8), (60, 20)])
+def test_multiset_permanents_vs_oracle(gpu, m, n):
+    import torch
+    from perceval_amd.mis import LearnedMIS, ProposalNet
+    U = haar_unitary(m, seed=m + n)
+    inp = [1] * n + [0] * (m - n)
+    rng = np.random.RandomState(n)
+    # sequences with many collisions, a few without, and all photons in one mode
+    seqs = [np.sort(rng.randint(0, max(2, m // 3), size=n)) for _ in range(12)]
+    seqs += [np.sort(rng.choice(m, n, replace=False)) for _ in range(3)]
+    seqs += [np.full(n, m - 1), np.array([0] * (n // 2) + [1] * (n - n // 2))]
+    seqs = np.array(seqs, dtype=np.int32)
+    mis = LearnedMIS(U, inp, ProposalNet(m, n, hidden=8), chunk=64)
+    got = mis.permanents(torch.from_numpy(seqs).cuda()).cpu().numpy()
+    for s, g in zip(seqs, got):
+        occ = np.bincount(s, minlength=m)
+        ref = oracle.perm(oracle.submatrix(U, inp, occ))
+        assert abs(g - ref) <= 1e-10 * abs(ref) + 1e-14, (s, g, ref)
+    mis.close()
+
+
 def test_chain_scan_vs_host_restatement(gpu):
     import torch
     from perceval_amd import _native
