@@ -161,18 +161,23 @@ def bench_slos(args, res):
     plan = SlosPlan(inp)
     dU = torch.from_numpy(U).cuda()
     out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
-    plan.run(dU, PROBS, out)
     reps = 5
-    with Timer() as t:
-        for _ in range(reps):
-            plan.run(dU, PROBS, out)
-    ms = t.ms / reps
+    per_kernel = {}
+    for kernel in (2, 1):   # incremental runs, then one thread per output state (the default)
+        plan.set_kernel(kernel)
+        plan.run(dU, PROBS, out)
+        with Timer() as t:
+            for _ in range(reps):
+                plan.run(dU, PROBS, out)
+        per_kernel[kernel] = t.ms / reps
+    plan.set_kernel(0)
+    ms = per_kernel[1]
     total = float(out.sum().item())
     plan.close()
     gbytes = slos_bytes(m, n) / 1e9
     res["slos_m24_n10"] = {"ms": ms, "outputs": plan.count, "algorithmic_GB": gbytes,
                            "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,
-                           "sum_probs": total}
+                           "sum_probs": total, "ms_incremental_kernel": per_kernel[2]}
     log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}")
     if not args.no_cpu:
         import oracle

@@ -52,8 +52,9 @@ int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offsets, int64_t 
 
 /* As pcvl_perm_plan_create, with the kernel variant of single-order batches:
  * 0 picks by order, 1 keeps Gray-code row 1 in VGPRs, 2 reads it from LDS
- * (fewer VGPRs, more resident waves).  Mixed-order batches ignore it.  Used
- * by the autotuner. */
+ * (fewer VGPRs, more resident waves); 3 and 4 are 1 and 2 with each row flip
+ * fused into the preceding term.  Mixed-order batches ignore it.  Used by the
+ * autotuner. */
 int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
                              int max_chunk_log2, int variant, pcvl_perm_plan *out);
 
@@ -125,6 +126,11 @@ int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, int mode, dou
                            void *d_work, size_t work_bytes, hipStream_t stream);
 
 int pcvl_slos_plan_destroy(pcvl_slos_plan plan);
+
+/* Kernel used by execute: 0 picks (currently 1), 1 one thread per output
+ * state, 2 incremental runs of consecutive output states (layers below 2^32
+ * states and 64 KiB of LDS only; -1 otherwise).  For comparisons. */
+int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel);
 
 /* ---------------------------------------------------------------------------
  * Exact boson sampling -- replaces xq.Clifford2017.samples

@@ -52,6 +52,7 @@ SIGNATURES = {
     "pcvl_slos_plan_info": (c_int, [c_void_p, P(c_int), P(c_i64), P(c_size_t), P(c_i32)]),
     "pcvl_slos_plan_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pcvl_slos_plan_destroy": (c_int, [c_void_p]),
+    "pcvl_slos_plan_set_kernel": (c_int, [c_void_p, c_int]),
     "pcvl_cc_workspace_bytes": (c_int, [c_int, c_i64, P(c_size_t)]),
     "pcvl_mlp_gemm": (c_int, [c_void_p, c_i64, c_i64, c_void_p, c_int, c_i64, c_i64, c_void_p, c_i64, c_int, c_int,
                               c_int, c_void_p, c_int, c_int, c_void_p]),

@@ -1,7 +1,8 @@
 """Autotuner for the batched permanent kernel.
 
 For a batch shape (order n, batch size) it times every kernel variant (row 1
-of the Gray code in VGPRs or in LDS) and lane-chunk length that the plan API
+of the Gray code in VGPRs or in LDS; row flips as a separate phase or fused
+into the product) and lane-chunk length that the plan API
 accepts, on the current GPU, and keeps the fastest.  Results are cached in a
 JSON file keyed by device name, order and batch size, so tuning runs once per
 machine.
@@ -17,7 +18,7 @@ from .permanent import PermanentPlan
 
 _CACHE_PATH = os.environ.get("PCVL_AUTOTUNE_CACHE",
                              os.path.join(os.path.expanduser("~"), ".cache", "perceval_amd", "autotune.json"))
-VARIANTS = (1, 2)
+VARIANTS = (1, 2, 3, 4)
 CHUNKS = (8, 10, 11, 12, 13, 14)
 
 

@@ -95,6 +95,48 @@ __device__ __forceinline__ void flip_lds(cd *cs, const cd *__restrict__ row, dou
     }
 }
 
+// c += s * a, placed after dep: the empty asm makes the update wait for the
+// product that consumed the old c, so both generations are never live.
+__device__ __forceinline__ void flip_one(cd &c, const cd &a, double s, const cd &dep)
+{
+    // Not volatile: a volatile asm has unmodelled side effects, which would
+    // also stop the scheduler from issuing LDS reads of the row above it.
+    asm("" : "+v"(c.r), "+v"(c.i) : "v"(dep.r), "v"(dep.i));
+    c.r = fma(s, a.r, c.r);
+    c.i = fma(s, a.i, c.i);
+}
+
+// Returns v, made to look dependent on dep: keeps loads at an offset of v
+// inside the loop that dep counts (they would otherwise be hoisted as loop
+// invariant, holding whole rows in VGPRs).
+__device__ __forceinline__ int loop_variant(int v, uint32_t dep)
+{
+    asm("" : "+s"(v) : "s"(dep));
+    return v;
+}
+
+// One Gray-code term (added, or subtracted when SUB) fused with the flip to the
+// next step: column j is flipped as soon as its product chain has consumed it,
+// so row reads from LDS spread over the product's arithmetic instead of
+// forming a separate latency-bound phase.
+template <int N, bool SUB>
+__device__ __forceinline__ void term_flip(cd &acc, cd *cs, const cd *row, double s)
+{
+    static_assert(N >= 9, "every chain needs a second factor");
+    cd p0 = cs[0], p1 = cs[1], p2 = cs[2], p3 = cs[3];
+#pragma unroll
+    for (int j = 4; j < N - 1; j++) {
+        cd &pk = ((j & 3) == 0) ? p0 : ((j & 3) == 1) ? p1 : ((j & 3) == 2) ? p2 : p3;
+        pk = cmul(pk, cs[j]);
+        if (j < 8) flip_one(cs[j - 4], row[j - 4], s, pk);
+        flip_one(cs[j], row[j], s, pk);
+    }
+    const cd h = cmul(cmul(p0, p1), cmul(p2, p3));
+    if (SUB) cfms_acc(acc, h, cs[N - 1]);
+    else cfma_acc(acc, h, cs[N - 1]);
+    flip_one(cs[N - 1], row[N - 1], s, acc);
+}
+
 // --------------------------------------------------------------- wave path
 
 // One wave item: 64 lanes x 2^c steps of one order-N matrix.  sA is this
@@ -133,10 +175,12 @@ __device__ __forceinline__ cd glynn_wave_item(const double *__restrict__ mats, c
 #pragma unroll
     for (int j = 0; j < N; j++) cs[j] = {0.5 * cs[j].r, 0.5 * cs[j].i};
 
-    // Row 1 flips every other step: keep it in VGPRs (R1 = 1), read it from
-    // LDS like the other rows (R1 = 2, fewer VGPRs and more waves), or pick
-    // by order (R1 = 0).
-    constexpr bool kRow1InRegs = (R1 == 1) || (R1 == 0 && N <= 22);
+    // Row 1 flips every other step: keep it in VGPRs (R1 = 1, 3), read it
+    // from LDS like the other rows (R1 = 2, 4: fewer VGPRs and more waves), or
+    // pick by order (R1 = 0).  R1 = 3, 4 fuse each flip into the preceding
+    // term.
+    constexpr bool kRow1InRegs = (R1 == 1) || (R1 == 3) || (R1 == 0 && N <= 22);
+    constexpr bool kFused = (R1 >= 3);
     cd r1[kRow1InRegs ? N : 1];
     if constexpr (kRow1InRegs) {
 #pragma unroll
@@ -144,6 +188,24 @@ __device__ __forceinline__ cd glynn_wave_item(const double *__restrict__ mats, c
     }
 
     cd acc = {0.0, 0.0};
+    if constexpr (kFused) {
+        for (uint64_t u = 0; u < steps; u += 4) {
+            const cd *row1 = kRow1InRegs ? r1 : sA + loop_variant(N, (uint32_t)u);
+            term_flip<N, false>(acc, cs, row1, -1.0);          // t = base+u, then row 1 -> delta -1
+            uint64_t t2 = base + u + 2;
+            const double s2 = (((t2 >> 1) ^ (t2 >> 2)) & 1) ? -1.0 : 1.0;
+            term_flip<N, true>(acc, cs, sA + loop_variant(2 * N, (uint32_t)u), s2);   // t+1, then row 2
+            term_flip<N, false>(acc, cs, row1, 1.0);           // t+2, then row 1 -> delta +1
+            // t+3, then row 1 + ctz(u+4) >= 3 (on the last block: first step of
+            // the next chunk, row c+1 <= N-6, result discarded)
+            const int b = __builtin_ctzll(u + 4);
+            uint64_t t4 = base + u + 4;
+            const double s4 = (((t4 >> b) ^ (t4 >> (b + 1))) & 1) ? -1.0 : 1.0;
+            term_flip<N, true>(acc, cs, sA + (b + 1) * N, s4);
+        }
+        __syncthreads();
+        return acc;
+    }
     // Steps are taken four at a time; c >= 2 so base has two zero low bits.
     for (uint64_t u = 0; u < steps; u += 4) {
         term_add<N>(acc, cs);                 // t = base+u, even
@@ -329,7 +391,7 @@ struct PermPlanImpl {
     int64_t count = 0;
     int64_t nitems = 0, nwave_mats = 0, nsmall = 0;
     int single_n = 0;          // order shared by every wave item, or 0 if mixed
-    int variant = 0;           // 0 auto, 1 row 1 in VGPRs, 2 row 1 in LDS (single-order plans)
+    int variant = 0;           // 0 auto, 1..4 (single-order plans): row 1 in VGPRs / LDS, separate / fused flips
     int grid = 0;
     PermItem *d_items = nullptr;
     unsigned long long *d_cursor = nullptr;
@@ -370,6 +432,8 @@ static int launch_single(PermPlanImpl *p, const double *mats, hipStream_t st)
         if (p->single_n == N) {
             if (p->variant == 1) return launch_wave<N, N, 1>(p, mats, st);
             if (p->variant == 2) return launch_wave<N, N, 2>(p, mats, st);
+            if (p->variant == 3) return launch_wave<N, N, 3>(p, mats, st);
+            if (p->variant == 4) return launch_wave<N, N, 4>(p, mats, st);
             return launch_wave<N, N, 0>(p, mats, st);
         }
         return launch_single<N + 1>(p, mats, st);
@@ -385,6 +449,8 @@ static const void *single_kernel_ptr(int n, int variant)
         if (n == N) {
             if (variant == 1) return reinterpret_cast<const void *>(&perm_wave_kernel<N, N, 1>);
             if (variant == 2) return reinterpret_cast<const void *>(&perm_wave_kernel<N, N, 2>);
+            if (variant == 3) return reinterpret_cast<const void *>(&perm_wave_kernel<N, N, 3>);
+            if (variant == 4) return reinterpret_cast<const void *>(&perm_wave_kernel<N, N, 4>);
             return reinterpret_cast<const void *>(&perm_wave_kernel<N, N, 0>);
         }
         return single_kernel_ptr<N + 1>(n, variant);
@@ -414,7 +480,7 @@ extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_off
                                         int max_chunk_log2, int variant, pcvl_perm_plan *out)
 {
     PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
-    PCVL_ARG_CHECK(variant >= 0 && variant <= 2, "variant outside [0, 2]");
+    PCVL_ARG_CHECK(variant >= 0 && variant <= 4, "variant outside [0, 4]");
     PCVL_ARG_CHECK(count >= 0, "count < 0");
     PCVL_ARG_CHECK(count == 0 || (h_n && h_offsets), "null order/offset array");
     if (max_chunk_log2 <= 0) max_chunk_log2 = 13;

@@ -39,6 +39,10 @@ class SlosPlan:
         self.path = path[:self.n].tolist()
         self._work = None
 
+    def set_kernel(self, kernel: int):
+        """0 = pick, 1 = one thread per output state, 2 = incremental runs."""
+        _native.check(self._lib.pcvl_slos_plan_set_kernel(self._plan, int(kernel)), "pcvl_slos_plan_set_kernel")
+
     def _workspace(self, device):
         import torch
         if self._work is None or self._work.device != torch.device(device):

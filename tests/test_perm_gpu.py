@@ -60,6 +60,23 @@ def test_wave_orders_vs_oracle(gpu, n):
         assert _rel(g, r) < REL_TOL, (n, g, r)
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("n", [9, 13, 20, 24])
+def test_kernel_variants_vs_oracle(gpu, n, variant):
+    import torch
+    from perceval_amd.permanent import PermanentPlan, pack
+    U = haar_unitary(64, seed=300 + n)
+    rng = np.random.RandomState(7 * n)
+    mats = [U[np.ix_(rng.choice(64, n, replace=False), rng.choice(64, n, replace=False))] for _ in range(3)]
+    orders, flat = pack(mats)
+    for chunk in (2, 11):
+        plan = PermanentPlan(orders, max_chunk_log2=chunk, variant=variant)
+        got = plan(torch.from_numpy(flat).cuda()).cpu().numpy()
+        plan.close()
+        for g, r in zip(got, _oracle(mats)):
+            assert _rel(g, r) < REL_TOL, (n, variant, chunk, g, r)
+
+
 def test_mixed_batch_config_shape(gpu):
     # Same generator as the batched-permanent workload (n uniform in 12..24 from
     # one 64x64 Haar unitary), 4096 matrices; every matrix of order <= 16 and a

@@ -44,11 +44,16 @@ def test_fock_rank_unrank_m24_n10_sampled(gpu):
     assert (bad == -1).all()
 
 
+KERNELS = (1, 2)   # one thread per output state; incremental runs
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("inp", [[1, 1, 1, 1, 0, 0, 0, 0], [2, 0, 1, 0, 0, 1, 0, 0], [0, 0, 0, 0, 0, 0, 0, 4]])
-def test_slos_config1_vs_oracle(gpu, inp):
+def test_slos_config1_vs_oracle(gpu, inp, kernel):
     from perceval_amd.slos import COEFS, PROBS, SlosPlan
     U = haar_unitary(8, seed=42)
     plan = SlosPlan(inp)
+    plan.set_kernel(kernel)
     assert plan.path == oracle.slos_path(inp)
     probs = plan.run(U, PROBS).cpu().numpy()
     coefs = plan.run(U, COEFS).cpu().numpy()
@@ -70,23 +75,53 @@ def test_slos_reference_unitary_u_random_8(gpu):
     assert np.max(np.abs(p - exact)) < ABS_TOL
 
 
-def test_slos_m12_n6_full_vs_oracle(gpu):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_slos_m12_n6_full_vs_oracle(gpu, kernel):
     from perceval_amd.slos import PROBS, SlosPlan
     U = haar_unitary(12, seed=7)
     inp = [1] * 6 + [0] * 6
-    p = SlosPlan(inp).run(U, PROBS).cpu().numpy()
+    plan = SlosPlan(inp)
+    plan.set_kernel(kernel)
+    p = plan.run(U, PROBS).cpu().numpy()
     ref = oracle.slos_probs(U, inp, "gather")
     assert len(p) == 12376
     assert np.max(np.abs(p - ref)) < ABS_TOL
 
 
-def test_slos_m24_n10_full_size(gpu):
+@pytest.mark.parametrize("inp", [[1, 1, 0, 2, 0, 0, 1, 0, 3, 0, 0, 1], [0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 9],
+                                 [5, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0], [1] * 12])
+def test_slos_incremental_matches_per_state_kernel(gpu, inp):
+    """Both kernels on bunched inputs (long runs through the last mode)."""
+    from perceval_amd.slos import COEFS, SlosPlan
+    U = haar_unitary(12, seed=sum(inp))
+    plan = SlosPlan(inp)
+    plan.set_kernel(1)
+    a = plan.run(U, COEFS).cpu().numpy()
+    plan.set_kernel(2)
+    b = plan.run(U, COEFS).cpu().numpy()
+    ref = oracle.slos_coefs(U, inp)
+    assert np.max(np.abs(a - ref)) < ABS_TOL and np.max(np.abs(b - ref)) < ABS_TOL
+    assert np.max(np.abs(a - b)) < 1e-14
+    plan.close()
+
+
+def test_slos_set_kernel_rejects_bad_values(gpu):
+    from perceval_amd.slos import SlosPlan
+    plan = SlosPlan([1, 1, 0])
+    with pytest.raises(ValueError):
+        plan.set_kernel(3)
+    plan.close()
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_slos_m24_n10_full_size(gpu, kernel):
     import torch
     from perceval_amd.slos import PROBS, SlosPlan
     U = haar_unitary(24, seed=24)
     inp = [1] * 10 + [0] * 14
     plan = SlosPlan(inp)
-    _say(f"plan: {plan.count} outputs, workspace {plan.work_bytes / 2**20:.0f} MiB")
+    plan.set_kernel(kernel)
+    _say(f"kernel {kernel} plan: {plan.count} outputs, workspace {plan.work_bytes / 2**20:.0f} MiB")
     p = plan.run(U, PROBS)
     torch.cuda.synchronize()
     _say("all_prob done")
