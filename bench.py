@@ -16,6 +16,12 @@ configurations are reported beside it:
   sampling      m = 60, n = 20: plain GPU Clifford-Clifford samples/s, the
                 learned Metropolis independence sampler's samples/s, and the
                 1- and 2-mode marginal TVD between the two sample sets
+  gurvits       NaiveApprox: Gurvits estimates, 64 order-30 matrices x 10^6
+                samples (samples/s)
+  copy          1 GiB device copy: the HBM rate a copy attains, and the
+                reference that scales FETCH_SIZE / WRITE_SIZE readings
+The perm section also times a pure-FMA FP64 probe, the vector rate the chip
+sustains at the clock it holds under FP64 load.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--sections ...]
 
@@ -39,7 +45,7 @@ if ROOT not in sys.path:
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 vector, FMA = 2 flops
 HBM_PEAK_GBS = 8000.0
-SECTIONS = ("perm", "mixed", "slos", "gradient", "sampling")
+SECTIONS = ("perm", "mixed", "slos", "gradient", "sampling", "gurvits", "copy")
 
 
 def glynn_flops(n: int) -> float:
@@ -98,6 +104,90 @@ def time_plan(plan, d_mats, out, steps: int, warmup: int):
 
 # ------------------------------------------------------------------ sections
 
+def fp64_probe_tflops() -> float:
+    """Attainable vector FP64 rate: pure FMA chains (pcvl_probe_fp64_fma),
+    8 waves per SIMD, timed with HIP events on the launch stream."""
+    import ctypes
+    import torch
+    from perceval_amd import _native
+    lib = _native.load()
+    blocks = torch.cuda.get_device_properties(0).multi_processor_count * 8
+    out = torch.empty(blocks * 256, dtype=torch.float64, device="cuda")
+    flops = ctypes.c_int64()
+    iters = 100000
+
+    def launch():
+        _native.check(lib.pcvl_probe_fp64_fma(blocks, iters, _native.ptr(out), ctypes.byref(flops),
+                                              _native.stream_handle()), "probe")
+    launch()
+    with Timer() as t:
+        for _ in range(3):
+            launch()
+    if not bool(torch.isfinite(out).all()):
+        raise RuntimeError("FP64 probe produced non-finite values")
+    return flops.value / (t.ms / 3 * 1e-3) / 1e12
+
+
+def bench_copy(args, res):
+    """Plain 1 GiB device-to-device copy, three times: the reference point that
+    scales FETCH_SIZE / WRITE_SIZE counter readings to bytes, and the HBM rate
+    a copy attains."""
+    import torch
+    nbytes = 1 << 30
+    x = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    with Timer() as t:
+        for _ in range(3):
+            y.copy_(x)
+    ms = t.ms / 3
+    res["copy_1GiB"] = {"ms": ms, "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}
+    log(f"[bench] copy 1 GiB: {ms:.3f} ms, {2 * nbytes / (ms * 1e-3) / 1e9:.0f} GB/s (read + write)")
+
+
+def bench_gurvits(args, res):
+    """NaiveApprox: Gurvits estimates of 64 order-30 submatrices of a 60-mode
+    Haar unitary (the size of the reference's NaiveApprox example), 10^6
+    samples each, matrices resident in HBM."""
+    import ctypes
+    import torch
+    from perceval_amd import _native
+    from perceval_amd.permanent import pack
+    from perceval_amd.utils.workloads import haar_unitary
+    lib = _native.load()
+    m, n, B, iters = 60, 30, 64, 1_000_000
+    U = haar_unitary(m, seed=61)
+    rng = np.random.RandomState(3)
+    mats = [np.ascontiguousarray(U[np.sort(rng.choice(m, n, replace=False))][:, :n]) for _ in range(B)]
+    orders, flat = pack(mats)
+    offsets = (np.arange(B, dtype=np.int64) * n * n)
+    d = torch.from_numpy(flat).cuda()
+    est = torch.empty(B, dtype=torch.complex128, device="cuda")
+    err = torch.empty(B, dtype=torch.float64, device="cuda")
+
+    def run():
+        _native.check(lib.pcvl_gurvits_batch(_native.ptr(d), orders.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                             offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), B, iters, 5, 0,
+                                             _native.ptr(est), _native.ptr(err), _native.stream_handle()), "gurvits")
+    run()
+    with Timer() as t:
+        run()
+    rate = B * iters / (t.ms * 1e-3)
+    flops = B * iters * (4 * n * n + 6 * (n - 1)) / (t.ms * 1e-3) / 1e12
+    res["gurvits_m60_n30"] = {"ms": t.ms, "matrices": B, "samples_per_matrix": iters, "samples_per_s": rate,
+                              "tflops": flops, "rel_err_bound_mean": float((err / est.abs()).mean().item())}
+    log(f"[bench] Gurvits n=30: {B} x {iters} samples in {t.ms:.1f} ms = {rate:.3e} samples/s ({flops:.1f} TFLOP/s)")
+    if not args.no_cpu:
+        import oracle
+        k = 20000
+        t0 = time.perf_counter()
+        oracle.gurvits(mats[0], k, seed=5)
+        dt = time.perf_counter() - t0
+        res["gurvits_cpu"] = {"value": k / dt, "unit": "samples/s", "cores": 1, "kind": "port",
+                              "sample": f"{k} samples of one order-30 matrix, oracle (oracle/pcvl_oracle.c), 1 thread"}
+        log(f"[bench] Gurvits CPU: {k / dt:.0f} samples/s on 1 thread")
+
+
 def bench_perm(args, res):
     import torch
     from perceval_amd.permanent import PermanentPlan, pack
@@ -116,6 +206,11 @@ def bench_perm(args, res):
     res["perm"] = {"ms": ms, "perms_per_s": len(o) / (ms * 1e-3), "tflops": achieved}
     log(f"[bench] perm: {ms:.3f} ms/step, {len(o) / (ms * 1e-3):.1f} perms/s, {achieved:.2f} TFLOP/s "
         f"({100 * achieved / FP64_PEAK_TFLOPS:.1f}% of FP64 peak)")
+    probe = fp64_probe_tflops()
+    res["perm"]["fp64_fma_probe_tflops"] = probe
+    res["perm"]["frac_of_probe"] = achieved / probe
+    log(f"[bench] FP64 FMA probe: {probe:.2f} TFLOP/s attainable; permanent kernel at {100 * achieved / probe:.1f}% "
+        f"of it (instruction ceiling of Glynn at n={n}: {100 * (8 * n - 4) / (2 * (6 * n - 4)):.1f}%)")
     if not args.no_cpu:
         import oracle
         th = cpu_threads()
@@ -163,21 +258,23 @@ def bench_slos(args, res):
     out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
     reps = 5
     per_kernel = {}
-    for kernel in (2, 1):   # incremental runs, then one thread per output state (the default)
+    # 1 per state, 2 incremental runs, 3 per state with 32-bit ranks, 0 the
+    # library's own pick (the headline number)
+    for kernel in (1, 2, 3, 0):
         plan.set_kernel(kernel)
         plan.run(dU, PROBS, out)
         with Timer() as t:
             for _ in range(reps):
                 plan.run(dU, PROBS, out)
         per_kernel[kernel] = t.ms / reps
-    plan.set_kernel(0)
-    ms = per_kernel[1]
+    ms = per_kernel[0]
     total = float(out.sum().item())
     plan.close()
     gbytes = slos_bytes(m, n) / 1e9
     res["slos_m24_n10"] = {"ms": ms, "outputs": plan.count, "algorithmic_GB": gbytes,
                            "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,
-                           "sum_probs": total, "ms_incremental_kernel": per_kernel[2]}
+                           "sum_probs": total, "ms_by_kernel": {"per_state": per_kernel[1], "incremental": per_kernel[2],
+                                                 "per_state_32bit": per_kernel[3]}}
     log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}")
     if not args.no_cpu:
         import oracle
@@ -262,11 +359,16 @@ def bench_sampling(args, res):
     occ_mis = seq_to_occ(out["seq"], m).cpu().numpy()
     tv = marginal_tvds(occ_mis, ref.cpu().numpy())
     tv_cc = marginal_tvds(train.cpu().numpy(), ref.cpu().numpy())  # noise floor of two independent CC sets
+    # A chain that (almost) never moves reports a high rate but samples nothing
+    # useful; the speed-up only counts when the marginals match to 0.01.
+    meets = bool(acc >= 0.01 and tv["tvd1_max"] <= 0.01)
     res["sampling_m60_n20"] = {
         "samples": N, "cc_samples_per_s": N / (cc_ms * 1e-3), "mis_samples_per_s": N / (mis_ms * 1e-3),
         "mis_vs_cc": (N / (mis_ms * 1e-3)) / (N / (cc_ms * 1e-3)), "mis_acceptance": acc,
-        "tvd_mis_vs_cc": tv, "tvd_cc_vs_cc": tv_cc, "train_seconds": train_s,
+        "mis_meets_tvd_0.01": meets, "tvd_mis_vs_cc": tv, "tvd_cc_vs_cc": tv_cc, "train_seconds": train_s,
         "train_nll": [float(h) for h in hist]}
+    if not meets:
+        log(f"[bench] MIS marginals do not meet TVD <= 0.01 (acceptance {acc:.3f}); the speed-up does not count")
     log(f"[bench] MIS: {N} samples in {mis_ms:.1f} ms = {N / (mis_ms * 1e-3):.0f} samples/s "
         f"({(N / (mis_ms * 1e-3)) / (N / (cc_ms * 1e-3)):.2f}x plain CC), acceptance {acc:.3f}, TVD {tv}, "
         f"CC-vs-CC floor {tv_cc}")
@@ -311,7 +413,8 @@ def main():
         sections.remove("mixed")
     res = {}
     for s in sections:
-        {"perm": bench_perm, "mixed": bench_mixed, "slos": bench_slos, "gradient": bench_gradient,
+        {"perm": bench_perm, "mixed": bench_mixed, "slos": bench_slos, "gradient": bench_gradient, "copy": bench_copy,
+         "gurvits": bench_gurvits,
          "sampling": bench_sampling}[s](args, res)
 
     perm = res.get("perm", {"ms": float("nan"), "perms_per_s": float("nan"), "tflops": float("nan")})

@@ -46,7 +46,9 @@ int pcvl_abi_version(void);
 typedef struct pcvl_perm_plan_s *pcvl_perm_plan;
 
 /* Build a reusable launch plan (work queue, partial-sum slots) for a batch
- * shape.  max_chunk_log2 <= 0 selects the default lane chunk (2^13 steps). */
+ * shape.  max_chunk_log2 <= 0 selects the default lane chunk (2^11 steps);
+ * the matrices that make up the last round of the queue get chunks 4x
+ * shorter. */
 int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
                           int max_chunk_log2, pcvl_perm_plan *out);
 
@@ -127,9 +129,11 @@ int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, int mode, dou
 
 int pcvl_slos_plan_destroy(pcvl_slos_plan plan);
 
-/* Kernel used by execute: 0 picks (currently 1), 1 one thread per output
- * state, 2 incremental runs of consecutive output states (layers below 2^32
- * states and 64 KiB of LDS only; -1 otherwise).  For comparisons. */
+/* Kernel used by execute: 0 picks (3 when every layer is below 2^32 states,
+ * else 1), 1 one thread per output state, 2 incremental runs of consecutive
+ * output states (layers below 2^32 states and 64 KiB of LDS only), 3 one
+ * thread per output state with 32-bit ranks (layers below 2^32 states).
+ * Kernels not applicable to the plan give -1.  For comparisons. */
 int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel);
 
 /* ---------------------------------------------------------------------------
@@ -192,6 +196,15 @@ int pcvl_mlp_relu_bwd(float *d_dH, const float *d_H, int64_t count, hipStream_t 
 /* d_db[j] = sum_i dY[i][j] for an M x N matrix. */
 int pcvl_mlp_colsum(const float *d_dY, int M, int N, float *d_db, hipStream_t stream);
 
+/* Weight and bias gradients over R rows: d_dW (Mo x No) = dY^T X and d_db
+ * (Mo) = colsum(dY), dY R x Mo (row stride ldd), X R x No (row stride ldx).
+ * Rows are split over blocks in a fixed partition and the partials summed
+ * in order (deterministic).  d_dW or d_db may be NULL; d_work holds
+ * pcvl_mlp_wgrad_workspace(R, Mo, No) floats. */
+int pcvl_mlp_wgrad_workspace(int R, int Mo, int No, int64_t *floats);
+int pcvl_mlp_wgrad(const float *d_dY, int64_t ldd, const float *d_X, int64_t ldx, int R, int Mo, int No,
+                   float *d_dW, float *d_db, float *d_work, int64_t work_floats, hipStream_t stream);
+
 /* Masked softmax cross-entropy per row (modes below d_lo[i] masked):
  * loss[i] = -log q(target[i]); d_dlogits (may be NULL) = (q - onehot) * scale. */
 int pcvl_mlp_ce(const float *d_logits, int M, int m, const int32_t *d_target, const int32_t *d_lo,
@@ -252,6 +265,30 @@ int pcvl_mis_occ_to_seq(const int32_t *d_occ, int64_t B, int m, int n, int32_t *
 /* Chain output: d_out[t] = proposal d_state[t] of d_prop, or d_x0 for -1. */
 int pcvl_mis_gather(const int32_t *d_prop, const int64_t *d_state, int64_t B, int n, const int32_t *d_x0,
                     int32_t *d_out, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Approximate permanents (NaiveApprox backend).
+ * ------------------------------------------------------------------------- */
+
+/* Gurvits estimates of a batch of matrices (same packing as the permanent
+ * plans, orders 0..48), replacing xq.estimate_permanent_cx(M, iterations)
+ * (perceval/backends/_naive_approx.py:52-62).  d_est receives one complex
+ * mean per matrix and d_err its 99% error bound (normal approximation,
+ * 2.5758 * sample standard deviation / sqrt(iters)).  Sample s of matrix b
+ * takes its signs from the counter-based stream (seed, stream0 + b, s).
+ * Copies the small item table to the device and synchronises `stream`. */
+int pcvl_gurvits_batch(const double *d_mats, const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                       int64_t iters, uint64_t seed, uint64_t stream0, double *d_est, double *d_err,
+                       hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Measurement helper (no counterpart in the reference).
+ * ------------------------------------------------------------------------- */
+
+/* Enqueue `blocks` x 256 lanes of pure FP64 FMA chains (`iters` rounds of 16
+ * chains per lane); *flops receives the flops issued (FMA = 2).  d_out holds
+ * blocks*256 doubles.  Timing it gives the attainable vector FP64 rate. */
+int pcvl_probe_fp64_fma(int blocks, int64_t iters, double *d_out, int64_t *flops, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,7 @@ def lib():
         L.oracle_uniform.restype = ctypes.c_double
         L.oracle_cc_samples.argtypes = [_dp, ctypes.c_int, _i32p, ctypes.c_int64, ctypes.c_uint64,
                                         ctypes.c_uint64, _i32p, ctypes.c_int]
+        L.oracle_gurvits.argtypes = [_dp, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64, _dp]
         L.oracle_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -245,6 +246,16 @@ def cc_samples(U, in_occ, count: int, seed: int, first_sample: int = 0, threads:
     _check(lib().oracle_cc_samples(_cd(U.view(np.float64)), m, _ci32(in_occ), count, seed, first_sample,
                                    _ci32(out), threads), "cc_samples")
     return out
+
+
+def gurvits(a, iters: int, seed: int, stream: int = 0):
+    """Gurvits estimate of perm(a): (complex mean, 99% error bound), samples
+    drawn from the counter-based stream (seed, stream, s)."""
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    n = a.shape[0] if a.ndim == 2 else 0
+    out = np.zeros(3, dtype=np.float64)
+    _check(lib().oracle_gurvits(_cd(a), n, int(iters), seed, stream, _cd(out)), "oracle_gurvits")
+    return complex(out[0], out[1]), float(out[2])
 
 
 def max_threads() -> int:

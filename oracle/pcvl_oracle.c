@@ -559,13 +559,67 @@ static inline uint64_t smix(uint64_t z)
     return z ^ (z >> 31);
 }
 
+/* 64 random bits for (seed, stream, counter). */
+static inline uint64_t rand64(uint64_t seed, uint64_t stream, uint64_t ctr)
+{
+    uint64_t key = smix(seed ^ smix(stream));
+    return smix(key + ctr * 0x9E3779B97F4A7C15ull);
+}
+
 /* Uniform double in [0,1) for (seed, stream, counter); identical definition
  * on the device side, so a sampler driven by the same seed draws the same
  * uniforms. */
 double oracle_uniform(uint64_t seed, uint64_t stream, uint64_t ctr)
 {
-    uint64_t key = smix(seed ^ smix(stream));
-    return (double)(smix(key + ctr * 0x9E3779B97F4A7C15ull) >> 11) * 0x1.0p-53;
+    return (double)(rand64(seed, stream, ctr) >> 11) * 0x1.0p-53;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gurvits' randomised permanent estimate                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Restates xq.estimate_permanent_cx(M, iterations) as called from
+ * NaiveApproxBackend (perceval/backends/_naive_approx.py:52-62): the mean of
+ * X(x) = (prod_j x_j) prod_i sum_j a_ij x_j over `iters` sign vectors, sample
+ * s taking x_j = -1 where bit j of rand64(seed, stream, s) is set, and the 99%
+ * normal error bound 2.5758 sqrt((mean |X|^2 - |mean X|^2) / iters).
+ * out = {Re mean, Im mean, error}.  Samples are summed in order s = 0, 1, ... */
+int oracle_gurvits(const double *a, int n, int64_t iters, uint64_t seed, uint64_t stream, double *out)
+{
+    if (n < 0 || n > 64 || iters < 1) return -1;
+    double sr = 0.0, si = 0.0, sq = 0.0;
+    double yr[64], yi[64];
+    for (int64_t s = 0; s < iters; s++) {
+        uint64_t bits = rand64(seed, stream, (uint64_t)s);
+        int neg = 0;
+        for (int i = 0; i < n; i++) { yr[i] = 0.0; yi[i] = 0.0; }
+        for (int j = 0; j < n; j++) {
+            double x = ((bits >> j) & 1) ? -1.0 : 1.0;
+            neg ^= (int)((bits >> j) & 1);
+            for (int i = 0; i < n; i++) {
+                yr[i] += x * a[2 * (i * n + j)];
+                yi[i] += x * a[2 * (i * n + j) + 1];
+            }
+        }
+        double pr = 1.0, pi = 0.0;
+        for (int i = 0; i < n; i++) {
+            double tr = pr * yr[i] - pi * yi[i];
+            double ti = pr * yi[i] + pi * yr[i];
+            pr = tr;
+            pi = ti;
+        }
+        if (neg) { pr = -pr; pi = -pi; }
+        sr += pr;
+        si += pi;
+        sq += pr * pr + pi * pi;
+    }
+    double inv = 1.0 / (double)iters;
+    double mr = sr * inv, mi = si * inv;
+    double var = sq * inv - (mr * mr + mi * mi);
+    out[0] = mr;
+    out[1] = mi;
+    out[2] = 2.5758293035489004 * sqrt((var > 0.0 ? var : 0.0) * inv);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -58,6 +58,9 @@ SIGNATURES = {
                               c_int, c_void_p, c_int, c_int, c_void_p]),
     "pcvl_mlp_relu_bwd": (c_int, [c_void_p, c_void_p, c_i64, c_void_p]),
     "pcvl_mlp_colsum": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "pcvl_mlp_wgrad_workspace": (c_int, [c_int, c_int, c_int, ctypes.POINTER(c_i64)]),
+    "pcvl_mlp_wgrad": (c_int, [c_void_p, c_i64, c_void_p, c_i64, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                               c_i64, c_void_p]),
     "pcvl_mlp_ce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, ctypes.c_float, c_void_p, c_void_p,
                             c_void_p]),
     "pcvl_mlp_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64, ctypes.c_float,
@@ -74,6 +77,9 @@ SIGNATURES = {
     "pcvl_mis_seq_to_occ": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
     "pcvl_mis_occ_to_seq": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
     "pcvl_mis_gather": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p]),
+    "pcvl_probe_fp64_fma": (c_int, [c_int, c_i64, c_void_p, ctypes.POINTER(c_i64), c_void_p]),
+    "pcvl_gurvits_batch": (c_int, [c_void_p, P(c_i32), P(c_i64), c_i64, c_i64, c_u64, c_u64, c_void_p, c_void_p,
+                                   c_void_p]),
     "pcvl_mesh_adjoint": (c_int, [P(c_i32), P(c_i32), P(c_i32), P(c_double), c_int, c_int, c_void_p, c_int, c_int,
                                   P(c_i32), P(c_i32), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pcvl_cc_samples": (c_int, [c_void_p, c_int, P(c_i32), c_i64, c_u64, c_u64, c_void_p, c_void_p, c_size_t,

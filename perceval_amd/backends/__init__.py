@@ -1,6 +1,6 @@
 """Backend registry and factory (perceval/backends/__init__.py:39-62).
 
-BACKEND_LIST holds the three backends this simulation core provides; every
+BACKEND_LIST holds the backends this simulation core provides; every
 caller that looks a backend up by name reads this same dict.
 """
 import warnings
@@ -8,11 +8,13 @@ import warnings
 from ._abstract_backends import ABackend, ASamplingBackend, AStrongSimulationBackend
 from ._clifford2017 import Clifford2017Backend
 from ._naive import NaiveBackend
+from ._naive_approx import NaiveApproxBackend
 from ._slos import SLOSBackend
 
 BACKEND_LIST = {
     "CliffordClifford2017": Clifford2017Backend,
     "Naive": NaiveBackend,
+    "NaiveApprox": NaiveApproxBackend,
     "SLOS": SLOSBackend,
 }
 

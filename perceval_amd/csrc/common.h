@@ -78,10 +78,16 @@ __host__ __device__ __forceinline__ uint64_t smix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
+// 64 random bits of (seed, stream, counter).
+__host__ __device__ __forceinline__ uint64_t rand64(uint64_t seed, uint64_t stream, uint64_t ctr)
+{
+    const uint64_t key = smix64(seed ^ smix64(stream));
+    return smix64(key + ctr * 0x9E3779B97F4A7C15ull);
+}
+
 __host__ __device__ __forceinline__ double uniform01(uint64_t seed, uint64_t stream, uint64_t ctr)
 {
-    uint64_t key = smix64(seed ^ smix64(stream));
-    return (double)(smix64(key + ctr * 0x9E3779B97F4A7C15ull) >> 11) * 0x1.0p-53;
+    return (double)(rand64(seed, stream, ctr) >> 11) * 0x1.0p-53;
 }
 
 } // namespace pcvl

@@ -54,32 +54,48 @@ __global__ void mis_logp_kernel(const cd *__restrict__ perms, const int32_t *__r
     logp[b] = (a2 > 0.0 ? log(a2) : -INFINITY) - lf - log_prod_in;
 }
 
-// Sequential accept/reject over B proposals (one lane; the scan is O(B) and
-// memory-light).  state[t] = index of the proposal the chain holds after step
-// t, or -1 for the initial state.  Uniform of step t: stream (seed, first + t),
-// counter 0.  Returns the number of accepted moves in *accepted.
-__global__ void mis_chain_kernel(const double *__restrict__ logw, int64_t B, double logw0, uint64_t seed,
-                                 uint64_t first, int64_t *__restrict__ state, int64_t *__restrict__ accepted)
+// Accept/reject scan over B proposals, one wave.  state[t] = index of the
+// proposal the chain holds after step t, or -1 for the initial state.
+// Uniform of step t: stream (seed, first + t), counter 0.  Returns the number
+// of accepted moves in *accepted.
+__global__ void __launch_bounds__(64) mis_chain_kernel(const double *__restrict__ logw, int64_t B, double logw0,
+                                                       uint64_t seed, uint64_t first, int64_t *__restrict__ state,
+                                                       int64_t *__restrict__ accepted)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // One wave takes 64 steps at a time.  The log weights and log uniforms of
+    // the 64 steps are loaded and computed in parallel; the sequential part
+    // is one ballot per accepted move: every lane after the last acceptance
+    // tests its own step against the current state, and the first lane that
+    // accepts becomes the new state.  This is the step-by-step rule exactly:
+    // take step t iff lw_t - cur >= 0 or log u_t < lw_t - cur.
+    if (blockIdx.x != 0) return;
+    const int lane = threadIdx.x;
     double cur = logw0;
     int64_t idx = -1, acc = 0;
-    for (int64_t t = 0; t < B; t++) {
-        const double lw = logw[t];
-        const double d = lw - cur;
-        bool take = d >= 0.0;
-        if (!take) {
-            const double u = uniform01(seed, first + (uint64_t)t, 0);
-            take = log(u) < d;
+    for (int64_t t0 = 0; t0 < B; t0 += 64) {
+        const int64_t t = t0 + lane;
+        const bool valid = t < B;
+        const double lw = valid ? logw[t] : 0.0;
+        const double lu = valid ? log(uniform01(seed, first + (uint64_t)t, 0)) : 0.0;
+        uint64_t taken = 0;      // steps of this group that accept
+        int after = -1;          // lanes <= after are decided
+        for (int guard = 0; guard < 64; guard++) {
+            const double d = lw - cur;
+            const bool take = valid && lane > after && (d >= 0.0 || lu < d);
+            const uint64_t mask = __ballot(take);
+            if (mask == 0) break;
+            const int f = __builtin_ctzll(mask);
+            taken |= 1ull << f;
+            cur = __shfl(lw, f, 64);
+            after = f;
         }
-        if (take) {
-            cur = lw;
-            idx = t;
-            acc++;
-        }
-        state[t] = idx;
+        const uint64_t upto = taken & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+        const int64_t mine = upto ? t0 + (63 - __builtin_clzll(upto)) : idx;
+        if (valid) state[t] = mine;
+        if (taken) idx = t0 + (63 - __builtin_clzll(taken));
+        acc += __builtin_popcountll(taken);
     }
-    *accepted = acc;
+    if (lane == 0) *accepted = acc;
 }
 
 // Occupation counts of sorted sequences (B x n) -> (B x m).
@@ -151,13 +167,50 @@ struct MispShared {
     int nu, ng;                      // unique rows (besides the fixed one), groups
 };
 
+// prod_c h[c]; four independent chains from N >= 8 (ILP for FP64 latency).
+template <int N>
+__device__ __forceinline__ cd misp_prod(const cd *h)
+{
+    if constexpr (N < 8) {
+        cd p = h[0];
+#pragma unroll
+        for (int c = 1; c < N; c++) p = cmul(p, h[c]);
+        return p;
+    } else {
+        cd p0 = h[0], p1 = h[1], p2 = h[2], p3 = h[3];
+#pragma unroll
+        for (int c = 4; c < N; c++) {
+            if ((c & 3) == 0) p0 = cmul(p0, h[c]);
+            else if ((c & 3) == 1) p1 = cmul(p1, h[c]);
+            else if ((c & 3) == 2) p2 = cmul(p2, h[c]);
+            else p3 = cmul(p3, h[c]);
+        }
+        return cmul(cmul(p0, p1), cmul(p2, p3));
+    }
+}
+
 template <int N>
 __device__ __forceinline__ void misp_prod_acc(cd &acc, const cd *h, bool neg)
 {
-    cd p = h[0];
-#pragma unroll
-    for (int c = 1; c < N; c++) p = cmul(p, h[c]);
+    const cd p = misp_prod<N>(h);
     if (neg) { acc.r -= p.r; acc.i -= p.i; } else { acc.r += p.r; acc.i += p.i; }
+}
+
+// Returns v, made to look dependent on dep, so LDS reads at offset v stay
+// inside the loop that dep counts instead of being hoisted into VGPRs.
+__device__ __forceinline__ int misp_loop_variant(int v, uint32_t dep)
+{
+    asm("" : "+s"(v) : "s"(dep));
+    return v;
+}
+
+// Keeps the next update of h behind the term that read it (no hoisting of
+// column-sum generations, which would multiply the live VGPRs).
+template <int N>
+__device__ __forceinline__ void misp_pin(cd *h, const cd &part)
+{
+#pragma unroll
+    for (int cc = 0; cc < N; cc++) asm volatile("" : "+v"(h[cc].r), "+v"(h[cc].i) : "v"(part.r), "v"(part.i));
 }
 
 template <int N>
@@ -209,9 +262,12 @@ __global__ void __launch_bounds__(64) mis_perm_kernel(const cd *__restrict__ U, 
         uint64_t O = 1;
         for (int g = 0; g < ng; g++) O *= (uint64_t)(sh.kg[g] + 1);
         const int inner_log2 = nu;
-        int c = inner_log2;                          // steps per unit = 2^c
+        // steps per unit = 2^c: at least four rounds of units over the lanes
+        // while units stay long (2^6 steps) against their set-up cost, and
+        // at least one unit per lane.
+        int c = inner_log2;
+        while (c > 6 && (O << (inner_log2 - c)) < 256) c--;
         while (c > 0 && (O << (inner_log2 - c)) < 64) c--;
-        if (c > 16) c = 16;
         const uint64_t nchunks = 1ull << (inner_log2 - c);
         const uint64_t units = O * nchunks;
         cd acc = {0.0, 0.0};
@@ -249,22 +305,69 @@ __global__ void __launch_bounds__(64) mis_perm_kernel(const cd *__restrict__ U, 
             }
             cd part = {0.0, 0.0};
             const uint64_t steps = 1ull << c;
-            for (uint64_t u = 0; u < steps; u++) {
-                const uint64_t t = base + u;
-                if (u) {
-                    const int bb = __builtin_ctzll(t);
-                    const double sg = (((t >> bb) ^ (t >> (bb + 1))) & 1) ? -1.0 : 1.0;
-                    const cd *row = sh.A + (bb + 1) * N;
+            if (c >= 2) {
+                // Four steps per iteration; base is a multiple of 4, so row 1
+                // (first unique row) flips to delta -1 after the first term
+                // and back after the third.  Row 1 sits in VGPRs up to order
+                // 22 and is read from LDS above (VGPR budget of two waves).
+                constexpr bool kR1Regs = N <= 22;
+                cd r1[kR1Regs ? N : 1];
+                if constexpr (kR1Regs) {
 #pragma unroll
-                    for (int cc = 0; cc < N; cc++) {
-                        h[cc].r = fma(sg, row[cc].r, h[cc].r);
-                        h[cc].i = fma(sg, row[cc].i, h[cc].i);
+                    for (int cc = 0; cc < N; cc++) r1[cc] = sh.A[N + cc];
+                }
+                for (uint64_t u = 0; u < steps; u += 4) {
+                    const cd *row1 = kR1Regs ? r1 : sh.A + misp_loop_variant(N, (uint32_t)u);
+                    const cd *row2 = sh.A + misp_loop_variant(2 * N, (uint32_t)u);
+                    misp_prod_acc<N>(part, h, false);             // t = base+u
+                    misp_pin<N>(h, part);
+#pragma unroll
+                    for (int cc = 0; cc < N; cc++) { h[cc].r -= row1[cc].r; h[cc].i -= row1[cc].i; }
+                    misp_prod_acc<N>(part, h, true);              // t+1
+                    misp_pin<N>(h, part);
+                    {
+                        const uint64_t t = base + u + 2;          // row 2
+                        const double sg = (((t >> 1) ^ (t >> 2)) & 1) ? -1.0 : 1.0;
+#pragma unroll
+                        for (int cc = 0; cc < N; cc++) {
+                            h[cc].r = fma(sg, row2[cc].r, h[cc].r);
+                            h[cc].i = fma(sg, row2[cc].i, h[cc].i);
+                        }
+                    }
+                    misp_prod_acc<N>(part, h, false);             // t+2
+                    misp_pin<N>(h, part);
+#pragma unroll
+                    for (int cc = 0; cc < N; cc++) { h[cc].r += row1[cc].r; h[cc].i += row1[cc].i; }
+                    misp_prod_acc<N>(part, h, true);              // t+3
+                    misp_pin<N>(h, part);
+                    if (u + 4 < steps) {                          // row 1 + ctz(u+4) <= c <= nu
+                        const uint64_t t = base + u + 4;
+                        const int bb = __builtin_ctzll(u + 4);
+                        const double sg = (((t >> bb) ^ (t >> (bb + 1))) & 1) ? -1.0 : 1.0;
+                        const cd *row = sh.A + (bb + 1) * N;
+#pragma unroll
+                        for (int cc = 0; cc < N; cc++) {
+                            h[cc].r = fma(sg, row[cc].r, h[cc].r);
+                            h[cc].i = fma(sg, row[cc].i, h[cc].i);
+                        }
                     }
                 }
-                misp_prod_acc<N>(part, h, (t & 1) != 0);
+            } else {
+                for (uint64_t u = 0; u < steps; u++) {
+                    const uint64_t t = base + u;
+                    if (u) {
+                        const int bb = __builtin_ctzll(t);
+                        const double sg = (((t >> bb) ^ (t >> (bb + 1))) & 1) ? -1.0 : 1.0;
+                        const cd *row = sh.A + (bb + 1) * N;
 #pragma unroll
-                for (int cc = 0; cc < N; cc++)
-                    asm volatile("" : "+v"(h[cc].r), "+v"(h[cc].i) : "v"(part.r), "v"(part.i));
+                        for (int cc = 0; cc < N; cc++) {
+                            h[cc].r = fma(sg, row[cc].r, h[cc].r);
+                            h[cc].i = fma(sg, row[cc].i, h[cc].i);
+                        }
+                    }
+                    misp_prod_acc<N>(part, h, (t & 1) != 0);
+                    misp_pin<N>(h, part);
+                }
             }
             acc.r = fma(W, part.r, acc.r);
             acc.i = fma(W, part.i, acc.i);

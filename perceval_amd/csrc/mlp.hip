@@ -88,6 +88,79 @@ __global__ void __launch_bounds__(256) gemm_kernel(const float *__restrict__ A, 
     }
 }
 
+// Weight gradient over a long row dimension, split deterministically:
+//   part[s][i][j] = sum_{r in split s} dY[r][i] * Xe[r][j],  j < No + 1,
+// with Xe = [X | 1], so column No of the result is db = colsum(dY).  Split s
+// covers rows [s*rows_per, min(R, (s+1)*rows_per)); the second kernel sums
+// the splits in order, so the result does not depend on the launch.
+constexpr int WG_ROWS = 16;
+
+__global__ void __launch_bounds__(256) wgrad_partial_kernel(const float *__restrict__ dY, int64_t ldd,
+                                                            const float *__restrict__ X, int64_t ldx, int R,
+                                                            int Mo, int No, int rows_per, float *__restrict__ part)
+{
+    __shared__ float As[WG_ROWS][GT + 1];
+    __shared__ float Bs[WG_ROWS][GT + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
+    const int split = blockIdx.z;
+    const int r0 = split * rows_per;
+    const int r1 = min(R, r0 + rows_per);
+    const int Ne = No + 1;
+    float acc[4][4] = {};
+    for (int rb = r0; rb < r1; rb += WG_ROWS) {
+        for (int e = threadIdx.x; e < WG_ROWS * GT; e += 256) {
+            const int kk = e / GT, ii = e % GT;
+            const int r = rb + kk;
+            const bool rv = r < r1;
+            const int i = i0 + ii, j = j0 + ii;
+            As[kk][ii] = (rv && i < Mo) ? dY[(size_t)r * ldd + i] : 0.0f;
+            Bs[kk][ii] = (rv && j < No) ? X[(size_t)r * ldx + j] : ((rv && j == No) ? 1.0f : 0.0f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < WG_ROWS; kk++) {
+            float a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) a[q] = As[kk][ty + 16 * q];
+#pragma unroll
+            for (int c = 0; c < 4; c++) b[c] = Bs[kk][tx + 16 * c];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[q][c] = fmaf(a[q], b[c], acc[q][c]);
+        }
+        __syncthreads();
+    }
+    float *P = part + (size_t)split * Mo * Ne;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int i = i0 + ty + 16 * q;
+        if (i >= Mo) continue;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int j = j0 + tx + 16 * c;
+            if (j < Ne) P[(size_t)i * Ne + j] = acc[q][c];
+        }
+    }
+}
+
+__global__ void wgrad_reduce_kernel(const float *__restrict__ part, int splits, int Mo, int No,
+                                    float *__restrict__ dW, float *__restrict__ db)
+{
+    const int Ne = No + 1;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)Mo * Ne) return;
+    float s = 0.0f;
+    for (int k = 0; k < splits; k++) s += part[(size_t)k * Mo * Ne + e];
+    const int i = (int)(e / Ne), j = (int)(e % Ne);
+    if (j < No) {
+        if (dW) dW[(size_t)i * No + j] = s;
+    } else if (db) {
+        db[i] = s;
+    }
+}
+
 // dH *= (H > 0)
 __global__ void relu_bwd_kernel(float *__restrict__ dH, const float *__restrict__ H, int64_t count)
 {
@@ -95,14 +168,20 @@ __global__ void relu_bwd_kernel(float *__restrict__ dH, const float *__restrict_
     if (e < count && !(H[e] > 0.0f)) dH[e] = 0.0f;
 }
 
-// db[j] = sum_i dY[i][j]  (one block per column chunk, fixed order)
-__global__ void colsum_kernel(const float *__restrict__ dY, int M, int N, float *__restrict__ db)
+// db[j] = sum_i dY[i][j]: 64 columns per block, four row groups combined in
+// a fixed order (coalesced rows; for long row counts use the weight-gradient
+// kernels, which split the rows over many blocks).
+__global__ void __launch_bounds__(256) colsum_kernel(const float *__restrict__ dY, int M, int N, float *__restrict__ db)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
+    __shared__ float part[4][64];
+    const int c = threadIdx.x % 64, g = threadIdx.x / 64;
+    const int j = blockIdx.x * 64 + c;
     float s = 0.0f;
-    for (int i = 0; i < M; i++) s += dY[(size_t)i * N + j];
-    db[j] = s;
+    if (j < N)
+        for (int i = g; i < M; i += 4) s += dY[(size_t)i * N + j];
+    part[g][c] = s;
+    __syncthreads();
+    if (g == 0 && j < N) db[j] = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
 }
 
 // Masked softmax cross-entropy of row i with target t[i], modes < lo[i]
@@ -251,6 +330,40 @@ extern "C" int pcvl_mlp_gemm(const float *d_A, int64_t sai, int64_t sak, const v
     return OK;
 }
 
+static int wgrad_splits(int R)
+{
+    int s = R / 256;
+    return s < 1 ? 1 : (s > 128 ? 128 : s);
+}
+
+extern "C" int pcvl_mlp_wgrad_workspace(int R, int Mo, int No, int64_t *floats)
+{
+    PCVL_ARG_CHECK(floats != nullptr, "null output pointer");
+    PCVL_ARG_CHECK(R >= 0 && Mo >= 0 && No >= 0, "negative size");
+    *floats = (int64_t)wgrad_splits(R) * Mo * (No + 1);
+    return OK;
+}
+
+extern "C" int pcvl_mlp_wgrad(const float *d_dY, int64_t ldd, const float *d_X, int64_t ldx, int R, int Mo, int No,
+                              float *d_dW, float *d_db, float *d_work, int64_t work_floats, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(R >= 0 && Mo >= 0 && No >= 0, "negative size");
+    if (Mo == 0) return OK;
+    PCVL_ARG_CHECK(d_dY && d_work && (No == 0 || d_X), "null device buffer");
+    PCVL_ARG_CHECK(ldd >= Mo && (No == 0 || ldx >= No), "leading dimension below the row length");
+    const int splits = wgrad_splits(R);
+    PCVL_ARG_CHECK(work_floats >= (int64_t)splits * Mo * (No + 1), "workspace too small");
+    const int rows_per = R > 0 ? (R + splits - 1) / splits : 0;
+    dim3 grid((No + 1 + GT - 1) / GT, (Mo + GT - 1) / GT, splits);
+    hipLaunchKernelGGL(wgrad_partial_kernel, grid, dim3(256), 0, stream, d_dY, ldd, d_X, ldx, R, Mo, No, rows_per,
+                       d_work);
+    PCVL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid1((int64_t)Mo * (No + 1))), dim3(256), 0, stream, d_work, splits,
+                       Mo, No, d_dW, d_db);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
 extern "C" int pcvl_mlp_relu_bwd(float *d_dH, const float *d_H, int64_t count, hipStream_t stream)
 {
     if (count <= 0) return OK;
@@ -262,7 +375,7 @@ extern "C" int pcvl_mlp_relu_bwd(float *d_dH, const float *d_H, int64_t count, h
 extern "C" int pcvl_mlp_colsum(const float *d_dY, int M, int N, float *d_db, hipStream_t stream)
 {
     if (N <= 0) return OK;
-    hipLaunchKernelGGL(colsum_kernel, dim3(grid1(N)), dim3(256), 0, stream, d_dY, M, N, d_db);
+    hipLaunchKernelGGL(colsum_kernel, dim3((N + 63) / 64), dim3(256), 0, stream, d_dY, M, N, d_db);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
 }

@@ -483,14 +483,14 @@ extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_off
     PCVL_ARG_CHECK(variant >= 0 && variant <= 4, "variant outside [0, 4]");
     PCVL_ARG_CHECK(count >= 0, "count < 0");
     PCVL_ARG_CHECK(count == 0 || (h_n && h_offsets), "null order/offset array");
-    if (max_chunk_log2 <= 0) max_chunk_log2 = 13;
+    if (max_chunk_log2 <= 0) max_chunk_log2 = 11;
     PCVL_ARG_CHECK(max_chunk_log2 >= 2 && max_chunk_log2 <= 24, "max_chunk_log2 outside [2, 24]");
     *out = nullptr;
 
     std::vector<PermItem> items;
     std::vector<int64_t> wave_idx, small_idx, small_off;
     std::vector<int32_t> wave_slot0, wave_nslot, wave_n, small_n;
-    int32_t slots = 0;
+    std::vector<int> wave_c;
     int nmin = 1 << 30, nmax = -1;
     for (int64_t b = 0; b < count; b++) {
         int n = h_n[b];
@@ -502,18 +502,46 @@ extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_off
             small_n.push_back(n);
             continue;
         }
-        int c = pick_chunk_log2(n, max_chunk_log2);
-        uint64_t chunks = 1ull << (n - 1 - c);
-        int64_t waves = (int64_t)(chunks / 64);
         wave_idx.push_back(b);
-        wave_slot0.push_back(slots);
-        wave_nslot.push_back((int32_t)waves);
         wave_n.push_back(n);
-        for (int64_t w = 0; w < waves; w++)
-            items.push_back(PermItem{h_offsets[b], (uint32_t)(w * 64), slots + (int32_t)w, n, c});
-        slots += (int32_t)waves;
+        wave_c.push_back(pick_chunk_log2(n, max_chunk_log2));
         nmin = std::min(nmin, n);
         nmax = std::max(nmax, n);
+    }
+    const int single_n = (nmin == nmax) ? nmax : 0;
+    const int kvariant = single_n ? variant : 0;
+    int rw = 0;
+    if (!wave_idx.empty()) {
+        const void *kp = single_n ? single_kernel_ptr<PERM_WAVE_MIN>(single_n, kvariant)
+                                  : reinterpret_cast<const void *>(&perm_wave_kernel<PERM_WAVE_MIN, PERM_WAVE_MAX, 0>);
+        int rc = resident_waves(kp, &rw);
+        if (rc != OK) return rc;
+    }
+    // Tail refinement: the matrices whose items form the last round of the
+    // queue (the last `rw` items in cost order) use chunks 4x shorter, so the
+    // waves that finish last wait for at most a quarter item.
+    {
+        std::vector<size_t> order(wave_idx.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            return ((int64_t)wave_n[a] << wave_c[a]) > ((int64_t)wave_n[b] << wave_c[b]);
+        });
+        int64_t tail = 0;
+        for (size_t r = order.size(); r-- > 0 && tail < rw;) {
+            const size_t i = order[r];
+            tail += (int64_t)((1ull << (wave_n[i] - 1 - wave_c[i])) / 64);
+            wave_c[i] = std::max(2, wave_c[i] - 2);
+        }
+    }
+    int32_t slots = 0;
+    for (size_t i = 0; i < wave_idx.size(); i++) {
+        const int n = wave_n[i], c = wave_c[i];
+        const int64_t waves = (int64_t)((1ull << (n - 1 - c)) / 64);
+        wave_slot0.push_back(slots);
+        wave_nslot.push_back((int32_t)waves);
+        for (int64_t w = 0; w < waves; w++)
+            items.push_back(PermItem{h_offsets[wave_idx[i]], (uint32_t)(w * 64), slots + (int32_t)w, n, c});
+        slots += (int32_t)waves;
     }
     // Largest items first: the atomic queue then gives a greedy
     // longest-processing-time schedule over the resident waves.
@@ -528,8 +556,8 @@ extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_off
     p.nitems = (int64_t)items.size();
     p.nwave_mats = (int64_t)wave_idx.size();
     p.nsmall = (int64_t)small_idx.size();
-    p.single_n = (nmin == nmax) ? nmax : 0;
-    p.variant = p.single_n ? variant : 0;
+    p.single_n = single_n;
+    p.variant = kvariant;
 
     // One device block: items | cursor | partials | wave arrays | small arrays.
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -580,14 +608,7 @@ extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_off
         return ERR_HIP;
     }
 
-    if (p.nitems > 0) {
-        const void *kp = p.single_n ? single_kernel_ptr<PERM_WAVE_MIN>(p.single_n, p.variant)
-                                    : reinterpret_cast<const void *>(&perm_wave_kernel<PERM_WAVE_MIN, PERM_WAVE_MAX, 0>);
-        int rw = 0;
-        int rc = resident_waves(kp, &rw);
-        if (rc != OK) { (void)hipFree(p.d_block); delete plan; return rc; }
-        p.grid = (int)std::min<int64_t>(p.nitems, (int64_t)rw);
-    }
+    if (p.nitems > 0) p.grid = (int)std::min<int64_t>(p.nitems, (int64_t)rw);
     *out = plan;
     return OK;
 }

@@ -124,6 +124,75 @@ __global__ void __launch_bounds__(SLOS_THREADS) slos_layer_kernel(
 }
 
 
+// Per-state kernel for layers below 2^32 states: the same walk with 32-bit
+// ranks and tables, the unitary column and the factorials in LDS, and the
+// walk ended once no photons remain (later modes are empty).
+template <int MODE>
+__global__ void __launch_bounds__(SLOS_THREADS) slos_layer32_kernel(
+    const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
+    int k, uint32_t nchild, const uint64_t *__restrict__ gT, const uint64_t *__restrict__ gE, int w,
+    double inv_prod_in)
+{
+    extern __shared__ uint64_t s_dyn32[];
+    // layout: u (m cd) | fact (w double) | T (m*w u32) | E (m*w u32)
+    cd *su = reinterpret_cast<cd *>(s_dyn32);
+    double *sf = reinterpret_cast<double *>(su + m);
+    uint32_t *sT = reinterpret_cast<uint32_t *>(sf + w);
+    uint32_t *sE = sT + (size_t)m * w;
+    for (int e = threadIdx.x; e < m * w; e += blockDim.x) {
+        sT[e] = (uint32_t)gT[e];
+        sE[e] = (uint32_t)gE[e];
+    }
+    for (int e = threadIdx.x; e < m; e += blockDim.x) su[e] = U[(size_t)e * m + mk];
+    if (threadIdx.x == 0) {
+        double f = 1.0;
+        for (int v = 0; v < w; v++) {
+            if (v > 1) f *= (double)v;
+            sf[v] = f;
+        }
+    }
+    __syncthreads();
+
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i64 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i64 < nchild; i64 += stride) {
+        const uint32_t idx = (uint32_t)i64;
+        uint32_t rem = idx, pe = 0;
+        int R = k;
+        cd acc = {0.0, 0.0};
+        double pfact = 1.0;
+        for (int q = 0; q < m && R > 0; q++) {
+            int Rn = 0;
+            if (q + 1 < m) {
+                const uint32_t *tq = sT + q * w;
+                Rn = R;
+                uint32_t tv = tq[Rn];
+                while (Rn > 0 && tv > rem) { --Rn; tv = tq[Rn]; }   // T_q(0) = 0 ends the walk
+                rem -= tv;
+            }
+            const int sq = R - Rn;
+            if (sq > 0) {
+                const uint32_t pidx = idx - pe;
+                if (pidx < nparent) {   // always true for a consistent rank; guards the read
+                    const cd p = parent[pidx];
+                    const cd u = su[q];
+                    acc.r = fma(p.r, u.r, acc.r);
+                    acc.r = fma(-p.i, u.i, acc.r);
+                    acc.i = fma(p.r, u.i, acc.i);
+                    acc.i = fma(p.i, u.r, acc.i);
+                }
+                if (MODE == 1) pfact *= sf[sq];
+            }
+            if (q + 1 < m) pe += sE[q * w + Rn];
+            R = Rn;
+        }
+        if (MODE == 0) {
+            reinterpret_cast<cd *>(out)[idx] = acc;
+        } else {
+            reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * pfact * inv_prod_in;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------
 // Incremental layer kernel: each thread walks a run of RUN consecutive
 // children.  Only the first child of a run is unranked; the next child in
@@ -321,7 +390,7 @@ struct SlosPlanImpl {
     uint64_t max_inter = 1;           // largest intermediate layer (k < n)
     uint64_t *d_T = nullptr, *d_E = nullptr;
     int grid_cap = 0;
-    int kernel = 0;                   // 0 auto, 1 one thread per child, 2 incremental runs
+    int kernel = 0;                   // 0 auto, 1 per state, 2 incremental runs, 3 per state 32-bit
 };
 
 static size_t run_kernel_lds(int m, int w, int k)
@@ -330,12 +399,17 @@ static size_t run_kernel_lds(int m, int w, int k)
            sizeof(StackEntry) * (size_t)(k + 1) * SLOS2_THREADS + sizeof(cd) * SLOS2_RUN * SLOS2_THREADS;
 }
 
-// The incremental kernel needs 32-bit ranks and its LDS below 64 KiB.
-static bool run_kernel_ok(const SlosPlanImpl &p)
+static bool all_layers_32bit(const SlosPlanImpl &p)
 {
     for (int k = 0; k <= p.n; k++)
         if (p.counts[k] >= (1ull << 32)) return false;
-    return run_kernel_lds(p.m, p.w, p.n) <= 64 * 1024;
+    return true;
+}
+
+// The incremental kernel needs 32-bit ranks and its LDS below 64 KiB.
+static bool run_kernel_ok(const SlosPlanImpl &p)
+{
+    return all_layers_32bit(p) && run_kernel_lds(p.m, p.w, p.n) <= 64 * 1024;
 }
 
 static int device_grid_cap(int *cap)
@@ -451,10 +525,17 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(bufA), 0);
     PCVL_HIP_CHECK(hipGetLastError());
     const cd *prev = bufA;
-    // Auto takes the per-state kernel: on MI355X at m=24, n=10 it runs the
-    // whole recursion in 3.4 ms against 7.8 ms for the incremental one, whose
-    // lanes gather parents at a stride of RUN elements (uncoalesced).
+    // Auto takes the per-state kernel (32-bit when every layer is below 2^32
+    // states).  The incremental kernel is slower on MI355X (7.8 ms against
+    // 3.4 ms at m=24, n=10): its lanes gather parents at a stride of RUN
+    // elements, and the resulting L2 misses read 41.5 GB for the last layer.
+    const bool small = all_layers_32bit(p);
     const bool use_run = (p.kernel == 2);
+    const bool use32 = (p.kernel == 3) || (p.kernel == 0 && small);
+    if (use32 && !small) {
+        set_error("32-bit SLOS kernel needs every layer below 2^32 states");
+        return ERR_ARG;
+    }
     if (use_run && !run_kernel_ok(p)) {
         set_error("incremental SLOS kernel needs layers below 2^32 states and 64 KiB of LDS");
         return ERR_ARG;
@@ -476,6 +557,17 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
                 hipLaunchKernelGGL(slos_layer_run_kernel<0>, dim3(grid), dim3(SLOS2_THREADS), lds2, stream, prev,
                                    (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
                                    p.d_E, p.w, p.inv_prod_in);
+        } else if (use32) {
+            const int grid = (int)std::min<uint64_t>((nc + SLOS_THREADS - 1) / SLOS_THREADS, (uint64_t)p.grid_cap);
+            const size_t lds32 = sizeof(cd) * p.m + sizeof(double) * p.w + 2 * sizeof(uint32_t) * p.m * p.w;
+            if (probs)
+                hipLaunchKernelGGL(slos_layer32_kernel<1>, dim3(grid), dim3(SLOS_THREADS), lds32, stream, prev,
+                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
+                                   p.d_E, p.w, p.inv_prod_in);
+            else
+                hipLaunchKernelGGL(slos_layer32_kernel<0>, dim3(grid), dim3(SLOS_THREADS), lds32, stream, prev,
+                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
+                                   p.d_E, p.w, p.inv_prod_in);
         } else {
             const int grid = (int)std::min<uint64_t>((nc + SLOS_THREADS - 1) / SLOS_THREADS, (uint64_t)p.grid_cap);
             if (probs)
@@ -494,8 +586,10 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
 extern "C" int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel)
 {
     PCVL_ARG_CHECK(plan != nullptr, "plan is null");
-    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 2, "kernel must be 0 (auto), 1 (per child) or 2 (incremental)");
+    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 3,
+                   "kernel must be 0 (auto), 1 (per state), 2 (incremental) or 3 (per state, 32-bit)");
     if (kernel == 2) PCVL_ARG_CHECK(run_kernel_ok(plan->impl), "incremental kernel not applicable to this plan");
+    if (kernel == 3) PCVL_ARG_CHECK(all_layers_32bit(plan->impl), "32-bit kernel needs layers below 2^32 states");
     plan->impl.kernel = kernel;
     return OK;
 }

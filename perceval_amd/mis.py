@@ -13,6 +13,7 @@ starting state, and the reference the chain is judged against.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -85,20 +86,31 @@ class ProposalNet:
                       "ce")
         dW1, db1 = torch.empty_like(self.W1), torch.empty_like(self.b1)
         dW2, db2 = torch.empty_like(self.W2), torch.empty_like(self.b2)
-        # dW2 = dz^T h ; db2 = colsum dz
-        _native.check(lib.pcvl_mlp_gemm(_p(dz), 1, self.m, _p(h), 0, self.H, 1, _p(dW2), self.H, self.m, self.H, R,
-                                        None, 0, 0, _s()), "gemm dW2")
-        _native.check(lib.pcvl_mlp_colsum(_p(dz), R, self.m, _p(db2), _s()), "colsum")
+        work = self._wgrad_workspace(R, x.device)
+        # dW2 = dz^T h ; db2 = colsum dz  (row-split, deterministic)
+        _native.check(lib.pcvl_mlp_wgrad(_p(dz), self.m, _p(h), self.H, R, self.m, self.H, _p(dW2), _p(db2),
+                                         _p(work), work.numel(), _s()), "wgrad 2")
         # dh = dz W2 (bf16 copy, as in the forward), relu'
         dh = torch.empty_like(h)
         _native.check(lib.pcvl_mlp_gemm(_p(dz), self.m, 1, _p(self.W2b), 1, self.H, 1, _p(dh), self.H, R, self.H,
                                         self.m, None, 0, 0, _s()), "gemm dh")
         _native.check(lib.pcvl_mlp_relu_bwd(_p(dh), _p(h), dh.numel(), _s()), "relu_bwd")
         # dW1 = dh^T x ; db1 = colsum dh
-        _native.check(lib.pcvl_mlp_gemm(_p(dh), 1, self.H, _p(x), 0, self.D, 1, _p(dW1), self.D, self.H, self.D, R,
-                                        None, 0, 0, _s()), "gemm dW1")
-        _native.check(lib.pcvl_mlp_colsum(_p(dh), R, self.H, _p(db1), _s()), "colsum")
+        _native.check(lib.pcvl_mlp_wgrad(_p(dh), self.H, _p(x), self.D, R, self.H, self.D, _p(dW1), _p(db1),
+                                         _p(work), work.numel(), _s()), "wgrad 1")
         return loss, [dW1, db1, dW2, db2]
+
+    def _wgrad_workspace(self, R: int, device):
+        import torch
+        lib = _lib()
+        need = 0
+        for mo, no in ((self.m, self.H), (self.H, self.D)):
+            f = ctypes.c_int64()
+            _native.check(lib.pcvl_mlp_wgrad_workspace(R, mo, no, ctypes.byref(f)), "wgrad_workspace")
+            need = max(need, f.value)
+        if getattr(self, "_wwork", None) is None or self._wwork.numel() < need or self._wwork.device != device:
+            self._wwork = torch.empty(max(1, need), dtype=torch.float32, device=device)
+        return self._wwork
 
     def adam_step(self, grads, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
         lib = _lib()

@@ -99,3 +99,24 @@ def permanents(mats, device="cuda") -> np.ndarray:
 def permanent(M) -> complex:
     """Permanent of one square complex matrix, computed on the GPU."""
     return complex(permanents([np.asarray(M, dtype=np.complex128)])[0])
+
+
+def gurvits(mats, iterations: int, seed: int = 0, stream0: int = 0):
+    """Gurvits estimates of a batch of matrices (list of n x n complex arrays,
+    n <= 48): returns (estimates complex128 (B,), 99% error bounds float64 (B,))
+    as numpy arrays.  Matrix b draws its samples from the counter-based stream
+    (seed, stream0 + b), so results are reproducible and batches can be split."""
+    import torch
+    lib = _native.load()
+    orders, flat = pack(mats)
+    offsets = np.zeros(len(orders), dtype=np.int64)
+    if len(orders):
+        offsets[1:] = np.cumsum(orders.astype(np.int64) ** 2)[:-1]
+    d = torch.from_numpy(flat if flat.size else np.zeros(1, dtype=np.complex128)).cuda()
+    est = torch.empty(max(1, len(orders)), dtype=torch.complex128, device="cuda")
+    err = torch.empty(max(1, len(orders)), dtype=torch.float64, device="cuda")
+    _native.check(lib.pcvl_gurvits_batch(_native.ptr(d), orders.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                         offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(orders),
+                                         int(iterations), int(seed), int(stream0), _native.ptr(est), _native.ptr(err),
+                                         _native.stream_handle()), "pcvl_gurvits_batch")
+    return est.cpu().numpy()[:len(orders)], err.cpu().numpy()[:len(orders)]

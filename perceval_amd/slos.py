@@ -40,7 +40,8 @@ class SlosPlan:
         self._work = None
 
     def set_kernel(self, kernel: int):
-        """0 = pick, 1 = one thread per output state, 2 = incremental runs."""
+        """0 = pick, 1 = one thread per output state, 2 = incremental runs,
+        3 = one thread per output state with 32-bit ranks."""
         _native.check(self._lib.pcvl_slos_plan_set_kernel(self._plan, int(kernel)), "pcvl_slos_plan_set_kernel")
 
     def _workspace(self, device):

@@ -94,7 +94,7 @@ def test_clements_mesh_shape_and_unitarity():
 
 
 def test_backend_registry_names():
-    assert sorted(BackendFactory.list()) == ["CliffordClifford2017", "Naive", "SLOS"]
+    assert sorted(BackendFactory.list()) == ["CliffordClifford2017", "Naive", "NaiveApprox", "SLOS"]
     with pytest.warns(UserWarning):
         b = BackendFactory.get_backend("NoSuchBackend")
     assert b.name == "SLOS"

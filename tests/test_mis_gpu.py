@@ -70,6 +70,43 @@ def test_adam_step_vs_oracle(gpu):
     assert np.array_equal(wb.float().cpu().numpy(), mlp_oracle.bf16_round(dw.cpu().numpy()).astype(np.float32))
 
 
+@pytest.mark.parametrize("R,Mo,No", [(1, 3, 5), (257, 60, 256), (40960, 256, 80), (1000, 70, 0)])
+def test_wgrad_and_colsum_vs_numpy(gpu, R, Mo, No):
+    import ctypes
+    import torch
+    from perceval_amd import _native
+    lib = _native.load()
+    rng = np.random.RandomState(R + Mo)
+    dY = rng.randn(R, Mo).astype(np.float32)
+    X = rng.randn(R, max(No, 1)).astype(np.float32)[:, :No].copy()
+    f = ctypes.c_int64()
+    _native.check(lib.pcvl_mlp_wgrad_workspace(R, Mo, No, ctypes.byref(f)), "ws")
+    work = torch.empty(f.value, dtype=torch.float32, device="cuda")
+    d_dY, d_X = torch.from_numpy(dY).cuda(), torch.from_numpy(X).cuda()
+    dW = torch.empty((Mo, No), dtype=torch.float32, device="cuda")
+    db = torch.empty(Mo, dtype=torch.float32, device="cuda")
+    for _ in range(2):   # deterministic: the same bits twice
+        _native.check(lib.pcvl_mlp_wgrad(_native.ptr(d_dY), Mo, _native.ptr(d_X) if No else None, max(No, 1), R, Mo,
+                                         No, _native.ptr(dW) if No else None, _native.ptr(db), _native.ptr(work),
+                                         work.numel(), _native.stream_handle()), "wgrad")
+        got = (dW.cpu().numpy().copy(), db.cpu().numpy().copy())
+        if _ == 0:
+            first = got
+    assert np.array_equal(first[0], got[0]) and np.array_equal(first[1], got[1])
+    ref_W = dY.astype(np.float64).T @ X.astype(np.float64)
+    ref_b = dY.astype(np.float64).sum(axis=0)
+    scale = np.sqrt(R)
+    assert np.max(np.abs(got[0] - ref_W), initial=0.0) <= 1e-5 * scale
+    assert np.max(np.abs(got[1] - ref_b)) <= 1e-5 * scale
+    cs = torch.empty(Mo, dtype=torch.float32, device="cuda")
+    _native.check(lib.pcvl_mlp_colsum(_native.ptr(d_dY), R, Mo, _native.ptr(cs), _native.stream_handle()), "colsum")
+    assert np.max(np.abs(cs.cpu().numpy() - ref_b)) <= 1e-5 * scale
+    with pytest.raises(ValueError):
+        lib_rc = lib.pcvl_mlp_wgrad(_native.ptr(d_dY), Mo, _native.ptr(d_X), max(No, 1), R, Mo, No, None,
+                                    _native.ptr(db), _native.ptr(work), 0, _native.stream_handle())
+        _native.check(lib_rc, "wgrad small workspace")
+
+
 def test_sampling_gives_exact_log_q(gpu):
     from perceval_amd.mis import ProposalNet
     m, n = 12, 5

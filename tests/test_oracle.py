@@ -294,3 +294,63 @@ def test_uniform_stream_is_deterministic_and_uniform():
     assert abs(u.mean() - 0.5) < 0.01
     assert oracle.uniform(7, 3, 5) == oracle.uniform(7, 3, 5)
     assert oracle.uniform(7, 3, 5) != oracle.uniform(7, 4, 5)
+
+
+# ------------------------------------------------------------- Gurvits estimate
+
+_M64 = (1 << 64) - 1
+
+
+def _smix(z):
+    z = (z + 0x9E3779B97F4A7C15) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def _rand64(seed, stream, ctr):
+    key = _smix(seed ^ _smix(stream))
+    return _smix((key + ctr * 0x9E3779B97F4A7C15) & _M64)
+
+
+def _glynn_sample(A, signs):
+    x = np.array(signs, dtype=float)
+    return np.prod(x) * np.prod(A @ x)
+
+
+def test_gurvits_estimator_is_unbiased_over_all_sign_vectors():
+    """Averaged over every x in {+-1}^n the estimator is exactly perm(A)."""
+    import itertools
+    rng = np.random.RandomState(5)
+    for n in (1, 2, 3, 6, 9):
+        A = rng.randn(n, n) + 1j * rng.randn(n, n)
+        tot = sum(_glynn_sample(A, s) for s in itertools.product((1, -1), repeat=n))
+        assert abs(tot / 2 ** n - oracle.perm(A)) <= 1e-12 * max(1.0, abs(oracle.perm(A)))
+
+
+def test_gurvits_oracle_matches_python_restatement():
+    """Sample s uses x_j = -1 where bit j of rand64(seed, stream, s) is set."""
+    rng = np.random.RandomState(6)
+    A = (rng.randn(7, 7) + 1j * rng.randn(7, 7)) / 3
+    seed, stream, N = 1234, 17, 300
+    xs = [_glynn_sample(A, [-1 if (_rand64(seed, stream, s) >> j) & 1 else 1 for j in range(7)]) for s in range(N)]
+    mean = np.mean(xs)
+    var = np.mean(np.abs(xs) ** 2) - abs(mean) ** 2
+    est, err = oracle.gurvits(A, N, seed=seed, stream=stream)
+    assert abs(est - mean) <= 1e-12 * np.sqrt(np.mean(np.abs(xs) ** 2))
+    assert abs(err - 2.5758293035489004 * np.sqrt(var / N)) <= 1e-9 * err
+
+
+def test_gurvits_oracle_exact_cases_and_coverage():
+    est, err = oracle.gurvits(np.eye(6), 50, seed=1)
+    assert est == 1.0
+    est, err = oracle.gurvits(np.array([[2 + 1j]]), 5, seed=1)
+    assert est == 2 + 1j
+    # 99% intervals cover the exact permanent (Haar submatrix, 200 independent estimates)
+    from perceval_amd.utils.workloads import haar_unitary
+    U = haar_unitary(10, seed=9)
+    A = U[:6, :6]
+    exact = oracle.perm(A)
+    misses = sum(abs(oracle.gurvits(A, 2000, seed=7, stream=k)[0] - exact) > oracle.gurvits(A, 2000, seed=7, stream=k)[1]
+                 for k in range(200))
+    assert misses <= 8   # expectation 2 at 99%

@@ -130,3 +130,27 @@ def test_rejects_order_above_max(gpu):
     from perceval_amd.permanent import PermanentPlan
     with pytest.raises(ValueError):
         PermanentPlan([29])
+
+
+def test_fp64_probe_counts_and_values(gpu):
+    import ctypes
+    import torch
+    from perceval_amd import _native
+    lib = _native.load()
+    out = torch.empty(4 * 256, dtype=torch.float64, device="cuda")
+    flops = ctypes.c_int64()
+    _native.check(lib.pcvl_probe_fp64_fma(4, 10, _native.ptr(out), ctypes.byref(flops), _native.stream_handle()),
+                  "probe")
+    assert flops.value == 4 * 256 * 16 * 10 * 2
+    got = out.cpu().numpy()
+    a, b = 0.999999999, 1e-9
+    for lane in (0, 1, 255, 300):
+        ref = 0.0
+        for k in range(16):
+            c = float((lane % 256) + k)
+            for _ in range(10):
+                c = c * a + b
+            ref += c
+        assert abs(got[lane] - ref) <= 1e-12 * abs(ref)
+    with pytest.raises(ValueError):
+        _native.check(lib.pcvl_probe_fp64_fma(0, 10, _native.ptr(out), ctypes.byref(flops), None), "probe")

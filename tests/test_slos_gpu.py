@@ -44,7 +44,7 @@ def test_fock_rank_unrank_m24_n10_sampled(gpu):
     assert (bad == -1).all()
 
 
-KERNELS = (1, 2)   # one thread per output state; incremental runs
+KERNELS = (1, 2, 3)   # per output state; incremental runs; per output state with 32-bit ranks
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -90,18 +90,20 @@ def test_slos_m12_n6_full_vs_oracle(gpu, kernel):
 
 @pytest.mark.parametrize("inp", [[1, 1, 0, 2, 0, 0, 1, 0, 3, 0, 0, 1], [0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 9],
                                  [5, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0], [1] * 12])
-def test_slos_incremental_matches_per_state_kernel(gpu, inp):
-    """Both kernels on bunched inputs (long runs through the last mode)."""
+def test_slos_kernels_agree(gpu, inp):
+    """Every kernel on bunched inputs (long runs through the last mode)."""
     from perceval_amd.slos import COEFS, SlosPlan
     U = haar_unitary(12, seed=sum(inp))
     plan = SlosPlan(inp)
-    plan.set_kernel(1)
-    a = plan.run(U, COEFS).cpu().numpy()
-    plan.set_kernel(2)
-    b = plan.run(U, COEFS).cpu().numpy()
+    got = {}
+    for kernel in KERNELS:
+        plan.set_kernel(kernel)
+        got[kernel] = plan.run(U, COEFS).cpu().numpy()
     ref = oracle.slos_coefs(U, inp)
-    assert np.max(np.abs(a - ref)) < ABS_TOL and np.max(np.abs(b - ref)) < ABS_TOL
-    assert np.max(np.abs(a - b)) < 1e-14
+    for kernel in KERNELS:
+        assert np.max(np.abs(got[kernel] - ref)) < ABS_TOL, kernel
+        assert np.max(np.abs(got[kernel] - got[1])) < 1e-14, kernel
+    assert np.array_equal(got[1], got[3])   # same walk, same summation order
     plan.close()
 
 
@@ -109,7 +111,7 @@ def test_slos_set_kernel_rejects_bad_values(gpu):
     from perceval_amd.slos import SlosPlan
     plan = SlosPlan([1, 1, 0])
     with pytest.raises(ValueError):
-        plan.set_kernel(3)
+        plan.set_kernel(4)
     plan.close()
 
 
