@@ -11,11 +11,15 @@ configurations are reported beside it:
   mixed_n12_24  4096 permanents, n uniform in 12..24, one launch
   slos_m24_n10  SLOS all_prob, Haar 24x24, 10 photons, 92,561,040 outputs
                 (achieved algorithmic HBM GB/s)
+  slos_tree     37 inputs (an 8-photon state and its 1- and 2-photon-loss
+                variants, m = 20) through one shared SLOS tree vs one plan each
   gradient      adjoint phase gradients, Clements 12 modes (132 phases),
                 6 photons, 256 parameter sets, 32 targets
   sampling      m = 60, n = 20: plain GPU Clifford-Clifford samples/s, the
                 learned Metropolis independence sampler's samples/s, and the
                 1- and 2-mode marginal TVD between the two sample sets
+  sampling_multi  2,000 samples from each of 37 inputs (m = 20, an 8-photon
+                state and its loss variants) in one batched call
   gurvits       NaiveApprox: Gurvits estimates, 64 order-30 matrices x 10^6
                 samples (samples/s)
   copy          1 GiB device copy: the HBM rate a copy attains, and the
@@ -45,7 +49,7 @@ if ROOT not in sys.path:
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 vector, FMA = 2 flops
 HBM_PEAK_GBS = 8000.0
-SECTIONS = ("perm", "mixed", "slos", "gradient", "sampling", "gurvits", "copy")
+SECTIONS = ("perm", "mixed", "slos", "slos_tree", "gradient", "sampling", "sampling_multi", "gurvits", "copy")
 
 
 def glynn_flops(n: int) -> float:
@@ -258,9 +262,9 @@ def bench_slos(args, res):
     out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
     reps = 5
     per_kernel = {}
-    # 1 per state, 2 incremental runs, 3 per state with 32-bit ranks, 0 the
-    # library's own pick (the headline number)
-    for kernel in (1, 2, 3, 0):
+    # 1 per state, 2 incremental runs, 3 per state with 32-bit ranks, 4 per
+    # state in two phases, 0 the library's own pick (the headline number)
+    for kernel in (1, 2, 3, 4, 0):
         plan.set_kernel(kernel)
         plan.run(dU, PROBS, out)
         with Timer() as t:
@@ -274,7 +278,8 @@ def bench_slos(args, res):
     res["slos_m24_n10"] = {"ms": ms, "outputs": plan.count, "algorithmic_GB": gbytes,
                            "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,
                            "sum_probs": total, "ms_by_kernel": {"per_state": per_kernel[1], "incremental": per_kernel[2],
-                                                 "per_state_32bit": per_kernel[3]}}
+                                                 "per_state_32bit": per_kernel[3],
+                                                 "two_phase": per_kernel[4]}}
     log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}")
     if not args.no_cpu:
         import oracle
@@ -285,6 +290,93 @@ def bench_slos(args, res):
         res["slos_cpu"] = {"value": dt * 1e3, "unit": "ms per all_prob (coefficients)", "cores": th, "kind": "port",
                            "sample": f"full m=24 n=10 layer recursion, oracle gather form, {th} POSIX threads"}
         log(f"[bench] SLOS CPU: {dt:.2f} s on {th} threads")
+
+
+def bench_slos_tree(args, res):
+    """Several inputs through one SLOS tree plan against one plan per input:
+    m = 20, the 8-photon input |1^8 0^12> with its 8 one-photon-loss and 28
+    two-photon-loss variants (37 inputs, the shape of a noisy-source
+    simulation)."""
+    import itertools
+    import torch
+    from perceval_amd.slos import PROBS, SlosPlan, SlosTree
+    from perceval_amd.utils.workloads import haar_unitary
+    m, n = 20, 8
+    U = haar_unitary(m, seed=20)
+    base = [1] * n + [0] * (m - n)
+    inputs = [base]
+    for lost in (1, 2):
+        for drop in itertools.combinations(range(n), lost):
+            inputs.append([0 if i in drop else v for i, v in enumerate(base)])
+    dU = torch.from_numpy(U).cuda()
+    tree = SlosTree(inputs)
+    tree.run(dU, PROBS)
+    with Timer() as t:
+        for _ in range(3):
+            outs = tree.run(dU, PROBS)
+    tree_ms = t.ms / 3
+    plans = [SlosPlan(s) for s in inputs]
+    for p in plans:
+        p.run(dU, PROBS)
+    with Timer() as t:
+        for _ in range(3):
+            for p in plans:
+                p.run(dU, PROBS)
+    sep_ms = t.ms / 3
+    sums = [float(o.sum().item()) for o in outs]
+    for p in plans:
+        p.close()
+    res["slos_tree_m20_37inputs"] = {"ms": tree_ms, "ms_separate_plans": sep_ms, "speedup": sep_ms / tree_ms,
+                                     "inputs": len(inputs), "layer_launches": tree.layer_launches,
+                                     "outputs": int(tree.offsets[-1]),
+                                     "max_abs_sum_err": max(abs(v - 1.0) for v in sums)}
+    tree.close()
+    log(f"[bench] SLOS tree m=20, {len(inputs)} inputs: {tree_ms:.3f} ms shared vs {sep_ms:.3f} ms one plan each "
+        f"({sep_ms / tree_ms:.2f}x)")
+    if not args.no_cpu:
+        import oracle
+        th = cpu_threads()
+        t0 = time.perf_counter()
+        for s in inputs:
+            oracle.slos_coefs(U, s, "gather", threads=th)
+        dt = time.perf_counter() - t0
+        res["slos_tree_cpu"] = {"value": dt * 1e3, "unit": "ms for all inputs (coefficients)", "cores": th,
+                                "kind": "port", "sample": f"{len(inputs)} inputs, oracle gather form, one run each"}
+        log(f"[bench] SLOS tree CPU: {dt * 1e3:.1f} ms on {th} threads")
+
+
+def bench_sampling_multi(args, res):
+    """SamplesProvider-shaped batch: 2,000 samples from each of 37 inputs (an
+    8-photon state in 20 modes and its 1- and 2-photon-loss variants) in one
+    pcvl_cc_samples_multi call, against one pcvl_cc_samples call per input."""
+    import itertools
+    import torch
+    from perceval_amd.sampling import cc_samples_device, cc_samples_multi_device
+    from perceval_amd.utils.workloads import haar_unitary
+    m, n, per = 20, 8, 2000
+    U = haar_unitary(m, seed=21)
+    base = [1] * n + [0] * (m - n)
+    inputs = [base]
+    for lost in (1, 2):
+        for drop in itertools.combinations(range(n), lost):
+            inputs.append([0 if i in drop else v for i, v in enumerate(base)])
+    counts = [per] * len(inputs)
+    dU = torch.from_numpy(U).cuda()
+    cc_samples_multi_device(dU, inputs, counts, seed=1)
+    with Timer() as t:
+        cc_samples_multi_device(dU, inputs, counts, seed=2)
+    multi_ms = t.ms
+    for s in inputs[:2]:
+        cc_samples_device(dU, s, per, seed=1)
+    with Timer() as t:
+        for k, s in enumerate(inputs):
+            cc_samples_device(dU, s, per, seed=2, first_sample=k * per)
+    sep_ms = t.ms
+    tot = per * len(inputs)
+    res["sampling_multi_m20_37inputs"] = {"samples": tot, "ms": multi_ms, "samples_per_s": tot / (multi_ms * 1e-3),
+                                          "ms_one_call_per_input": sep_ms, "speedup": sep_ms / multi_ms}
+    log(f"[bench] multi-input sampling: {tot} samples of {len(inputs)} inputs in {multi_ms:.1f} ms "
+        f"({tot / (multi_ms * 1e-3):.0f} samples/s) vs {sep_ms:.1f} ms one call per input")
 
 
 def bench_gradient(args, res):
@@ -414,7 +506,7 @@ def main():
     res = {}
     for s in sections:
         {"perm": bench_perm, "mixed": bench_mixed, "slos": bench_slos, "gradient": bench_gradient, "copy": bench_copy,
-         "gurvits": bench_gurvits,
+         "gurvits": bench_gurvits, "slos_tree": bench_slos_tree, "sampling_multi": bench_sampling_multi,
          "sampling": bench_sampling}[s](args, res)
 
     perm = res.get("perm", {"ms": float("nan"), "perms_per_s": float("nan"), "tflops": float("nan")})

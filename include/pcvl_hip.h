@@ -129,11 +129,29 @@ int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, int mode, dou
 
 int pcvl_slos_plan_destroy(pcvl_slos_plan plan);
 
-/* Kernel used by execute: 0 picks (3 when every layer is below 2^32 states,
- * else 1), 1 one thread per output state, 2 incremental runs of consecutive
- * output states (layers below 2^32 states and 64 KiB of LDS only), 3 one
- * thread per output state with 32-bit ranks (layers below 2^32 states).
- * Kernels not applicable to the plan give -1.  For comparisons. */
+/* Several input states in one plan: the _Path tree of
+ * SLOSBackend.preprocess([inputs]) (perceval/backends/_slos.py:37-86,170-185),
+ * so layers on a shared photon-addition prefix are computed once.  h_inputs
+ * holds count rows of m occupations.  Outputs are concatenated in input
+ * order: input i occupies elements [h_out_offsets[i], h_out_offsets[i+1]) of
+ * d_out (complex for COEFS, double for PROBS, FSArray order each).  The
+ * workspace holds one layer per depth below the largest photon number. */
+typedef struct pcvl_slos_tree_s *pcvl_slos_tree;
+int pcvl_slos_tree_create(int m, const int32_t *h_inputs, int count, pcvl_slos_tree *out);
+/* h_out_offsets receives count + 1 entries; any output pointer may be NULL. */
+int pcvl_slos_tree_info(pcvl_slos_tree plan, int *nmax, int64_t *h_out_offsets, size_t *work_bytes,
+                        int64_t *layer_launches);
+int pcvl_slos_tree_execute(pcvl_slos_tree plan, const double *d_U, int mode, void *d_out, void *d_work,
+                           size_t work_bytes, hipStream_t stream);
+int pcvl_slos_tree_destroy(pcvl_slos_tree plan);
+
+/* Kernel used by execute: 0 picks (4 when every parent layer is below 2^26
+ * states, else 1), 1 one thread per output state, 2 incremental runs of
+ * consecutive output states (layers below 2^32 states and 64 KiB of LDS
+ * only), 3 one thread per output state with 32-bit ranks (layers below 2^32
+ * states), 4 one thread per output state with the walk and the parent gather
+ * in two phases (parent layers below 2^26 states).  Kernels not applicable to
+ * the plan give -1.  For comparisons. */
 int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel);
 
 /* ---------------------------------------------------------------------------
@@ -152,6 +170,18 @@ int pcvl_cc_workspace_bytes(int n, int64_t count, size_t *bytes);
 int pcvl_cc_samples(const double *d_U, int m, const int32_t *h_in_occ, int64_t count, uint64_t seed,
                     uint64_t first_sample, int32_t *d_out, void *d_work, size_t work_bytes,
                     hipStream_t stream);
+
+/* Samples of several input states in one call, for SamplesProvider.prepare
+ * (perceval/simulators/noisy_sampling_simulator.py:55-74), which fills a
+ * sample pool per input state: h_inputs holds ninputs rows of m occupations,
+ * h_counts the samples wanted from each.  Rows of d_out (sum of counts, m
+ * int32 each) are grouped by input in input order.  Sample s of input i is
+ * global sample row0_i + s and uses the counter stream (seed, first_sample +
+ * row0_i + s): the same draws as pcvl_cc_samples on input i with first_sample
+ * + row0_i.  Inputs of equal photon number share one launch per photon step.
+ * Allocates its workspace stream-ordered and synchronises `stream`. */
+int pcvl_cc_samples_multi(const double *d_U, int m, const int32_t *h_inputs, const int64_t *h_counts, int ninputs,
+                          uint64_t seed, uint64_t first_sample, int32_t *d_out, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Phase gradients -- no counterpart in the reference, which differentiates

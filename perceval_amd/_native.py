@@ -53,6 +53,10 @@ SIGNATURES = {
     "pcvl_slos_plan_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pcvl_slos_plan_destroy": (c_int, [c_void_p]),
     "pcvl_slos_plan_set_kernel": (c_int, [c_void_p, c_int]),
+    "pcvl_slos_tree_create": (c_int, [c_int, P(c_i32), c_int, ctypes.POINTER(c_void_p)]),
+    "pcvl_slos_tree_info": (c_int, [c_void_p, P(c_int), P(c_i64), P(ctypes.c_size_t), P(c_i64)]),
+    "pcvl_slos_tree_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
+    "pcvl_slos_tree_destroy": (c_int, [c_void_p]),
     "pcvl_cc_workspace_bytes": (c_int, [c_int, c_i64, P(c_size_t)]),
     "pcvl_mlp_gemm": (c_int, [c_void_p, c_i64, c_i64, c_void_p, c_int, c_i64, c_i64, c_void_p, c_i64, c_int, c_int,
                               c_int, c_void_p, c_int, c_int, c_void_p]),
@@ -84,6 +88,7 @@ SIGNATURES = {
                                   P(c_i32), P(c_i32), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pcvl_cc_samples": (c_int, [c_void_p, c_int, P(c_i32), c_i64, c_u64, c_u64, c_void_p, c_void_p, c_size_t,
                                 c_void_p]),
+    "pcvl_cc_samples_multi": (c_int, [c_void_p, c_int, P(c_i32), P(c_i64), c_int, c_u64, c_u64, c_void_p, c_void_p]),
 }
 
 

@@ -27,6 +27,23 @@ class Clifford2017Backend(ASamplingBackend):
         occ = cc_samples(self._umat, list(self._input_state), int(count), seed=next_sampler_seed())
         return BSSamples.from_array(occ)
 
+    def samples_for_inputs(self, counts: dict) -> dict:
+        """Samples of several input states at once ({input: count} ->
+        {input: BSSamples}), one batched GPU call: what
+        SamplesProvider.prepare (noisy_sampling_simulator.py:55-74) obtains by
+        one set_input_state + samples(count) per input."""
+        from ..sampling import cc_samples_multi_device
+        inputs = [FockState(s) for s in counts]
+        for s in inputs:
+            self._check_state(s)
+        occ = cc_samples_multi_device(self._umat, [list(s) for s in inputs], [int(counts[s]) for s in counts],
+                                      seed=next_sampler_seed()).cpu().numpy()
+        res, row = {}, 0
+        for s, c in zip(inputs, counts.values()):
+            res[s] = BSSamples.from_array(occ[row:row + int(c)])
+            row += int(c)
+        return res
+
     @property
     def name(self) -> str:
         return "CliffordClifford2017"

@@ -1,7 +1,9 @@
 """SLOS backend (perceval/backends/_slos.py:105-223) on the GPU.
 
-Each distinct input state owns a deployed path (a SlosPlan: photon-addition
-order from _Path._decompose, :61-86, and the Fock layers).  A new circuit with
+Each input state deployed on its own owns a path (a SlosPlan: photon-addition
+order from _Path._decompose, :61-86, and the Fock layers); several inputs
+deployed by one preprocess call share a tree (a SlosTree, one _Path root,
+:37-86,170-185), whose shared prefixes are computed once.  A new circuit with
 the same number of modes keeps the deployed paths and only recomputes the
 coefficients (:130-141).  all_prob returns the final layer's probabilities in
 FSArray order (:205-214); prob_amplitude looks a state up by its rank (:187-193).
@@ -12,7 +14,7 @@ import math
 
 import numpy as np
 
-from ..slos import COEFS, PROBS, SlosPlan
+from ..slos import COEFS, PROBS, SlosPlan, SlosTree
 from ..utils.states import BSDistribution, FockState, StateVector, fock_space
 from ._abstract_backends import AStrongSimulationBackend, _as_circuit
 
@@ -86,13 +88,23 @@ class SLOSBackend(AStrongSimulationBackend):
                 self._fsms.append(_Layer(m, k))
 
     def preprocess(self, input_list) -> bool:
+        """Deploy paths for new input states (_slos.py:170-185).  Several new
+        inputs share one tree plan (one _Path root), so their common
+        photon-addition prefixes are computed once per circuit."""
         input_list = [FockState(s) for s in input_list]
-        new = [s for s in input_list if s not in self._state_mapping]
+        new = []
+        for s in input_list:
+            if s not in self._state_mapping and s not in new:
+                new.append(s)
         if not new:
             return False
         self._deploy(new)
-        for s in new:
-            self._state_mapping[s] = SlosPlan(list(s))
+        if len(new) == 1:
+            self._state_mapping[new[0]] = SlosPlan(list(new[0]))
+        else:
+            tree = SlosTree([list(s) for s in new])
+            for i, s in enumerate(new):
+                self._state_mapping[s] = (tree, i)
         return True
 
     # ------------------------------------------------------------ results
@@ -100,7 +112,14 @@ class SLOSBackend(AStrongSimulationBackend):
         key = (istate, mode)
         if key not in self._results:
             plan = self._state_mapping[istate]
-            self._results[key] = plan.run(self._umat, mode).cpu().numpy()
+            if isinstance(plan, tuple):   # a tree: one execution gives every input of it
+                tree, _ = plan
+                outs = tree.run(self._umat, mode)
+                for s, (t2, i) in [(s, v) for s, v in self._state_mapping.items() if isinstance(v, tuple)]:
+                    if t2 is tree:
+                        self._results[(s, mode)] = outs[i].cpu().numpy()
+            else:
+                self._results[key] = plan.run(self._umat, mode).cpu().numpy()
         return self._results[key]
 
     def _masked_indices(self, istate):
@@ -130,7 +149,10 @@ class SLOSBackend(AStrongSimulationBackend):
         """Probabilities of every output state as a CUDA float64 tensor (no host copy)."""
         if input_state is not None:
             self.set_input_state(input_state)
-        return self._state_mapping[self._input_state].run(self._umat, PROBS)
+        plan = self._state_mapping[self._input_state]
+        if isinstance(plan, tuple):
+            return plan[0].run(self._umat, PROBS)[plan[1]]
+        return plan.run(self._umat, PROBS)
 
     def prob_distribution(self) -> BSDistribution:
         bsd = BSDistribution()

@@ -179,18 +179,54 @@ __global__ void cc_init_kernel(const int32_t *__restrict__ cols, int n, int64_t 
     for (int l = 0; l < n; l++) pc[s * n + l] = cols[perm[l]];
 }
 
+// Several input states with the same photon number: sample s of the batch
+// is global sample gidx[s] (its counter stream) of input inp[s], whose
+// photon columns are cols[inp[s] * n ...].
+__global__ void cc_init_multi_kernel(const int32_t *__restrict__ cols, const int32_t *__restrict__ inp,
+                                     const int64_t *__restrict__ gidx, int n, int64_t count, uint64_t seed,
+                                     uint64_t first, int32_t *__restrict__ pc)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    const int32_t *c = cols + (size_t)inp[s] * n;
+    int perm[CC_MAX_N];
+    for (int l = 0; l < n; l++) perm[l] = l;
+    uint64_t ctr = 0;
+    const uint64_t stream = first + (uint64_t)gidx[s];
+    for (int i = n - 1; i >= 1; i--) {
+        const double u = uniform01(seed, stream, ctr++);
+        int j = (int)(u * (i + 1));
+        if (j > i) j = i;
+        const int t = perm[i]; perm[i] = perm[j]; perm[j] = t;
+    }
+    for (int l = 0; l < n; l++) pc[s * n + l] = c[perm[l]];
+}
+
+// Output rows of a multi-input batch: sample s goes to row gidx[s].
+__global__ void cc_finish_multi_kernel(const int32_t *__restrict__ r_all, const int64_t *__restrict__ gidx, int m,
+                                       int n, int64_t count, int32_t *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count * m) return;
+    const int64_t s = e / m;
+    const int i = (int)(e % m);
+    int v = 0;
+    for (int k = 0; k < n; k++) v += (r_all[s * n + k] == i) ? 1 : 0;
+    out[gidx[s] * m + i] = v;
+}
+
 // Step k (= K) of every sample: one wavefront per sample, persistent.
 // r[s*n + i] holds the output mode of photon i (i < k-1 already drawn).
 template <int K>
 __global__ void __launch_bounds__(64) cc_step_kernel(const cd *__restrict__ U, int m, int n, int64_t count,
                                                      uint64_t seed, uint64_t first, const int32_t *__restrict__ pc_all,
-                                                     int32_t *__restrict__ r_all)
+                                                     int32_t *__restrict__ r_all, const int64_t *__restrict__ gidx)
 {
     __shared__ CCShared sh;
     const int lane = threadIdx.x;
     constexpr int CH = CC_MAX_M / 64;
     for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
-        const uint64_t stream = first + (uint64_t)s;
+        const uint64_t stream = first + (uint64_t)(gidx ? gidx[s] : s);
         if (lane < K) sh.pc[lane] = pc_all[s * n + lane];
         if (lane < K - 1) sh.r[lane] = r_all[s * n + lane];
         __syncthreads();
@@ -267,19 +303,20 @@ __global__ void cc_finish_kernel(const int32_t *__restrict__ r_all, int m, int n
 
 template <int K>
 static int launch_step(int k, const cd *U, int m, int n, int64_t count, uint64_t seed, uint64_t first,
-                       const int32_t *pc, int32_t *r, int cus, hipStream_t st)
+                       const int32_t *pc, int32_t *r, int cus, hipStream_t st, const int64_t *gidx = nullptr)
 {
     if constexpr (K > CC_MAX_N) {
         set_error("internal: photon step out of range");
         return ERR_ARG;
     } else {
-        if (k != K) return launch_step<K + 1>(k, U, m, n, count, seed, first, pc, r, cus, st);
+        if (k != K) return launch_step<K + 1>(k, U, m, n, count, seed, first, pc, r, cus, st, gidx);
         int per_cu = 0;
         PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(&cc_step_kernel<K>), 64, 0));
         if (per_cu < 1) per_cu = 1;
         const int grid = (int)std::min<int64_t>(count, (int64_t)cus * per_cu);
-        hipLaunchKernelGGL((cc_step_kernel<K>), dim3(grid), dim3(64), 0, st, U, m, n, count, seed, first, pc, r);
+        hipLaunchKernelGGL((cc_step_kernel<K>), dim3(grid), dim3(64), 0, st, U, m, n, count, seed, first, pc, r,
+                           gidx);
         PCVL_HIP_CHECK(hipGetLastError());
         return OK;
     }
@@ -346,4 +383,99 @@ extern "C" int pcvl_cc_samples(const double *d_U, int m, const int32_t *h_in_occ
     // the host column list copied above must outlive the copy
     PCVL_HIP_CHECK(hipStreamSynchronize(stream));
     return OK;
+}
+
+extern "C" int pcvl_cc_samples_multi(const double *d_U, int m, const int32_t *h_inputs, const int64_t *h_counts,
+                                     int ninputs, uint64_t seed, uint64_t first_sample, int32_t *d_out,
+                                     hipStream_t stream)
+{
+    PCVL_ARG_CHECK(m >= 1 && m <= CC_MAX_M, "mode count outside [1, 256]");
+    PCVL_ARG_CHECK(ninputs >= 0 && (ninputs == 0 || (h_inputs && h_counts)), "null input table");
+    // photon columns of each input, its photon number and its first output row
+    std::vector<std::vector<int32_t>> cols((size_t)ninputs);
+    std::vector<int64_t> row0((size_t)ninputs + 1, 0);
+    for (int i = 0; i < ninputs; i++) {
+        PCVL_ARG_CHECK(h_counts[i] >= 0, "negative sample count");
+        for (int q = 0; q < m; q++) {
+            const int32_t v = h_inputs[(size_t)i * m + q];
+            PCVL_ARG_CHECK(v >= 0, "negative occupation");
+            for (int t = 0; t < v; t++) cols[(size_t)i].push_back(q);
+        }
+        PCVL_ARG_CHECK(cols[(size_t)i].size() <= (size_t)CC_MAX_N, "more than 24 photons");
+        row0[(size_t)i + 1] = row0[(size_t)i] + h_counts[i];
+    }
+    const int64_t total = row0[(size_t)ninputs];
+    if (total == 0) return OK;
+    PCVL_ARG_CHECK(d_U && d_out, "null device buffer");
+    // group the samples by photon number; each group runs one launch sequence
+    std::vector<int> ns;
+    for (int i = 0; i < ninputs; i++)
+        if (h_counts[i] > 0 && std::find(ns.begin(), ns.end(), (int)cols[(size_t)i].size()) == ns.end())
+            ns.push_back((int)cols[(size_t)i].size());
+    int64_t gmax = 0;
+    for (int n : ns) {
+        int64_t c = 0;
+        for (int i = 0; i < ninputs; i++) if ((int)cols[(size_t)i].size() == n) c += h_counts[i];
+        gmax = std::max(gmax, c);
+    }
+    int nmax = 1;
+    for (int n : ns) nmax = std::max(nmax, n);
+    // device block: pc | r (gmax * nmax each) | inp | gidx (gmax) | cols (ninputs * nmax)
+    const size_t b_pr = sizeof(int32_t) * (size_t)gmax * nmax;
+    const size_t b_inp = sizeof(int32_t) * (size_t)gmax;
+    const size_t b_gidx = sizeof(int64_t) * (size_t)gmax;
+    const size_t b_cols = sizeof(int32_t) * (size_t)ninputs * nmax;
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    char *blk = nullptr;
+    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), 2 * al(b_pr) + al(b_inp) + al(b_gidx) + al(b_cols),
+                                  stream));
+    int32_t *pc = reinterpret_cast<int32_t *>(blk);
+    int32_t *r = reinterpret_cast<int32_t *>(blk + al(b_pr));
+    int32_t *d_inp = reinterpret_cast<int32_t *>(blk + 2 * al(b_pr));
+    int64_t *d_gidx = reinterpret_cast<int64_t *>(blk + 2 * al(b_pr) + al(b_inp));
+    int32_t *d_cols = reinterpret_cast<int32_t *>(blk + 2 * al(b_pr) + al(b_inp) + al(b_gidx));
+    const cd *U = reinterpret_cast<const cd *>(d_U);
+    int dev = 0, cus = 0;
+    int rc = OK;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) { set_error(std::string("device query: ") + hipGetErrorString(e)); rc = ERR_HIP; }
+    for (int n : ns) {
+        if (rc != OK) break;
+        std::vector<int32_t> inp, ctab((size_t)ninputs * nmax, 0);
+        std::vector<int64_t> gidx;
+        for (int i = 0; i < ninputs; i++) {
+            if ((int)cols[(size_t)i].size() != n) continue;
+            for (int l = 0; l < n; l++) ctab[(size_t)i * n + l] = cols[(size_t)i][(size_t)l];
+            for (int64_t s = 0; s < h_counts[i]; s++) { inp.push_back(i); gidx.push_back(row0[(size_t)i] + s); }
+        }
+        const int64_t count = (int64_t)gidx.size();
+        if (count == 0) continue;
+        e = hipMemcpyAsync(d_inp, inp.data(), sizeof(int32_t) * inp.size(), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_gidx, gidx.data(), sizeof(int64_t) * gidx.size(), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_cols, ctab.data(), sizeof(int32_t) * ctab.size(), hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) { set_error(std::string("hipMemcpyAsync(cc multi): ") + hipGetErrorString(e)); rc = ERR_HIP; break; }
+        if (n > 0) {
+            hipLaunchKernelGGL(cc_init_multi_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, d_cols,
+                               d_inp, d_gidx, n, count, seed, first_sample, pc);
+            e = hipGetLastError();
+            if (e != hipSuccess) { set_error(std::string("cc_init_multi: ") + hipGetErrorString(e)); rc = ERR_HIP; break; }
+            for (int k = 1; k <= n && rc == OK; k++)
+                rc = launch_step<1>(k, U, m, n, count, seed, first_sample, pc, r, cus, stream, d_gidx);
+            if (rc != OK) break;
+        }
+        const int64_t tot = count * m;
+        hipLaunchKernelGGL(cc_finish_multi_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, r, d_gidx,
+                           m, n, count, d_out);
+        e = hipGetLastError();
+        if (e != hipSuccess) { set_error(std::string("cc_finish_multi: ") + hipGetErrorString(e)); rc = ERR_HIP; break; }
+        // the host tables of this group must outlive their copies
+        e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) { set_error(std::string("cc multi: ") + hipGetErrorString(e)); rc = ERR_HIP; break; }
+    }
+    hipError_t se = hipStreamSynchronize(stream);
+    hipError_t fe = hipFreeAsync(blk, stream);
+    if (rc == OK && se != hipSuccess) { set_error(std::string("cc multi: ") + hipGetErrorString(se)); rc = ERR_HIP; }
+    if (rc == OK && fe != hipSuccess) { set_error(std::string("hipFreeAsync(cc multi): ") + hipGetErrorString(fe)); rc = ERR_HIP; }
+    return rc;
 }

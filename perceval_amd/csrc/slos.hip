@@ -193,6 +193,102 @@ __global__ void __launch_bounds__(SLOS_THREADS) slos_layer32_kernel(
     }
 }
 
+// Per-state kernel in two phases (parent layers below 2^26 states): the walk first
+// records, per occupied mode, the parent rank and the mode in per-thread LDS
+// slots, and a second loop gathers; its loads do not depend on one another,
+// so the unrolled loop keeps several in flight instead of waiting on each
+// inside the walk.  Summation order is the per-state kernel's (increasing
+// mode), so the results agree with it to the last bit for non-zero sums.
+constexpr int SLOS4_THREADS = 256;
+
+template <int MODE>
+__global__ void __launch_bounds__(SLOS4_THREADS) slos_layer2p_kernel(
+    const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
+    int k, uint32_t nchild, const uint64_t *__restrict__ gT, const uint64_t *__restrict__ gE, int w,
+    double inv_prod_in)
+{
+    extern __shared__ uint64_t s_dyn2p[];
+    // layout: u (m cd) | fact (w double) | T (m*w u32) | E (m*w u32) | slots (k*THREADS u32)
+    cd *su = reinterpret_cast<cd *>(s_dyn2p);
+    double *sf = reinterpret_cast<double *>(su + m);
+    uint32_t *sT = reinterpret_cast<uint32_t *>(sf + w);
+    uint32_t *sE = sT + (size_t)m * w;
+    uint32_t *slot = sE + (size_t)m * w;
+    for (int e = threadIdx.x; e < m * w; e += blockDim.x) {
+        sT[e] = (uint32_t)gT[e];
+        sE[e] = (uint32_t)gE[e];
+    }
+    for (int e = threadIdx.x; e < m; e += blockDim.x) su[e] = U[(size_t)e * m + mk];
+    if (threadIdx.x == 0) {
+        double f = 1.0;
+        for (int v = 0; v < w; v++) {
+            if (v > 1) f *= (double)v;
+            sf[v] = f;
+        }
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i64 = (uint64_t)blockIdx.x * blockDim.x + tid; i64 < nchild; i64 += stride) {
+        const uint32_t idx = (uint32_t)i64;
+        uint32_t rem = idx, pe = 0;
+        int R = k, depth = 0;
+        double pfact = 1.0;
+        for (int q = 0; q < m && R > 0; q++) {
+            int Rn = 0;
+            if (q + 1 < m) {
+                const uint32_t *tq = sT + q * w;
+                Rn = R;
+                uint32_t tv = tq[Rn];
+                while (Rn > 0 && tv > rem) { --Rn; tv = tq[Rn]; }
+                rem -= tv;
+            }
+            const int sq = R - Rn;
+            if (sq > 0) {
+                // parent rank (< 2^26) and mode (< 64) packed in one word
+                slot[depth * SLOS4_THREADS + tid] = ((idx - pe) << 6) | (uint32_t)q;
+                depth++;
+                if (MODE == 1) pfact *= sf[sq];
+            }
+            if (q + 1 < m) pe += sE[q * w + Rn];
+            R = Rn;
+        }
+        // Gather four parents at a time with unconditional loads (indices
+        // clamped to the layer), so the four are in flight together; an
+        // entry past the depth, or an index outside the parent layer (never
+        // for a consistent rank), contributes an exact zero.
+        cd acc = {0.0, 0.0};
+        for (int d0 = 0; d0 < depth; d0 += 4) {
+            cd pv[4];
+            int qv[4];
+            bool ok[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int d = d0 + t;
+                const uint32_t w32 = d < depth ? slot[d * SLOS4_THREADS + tid] : 0u;
+                const uint32_t pidx = w32 >> 6;
+                ok[t] = d < depth && pidx < nparent;
+                pv[t] = parent[ok[t] ? pidx : 0u];
+                qv[t] = (int)(w32 & 63u);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const cd u0 = su[qv[t]];
+                const cd u = {ok[t] ? u0.r : 0.0, ok[t] ? u0.i : 0.0};
+                acc.r = fma(pv[t].r, u.r, acc.r);
+                acc.r = fma(-pv[t].i, u.i, acc.r);
+                acc.i = fma(pv[t].r, u.i, acc.i);
+                acc.i = fma(pv[t].i, u.r, acc.i);
+            }
+        }
+        if (MODE == 0) {
+            reinterpret_cast<cd *>(out)[idx] = acc;
+        } else {
+            reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * pfact * inv_prod_in;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------
 // Incremental layer kernel: each thread walks a run of RUN consecutive
 // children.  Only the first child of a run is unranked; the next child in
@@ -390,13 +486,28 @@ struct SlosPlanImpl {
     uint64_t max_inter = 1;           // largest intermediate layer (k < n)
     uint64_t *d_T = nullptr, *d_E = nullptr;
     int grid_cap = 0;
-    int kernel = 0;                   // 0 auto, 1 per state, 2 incremental runs, 3 per state 32-bit
+    int kernel = 0;                   // 0 auto, 1 per state, 2 incremental runs, 3 per state 32-bit,
+                                      // 4 per state in two phases
 };
 
 static size_t run_kernel_lds(int m, int w, int k)
 {
     return sizeof(uint64_t) * ((size_t)m * w + ((size_t)m * w + 1) / 2) + sizeof(cd) * m +
            sizeof(StackEntry) * (size_t)(k + 1) * SLOS2_THREADS + sizeof(cd) * SLOS2_RUN * SLOS2_THREADS;
+}
+
+static size_t twophase_lds(int m, int w, int k)
+{
+    return sizeof(cd) * m + sizeof(double) * w + 2 * sizeof(uint32_t) * (size_t)m * w +
+           sizeof(uint32_t) * (size_t)k * SLOS4_THREADS;
+}
+
+// The two-phase kernel packs a parent rank and a mode in 32 bits.
+static bool twophase_ok(const SlosPlanImpl &p)
+{
+    for (int k = 0; k < p.n; k++)
+        if (p.counts[k] >= (1ull << 26)) return false;
+    return p.counts[p.n] < (1ull << 32) && twophase_lds(p.m, p.w, p.n) <= 64 * 1024;
 }
 
 static bool all_layers_32bit(const SlosPlanImpl &p)
@@ -525,15 +636,17 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(bufA), 0);
     PCVL_HIP_CHECK(hipGetLastError());
     const cd *prev = bufA;
-    // Auto takes the per-state kernel (32-bit when every layer is below 2^32
-    // states).  The incremental kernel is slower on MI355X (7.8 ms against
-    // 3.4 ms at m=24, n=10): its lanes gather parents at a stride of RUN
-    // elements, and the resulting L2 misses read 41.5 GB for the last layer.
+    // Auto takes the two-phase per-state kernel where its packed parent
+    // ranks fit (parent layers below 2^26 states), else the per-state kernel.
+    // The incremental kernel is slower on MI355X (7.8 ms against 3.4 ms at
+    // m=24, n=10): its lanes gather parents at a stride of RUN elements, and
+    // the resulting L2 misses read 41.5 GB for the last layer.
     const bool small = all_layers_32bit(p);
     const bool use_run = (p.kernel == 2);
-    const bool use32 = (p.kernel == 3) || (p.kernel == 0 && small);
-    if (use32 && !small) {
-        set_error("32-bit SLOS kernel needs every layer below 2^32 states");
+    const bool use32 = (p.kernel == 3);
+    const bool use2p = (p.kernel == 4) || (p.kernel == 0 && twophase_ok(p));
+    if ((use32 && !small) || (use2p && !twophase_ok(p))) {
+        set_error("SLOS kernel not applicable to this plan's layer sizes");
         return ERR_ARG;
     }
     if (use_run && !run_kernel_ok(p)) {
@@ -555,6 +668,17 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
                                    p.d_E, p.w, p.inv_prod_in);
             else
                 hipLaunchKernelGGL(slos_layer_run_kernel<0>, dim3(grid), dim3(SLOS2_THREADS), lds2, stream, prev,
+                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
+                                   p.d_E, p.w, p.inv_prod_in);
+        } else if (use2p) {
+            const int grid = (int)std::min<uint64_t>((nc + SLOS4_THREADS - 1) / SLOS4_THREADS, (uint64_t)p.grid_cap);
+            const size_t lds2p = twophase_lds(p.m, p.w, k);
+            if (probs)
+                hipLaunchKernelGGL(slos_layer2p_kernel<1>, dim3(grid), dim3(SLOS4_THREADS), lds2p, stream, prev,
+                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
+                                   p.d_E, p.w, p.inv_prod_in);
+            else
+                hipLaunchKernelGGL(slos_layer2p_kernel<0>, dim3(grid), dim3(SLOS4_THREADS), lds2p, stream, prev,
                                    (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
                                    p.d_E, p.w, p.inv_prod_in);
         } else if (use32) {
@@ -586,8 +710,10 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
 extern "C" int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel)
 {
     PCVL_ARG_CHECK(plan != nullptr, "plan is null");
-    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 3,
-                   "kernel must be 0 (auto), 1 (per state), 2 (incremental) or 3 (per state, 32-bit)");
+    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 4,
+                   "kernel must be 0 (auto), 1 (per state), 2 (incremental), 3 (per state, 32-bit) "
+                   "or 4 (per state, two phases)");
+    if (kernel == 4) PCVL_ARG_CHECK(twophase_ok(plan->impl), "two-phase kernel needs parent layers below 2^26 states");
     if (kernel == 2) PCVL_ARG_CHECK(run_kernel_ok(plan->impl), "incremental kernel not applicable to this plan");
     if (kernel == 3) PCVL_ARG_CHECK(all_layers_32bit(plan->impl), "32-bit kernel needs layers below 2^32 states");
     plan->impl.kernel = kernel;
@@ -603,6 +729,234 @@ extern "C" int pcvl_slos_plan_destroy(pcvl_slos_plan plan)
     if (e != hipSuccess) { set_error(std::string("hipFree(slos plan): ") + hipGetErrorString(e)); return ERR_HIP; }
     return OK;
 }
+
+// ------------------------------------------------------------- input trees
+//
+// SLOSBackend.preprocess([inputs]) builds one _Path tree over several inputs
+// (perceval/backends/_slos.py:37-86,170-185): at each node the photon counts
+// still to add are summed over the node's targets, the mode with the largest
+// sum (lowest on ties) becomes a child taking every target that still has a
+// photon there, and the rest is decomposed again; a target with no photon
+// left is mapped to the node.  Shared prefixes are computed once.  Execution
+// is depth first: node layers live in one buffer per depth, so a node's
+// parent layer is still in place when the node is computed.
+
+struct TreeOp {
+    int k;              // layer of the node (photons)
+    int mk;             // input mode of the photon added by the node
+    int to_level;       // 1: write the node's coefficients to its depth buffer
+    int input;          // >= 0: also write this input's output from the node
+    double inv_prod_in; // 1 / prod t! of that input
+};
+
+struct SlosTreeImpl {
+    int m = 0, nmax = 0, w = 0, ninputs = 0;
+    std::vector<uint64_t> counts;       // N_0..N_nmax
+    std::vector<int64_t> out_off;       // element offset of each input's output
+    std::vector<int> in_n;              // photons of each input
+    std::vector<size_t> level_off;      // byte offset of depth buffer k in the workspace
+    size_t work_bytes = 0;
+    std::vector<TreeOp> ops;            // depth-first launch list
+    uint64_t *d_T = nullptr, *d_E = nullptr;
+    int grid_cap = 0;
+};
+
+static void tree_decompose(SlosTreeImpl &t, int k, int mk, std::vector<std::vector<int32_t>> targets,
+                           std::vector<int> ids, const std::vector<double> &inv_pin)
+{
+    std::vector<int> here;
+    std::vector<std::vector<int32_t>> tg;
+    std::vector<int> tid;
+    for (size_t i = 0; i < targets.size(); i++) {
+        int left = 0;
+        for (int v : targets[i]) left += v;
+        if (left == 0) here.push_back(ids[i]);
+        else { tg.push_back(targets[i]); tid.push_back(ids[i]); }
+    }
+    // the node itself: its layer feeds children (to_level) and/or outputs
+    const bool has_children = !tg.empty();
+    if (k > 0) {
+        if (has_children) t.ops.push_back(TreeOp{k, mk, 1, -1, 1.0});
+        for (int id : here) t.ops.push_back(TreeOp{k, mk, 0, id, inv_pin[(size_t)id]});
+    } else {
+        for (int id : here) t.ops.push_back(TreeOp{0, -1, 0, id, 1.0});
+    }
+    while (!tg.empty()) {
+        std::vector<int> cnt((size_t)t.m, 0);
+        for (auto &x : tg) for (int q = 0; q < t.m; q++) cnt[(size_t)q] += x[(size_t)q];
+        int best = 0;
+        for (int q = 1; q < t.m; q++) if (cnt[(size_t)q] > cnt[(size_t)best]) best = q;
+        std::vector<std::vector<int32_t>> child, rest;
+        std::vector<int> cid, rid;
+        for (size_t i = 0; i < tg.size(); i++) {
+            if (tg[i][(size_t)best] > 0) {
+                tg[i][(size_t)best] -= 1;
+                child.push_back(tg[i]);
+                cid.push_back(tid[i]);
+            } else {
+                rest.push_back(tg[i]);
+                rid.push_back(tid[i]);
+            }
+        }
+        tree_decompose(t, k + 1, best, std::move(child), std::move(cid), inv_pin);
+        tg = std::move(rest);
+        tid = std::move(rid);
+    }
+}
+
+struct pcvl_slos_tree_s {
+    SlosTreeImpl impl;
+};
+
+extern "C" int pcvl_slos_tree_create(int m, const int32_t *h_inputs, int count, pcvl_slos_tree *out)
+{
+    PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
+    PCVL_ARG_CHECK(count >= 1 && h_inputs != nullptr, "no input states");
+    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 64]");
+    *out = nullptr;
+    auto *plan = new (std::nothrow) pcvl_slos_tree_s();
+    if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
+    SlosTreeImpl &t = plan->impl;
+    t.m = m;
+    t.ninputs = count;
+    std::vector<std::vector<int32_t>> targets((size_t)count);
+    std::vector<int> ids((size_t)count);
+    std::vector<double> inv_pin((size_t)count);
+    int64_t total = 0;
+    for (int i = 0; i < count; i++) {
+        const int32_t *row = h_inputs + (size_t)i * m;
+        int n = 0;
+        double pin = 1.0;
+        for (int q = 0; q < m; q++) {
+            if (row[q] < 0) { delete plan; set_error("negative occupation"); return ERR_ARG; }
+            n += row[q];
+            for (int v = 2; v <= row[q]; v++) pin *= v;
+        }
+        if (n > SLOS_MAX_PHOTONS) { delete plan; set_error("more than 40 photons"); return ERR_ARG; }
+        targets[(size_t)i].assign(row, row + m);
+        ids[(size_t)i] = i;
+        inv_pin[(size_t)i] = 1.0 / pin;
+        t.in_n.push_back(n);
+        t.nmax = std::max(t.nmax, n);
+    }
+    t.w = t.nmax + 1;
+    for (int k = 0; k <= t.nmax; k++) {
+        const uint64_t c = binom_u64((int64_t)m + k - 1, k);
+        if (c == UINT64_MAX || c > (1ull << 40)) {
+            delete plan;
+            set_error("Fock space too large (more than 2^40 states in a layer)");
+            return ERR_RANGE;
+        }
+        t.counts.push_back(c);
+    }
+    for (int i = 0; i < count; i++) {
+        t.out_off.push_back(total);
+        total += (int64_t)t.counts[(size_t)t.in_n[(size_t)i]];
+    }
+    t.out_off.push_back(total);
+    size_t off = 0;
+    for (int k = 0; k < t.nmax; k++) {
+        t.level_off.push_back(off);
+        off += ((sizeof(cd) * (size_t)t.counts[(size_t)k]) + 255) & ~size_t(255);
+    }
+    t.work_bytes = off;
+    tree_decompose(t, 0, -1, std::move(targets), std::move(ids), inv_pin);
+
+    std::vector<uint64_t> T, E;
+    build_tables(m, t.nmax, T, E);
+    const size_t bytes = sizeof(uint64_t) * T.size();
+    hipError_t e = hipMalloc(&t.d_T, 2 * bytes);
+    if (e != hipSuccess) { delete plan; set_error(std::string("hipMalloc(slos tables): ") + hipGetErrorString(e)); return ERR_ALLOC; }
+    t.d_E = t.d_T + T.size();
+    e = hipMemcpy(t.d_T, T.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.d_E, E.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(t.d_T);
+        delete plan;
+        set_error(std::string("hipMemcpy(slos tables): ") + hipGetErrorString(e));
+        return ERR_HIP;
+    }
+    int rc = device_grid_cap(&t.grid_cap);
+    if (rc != OK) { (void)hipFree(t.d_T); delete plan; return rc; }
+    *out = plan;
+    return OK;
+}
+
+extern "C" int pcvl_slos_tree_info(pcvl_slos_tree plan, int *nmax, int64_t *h_out_offsets, size_t *work_bytes,
+                                   int64_t *layer_launches)
+{
+    PCVL_ARG_CHECK(plan != nullptr, "plan is null");
+    const SlosTreeImpl &t = plan->impl;
+    if (nmax) *nmax = t.nmax;
+    if (h_out_offsets) for (size_t i = 0; i < t.out_off.size(); i++) h_out_offsets[i] = t.out_off[i];
+    if (work_bytes) *work_bytes = t.work_bytes;
+    if (layer_launches) {
+        int64_t n = 0;
+        for (const TreeOp &op : t.ops) n += op.k > 0 ? 1 : 0;
+        *layer_launches = n;
+    }
+    return OK;
+}
+
+extern "C" int pcvl_slos_tree_execute(pcvl_slos_tree plan, const double *d_U, int mode, void *d_out, void *d_work,
+                                      size_t work_bytes, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(plan != nullptr, "plan is null");
+    PCVL_ARG_CHECK(mode == PCVL_SLOS_COEFS || mode == PCVL_SLOS_PROBS, "mode must be COEFS or PROBS");
+    PCVL_ARG_CHECK(d_U && d_out, "null device buffer");
+    const SlosTreeImpl &t = plan->impl;
+    PCVL_ARG_CHECK(work_bytes >= t.work_bytes && (t.work_bytes == 0 || d_work), "workspace too small");
+    const cd *U = reinterpret_cast<const cd *>(d_U);
+    char *work = static_cast<char *>(d_work);
+    const size_t esz = mode == PCVL_SLOS_PROBS ? sizeof(double) : sizeof(cd);
+    const size_t lds = 2 * sizeof(uint64_t) * (size_t)t.m * t.w;
+    if (t.nmax > 0) {   // depth-0 buffer holds the vacuum coefficient
+        hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(work + t.level_off[0]), 0);
+        PCVL_HIP_CHECK(hipGetLastError());
+    }
+    for (const TreeOp &op : t.ops) {
+        if (op.k == 0) {   // input with no photon: vacuum output
+            hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream,
+                               reinterpret_cast<double *>(static_cast<char *>(d_out) + esz * (size_t)t.out_off[(size_t)op.input]),
+                               mode == PCVL_SLOS_PROBS ? 1 : 0);
+            PCVL_HIP_CHECK(hipGetLastError());
+            continue;
+        }
+        const cd *parent = reinterpret_cast<const cd *>(work + t.level_off[(size_t)op.k - 1]);
+        const uint64_t nc = t.counts[(size_t)op.k];
+        const int grid = (int)std::min<uint64_t>((nc + SLOS_THREADS - 1) / SLOS_THREADS, (uint64_t)t.grid_cap);
+        void *dst;
+        bool probs;
+        if (op.to_level) {
+            dst = work + t.level_off[(size_t)op.k];
+            probs = false;
+        } else {
+            dst = static_cast<char *>(d_out) + esz * (size_t)t.out_off[(size_t)op.input];
+            probs = mode == PCVL_SLOS_PROBS;
+        }
+        if (probs)
+            hipLaunchKernelGGL(slos_layer_kernel<1>, dim3(grid), dim3(SLOS_THREADS), lds, stream, parent,
+                               t.counts[(size_t)op.k - 1], dst, U, t.m, op.mk, op.k, nc, t.d_T, t.d_E, t.w,
+                               op.inv_prod_in);
+        else
+            hipLaunchKernelGGL(slos_layer_kernel<0>, dim3(grid), dim3(SLOS_THREADS), lds, stream, parent,
+                               t.counts[(size_t)op.k - 1], dst, U, t.m, op.mk, op.k, nc, t.d_T, t.d_E, t.w,
+                               op.inv_prod_in);
+        PCVL_HIP_CHECK(hipGetLastError());
+    }
+    return OK;
+}
+
+extern "C" int pcvl_slos_tree_destroy(pcvl_slos_tree plan)
+{
+    if (!plan) return OK;
+    hipError_t e = hipSuccess;
+    if (plan->impl.d_T) e = hipFree(plan->impl.d_T);
+    delete plan;
+    if (e != hipSuccess) { set_error(std::string("hipFree(slos tree): ") + hipGetErrorString(e)); return ERR_HIP; }
+    return OK;
+}
+
 
 // ------------------------------------------------------------------ ranking
 

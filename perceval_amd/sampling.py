@@ -44,3 +44,29 @@ def cc_samples(U, in_occ, count: int, seed: int, first_sample: int = 0) -> np.nd
     if count == 0:
         return np.zeros((0, len(list(in_occ))), dtype=np.int32)
     return cc_samples_device(U, in_occ, count, seed, first_sample).cpu().numpy()
+
+
+def cc_samples_multi_device(U, inputs, counts, seed: int, first_sample: int = 0, stream=None):
+    """Samples of several input states in one call (pcvl_cc_samples_multi):
+    a (sum(counts), m) int32 CUDA tensor, rows grouped by input in order.
+    Input i's rows are the samples pcvl_cc_samples would draw for it with
+    first_sample + sum(counts[:i])."""
+    import torch
+    lib = _native.load()
+    rows = np.ascontiguousarray(np.atleast_2d(np.asarray([list(s) for s in inputs], dtype=np.int32)))
+    cnt = np.ascontiguousarray(np.asarray(list(counts), dtype=np.int64))
+    if len(cnt) != rows.shape[0]:
+        raise ValueError("one count per input state")
+    m = rows.shape[1]
+    if isinstance(U, np.ndarray):
+        dU = torch.from_numpy(np.ascontiguousarray(U, dtype=np.complex128)).cuda()
+    else:
+        dU = U.to(dtype=torch.complex128).contiguous()
+    if tuple(dU.shape) != (m, m):
+        raise ValueError(f"unitary must be {m} x {m}")
+    out = torch.empty((max(1, int(cnt.sum())), m), dtype=torch.int32, device=dU.device)
+    _native.check(lib.pcvl_cc_samples_multi(_native.ptr(dU), m, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), rows.shape[0],
+                                            int(seed) & (2 ** 64 - 1), int(first_sample), _native.ptr(out),
+                                            _native.stream_handle(stream)), "pcvl_cc_samples_multi")
+    return out[:int(cnt.sum())]

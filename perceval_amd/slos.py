@@ -41,7 +41,8 @@ class SlosPlan:
 
     def set_kernel(self, kernel: int):
         """0 = pick, 1 = one thread per output state, 2 = incremental runs,
-        3 = one thread per output state with 32-bit ranks."""
+        3 = one thread per output state with 32-bit ranks, 4 = one thread per
+        output state, walk and gather in two phases."""
         _native.check(self._lib.pcvl_slos_plan_set_kernel(self._plan, int(kernel)), "pcvl_slos_plan_set_kernel")
 
     def _workspace(self, device):
@@ -75,6 +76,65 @@ class SlosPlan:
     def close(self):
         if getattr(self, "_plan", None) is not None and self._plan.value:
             _native.check(self._lib.pcvl_slos_plan_destroy(self._plan), "pcvl_slos_plan_destroy")
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SlosTree:
+    """Several input states in one plan: the _Path tree that
+    SLOSBackend.preprocess([inputs]) builds (perceval/backends/_slos.py:37-86,
+    170-185), so layers on a shared photon-addition prefix are computed once.
+    ``run`` returns one CUDA tensor per input (views of one output buffer)."""
+
+    def __init__(self, inputs):
+        lib = _native.load()
+        rows = np.ascontiguousarray(np.atleast_2d(np.asarray([list(s) for s in inputs], dtype=np.int32)))
+        self.inputs = rows
+        self.count, self.m = rows.shape
+        self._plan = ctypes.c_void_p()
+        _native.check(lib.pcvl_slos_tree_create(self.m, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                self.count, ctypes.byref(self._plan)), "pcvl_slos_tree_create")
+        self._lib = lib
+        nmax, work, launches = ctypes.c_int(), ctypes.c_size_t(), ctypes.c_int64()
+        offs = np.zeros(self.count + 1, dtype=np.int64)
+        _native.check(lib.pcvl_slos_tree_info(self._plan, ctypes.byref(nmax),
+                                              offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                              ctypes.byref(work), ctypes.byref(launches)), "pcvl_slos_tree_info")
+        self.nmax = nmax.value
+        self.offsets = offs
+        self.work_bytes = work.value
+        self.layer_launches = launches.value
+        self._work = None
+
+    def run(self, U, mode: int = PROBS, out=None, stream=None):
+        import torch
+        if isinstance(U, np.ndarray):
+            dU = torch.from_numpy(np.ascontiguousarray(U, dtype=np.complex128)).cuda()
+        else:
+            dU = U.to(dtype=torch.complex128).contiguous()
+        if tuple(dU.shape) != (self.m, self.m):
+            raise ValueError(f"unitary must be {self.m} x {self.m}")
+        dtype = torch.complex128 if mode == COEFS else torch.float64
+        total = int(self.offsets[-1])
+        if out is None:
+            out = torch.empty(total, dtype=dtype, device=dU.device)
+        elif out.dtype != dtype or out.numel() < total or not out.is_contiguous():
+            raise ValueError("out has the wrong dtype, size or layout")
+        if self._work is None or self._work.device != dU.device:
+            self._work = torch.empty(max(16, self.work_bytes), dtype=torch.uint8, device=dU.device)
+        _native.check(self._lib.pcvl_slos_tree_execute(self._plan, _native.ptr(dU), mode, _native.ptr(out),
+                                                       _native.ptr(self._work), self._work.numel(),
+                                                       _native.stream_handle(stream)), "pcvl_slos_tree_execute")
+        return [out[int(self.offsets[i]):int(self.offsets[i + 1])] for i in range(self.count)]
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            _native.check(self._lib.pcvl_slos_tree_destroy(self._plan), "pcvl_slos_tree_destroy")
             self._plan = ctypes.c_void_p()
 
     def __del__(self):

@@ -90,3 +90,44 @@ def test_deterministic_replay(gpu):
     c = cc_samples(U, [1] * 5 + [0] * 5, 500, seed=4)
     assert np.array_equal(a, b)
     assert not np.array_equal(a, c)
+
+
+def test_cc_samples_multi_equals_single_input_calls(gpu):
+    """Rows of input i equal pcvl_cc_samples on input i with first_sample +
+    row0_i, bit for bit (same counter streams), across photon numbers."""
+    from perceval_amd.sampling import cc_samples, cc_samples_multi_device
+    m = 10
+    U = haar_unitary(m, seed=101)
+    inputs = [[1, 1, 1, 1, 0, 0, 0, 0, 0, 0], [1, 1, 1, 0, 0, 0, 0, 0, 0, 0], [0] * m,
+              [2, 0, 1, 0, 0, 0, 0, 0, 0, 1], [1, 0, 1, 1, 0, 0, 0, 0, 0, 0], [0, 0, 0, 0, 0, 0, 0, 0, 0, 5]]
+    counts = [300, 257, 5, 0, 64, 100]
+    seed, first = 4242, 1000
+    out = cc_samples_multi_device(U, inputs, counts, seed=seed, first_sample=first).cpu().numpy()
+    assert out.shape == (sum(counts), m)
+    row = 0
+    for inp, c in zip(inputs, counts):
+        ref = cc_samples(U, inp, c, seed=seed, first_sample=first + row)
+        assert np.array_equal(out[row:row + c], ref), inp
+        assert (out[row:row + c].sum(axis=1) == sum(inp)).all()
+        row += c
+    # and seed for seed with the oracle on one of them
+    row0 = counts[0] + counts[1] + counts[2] + counts[3]
+    ref = oracle.cc_samples(U, inputs[4], counts[4], seed=seed, first_sample=first + row0)
+    assert np.array_equal(out[row0:row0 + counts[4]], ref)
+
+
+def test_backend_samples_for_inputs(gpu):
+    from perceval_amd.backends import BackendFactory
+    from perceval_amd.components import Unitary
+    from perceval_amd.utils.states import BasicState
+    U = haar_unitary(6, seed=7)
+    b = BackendFactory.get_backend("CliffordClifford2017")
+    b.set_circuit(Unitary(U))
+    want = {BasicState([1, 1, 0, 0, 0, 0]): 50, BasicState([1, 0, 1, 0, 0, 0]): 20, BasicState([0, 0, 0, 0, 0, 0]): 3}
+    got = b.samples_for_inputs(want)
+    assert list(got) == list(want)
+    for s, c in want.items():
+        assert len(got[s]) == c
+        assert all(o.n == s.n and o.m == 6 for o in got[s])
+    with pytest.raises(AssertionError):
+        b.samples_for_inputs({BasicState([1, 1, 0]): 2})

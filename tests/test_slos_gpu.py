@@ -44,7 +44,7 @@ def test_fock_rank_unrank_m24_n10_sampled(gpu):
     assert (bad == -1).all()
 
 
-KERNELS = (1, 2, 3)   # per output state; incremental runs; per output state with 32-bit ranks
+KERNELS = (1, 2, 3, 4)   # per output state; incremental runs; 32-bit ranks; two-phase gather
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -104,6 +104,7 @@ def test_slos_kernels_agree(gpu, inp):
         assert np.max(np.abs(got[kernel] - ref)) < ABS_TOL, kernel
         assert np.max(np.abs(got[kernel] - got[1])) < 1e-14, kernel
     assert np.array_equal(got[1], got[3])   # same walk, same summation order
+    assert np.max(np.abs(got[1] - got[4])) == 0.0
     plan.close()
 
 
@@ -111,7 +112,7 @@ def test_slos_set_kernel_rejects_bad_values(gpu):
     from perceval_amd.slos import SlosPlan
     plan = SlosPlan([1, 1, 0])
     with pytest.raises(ValueError):
-        plan.set_kernel(4)
+        plan.set_kernel(5)
     plan.close()
 
 
@@ -163,3 +164,72 @@ def test_slos_rejects_bad_inputs(gpu):
         SlosPlan([-1, 2])
     with pytest.raises(ValueError):
         SlosPlan([1] * 65)
+
+
+def _tree_layer_launches(inputs):
+    """Layer launches of the _Path tree (perceval/backends/_slos.py:61-86),
+    restated: one per node that has children, one per input that ends at a
+    node of depth >= 1."""
+    def walk(k, targets):
+        here = [t for t in targets if sum(t) == 0]
+        rest = [list(t) for t in targets if sum(t) > 0]
+        n = (1 if (k > 0 and rest) else 0) + (len(here) if k > 0 else 0)
+        while rest:
+            counts = [sum(col) for col in zip(*rest)]
+            best = counts.index(max(counts))
+            child, nxt = [], []
+            for t in rest:
+                if t[best]:
+                    t[best] -= 1
+                    child.append(t)
+                else:
+                    nxt.append(t)
+            n += walk(k + 1, child)
+            rest = nxt
+        return n
+    return walk(0, [list(s) for s in inputs])
+
+
+@pytest.mark.parametrize("m,inputs", [
+    (4, [[1, 1, 0, 0], [1, 0, 1, 0], [0, 1, 1, 0], [1, 1, 1, 0], [2, 0, 0, 0], [0, 0, 0, 0], [1, 0, 0, 0]]),
+    (12, [[1] * 6 + [0] * 6] + [[0 if j == i else 1 for j in range(6)] + [0] * 6 for i in range(6)]),
+    (6, [[3, 0, 0, 0, 0, 0], [2, 1, 0, 0, 0, 0], [0, 0, 0, 0, 0, 4], [1, 1, 1, 1, 1, 1]]),
+])
+def test_slos_tree_vs_oracle_and_single_plans(gpu, m, inputs):
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan, SlosTree
+    U = haar_unitary(m, seed=m + len(inputs))
+    tree = SlosTree(inputs)
+    assert tree.layer_launches == _tree_layer_launches(inputs)
+    probs = [p.cpu().numpy() for p in tree.run(U, PROBS)]
+    coefs = [c.cpu().numpy() for c in tree.run(U, COEFS)]
+    for inp, p, c in zip(inputs, probs, coefs):
+        assert np.max(np.abs(p - oracle.slos_probs(U, inp))) < ABS_TOL, inp
+        assert np.max(np.abs(c - oracle.slos_coefs(U, inp))) < ABS_TOL, inp
+        single = SlosPlan(inp).run(U, PROBS).cpu().numpy()
+        assert np.max(np.abs(p - single)) < 1e-13, inp
+    tree.close()
+
+
+def test_slos_backend_preprocess_shares_a_tree(gpu):
+    from perceval_amd.backends import SLOSBackend
+    from perceval_amd.components import Unitary
+    from perceval_amd.slos import SlosTree
+    from perceval_amd.utils.states import BasicState
+    m = 8
+    U = haar_unitary(m, seed=81)
+    inputs = [BasicState([1, 1, 1, 1, 0, 0, 0, 0]), BasicState([1, 1, 1, 0, 0, 0, 0, 0]),
+              BasicState([0, 1, 1, 1, 0, 0, 0, 0]), BasicState([1, 1, 0, 1, 0, 0, 0, 0])]
+    b = SLOSBackend()
+    b.set_circuit(Unitary(U))
+    assert b.preprocess(inputs) is True
+    assert all(isinstance(b._state_mapping[s], tuple) and isinstance(b._state_mapping[s][0], SlosTree)
+               for s in inputs)
+    assert b.preprocess(inputs) is False
+    for s in inputs:
+        b.set_input_state(s)
+        assert np.max(np.abs(np.array(b.all_prob()) - oracle.slos_probs(U, list(s)))) < ABS_TOL
+    # a new circuit of the same size keeps the tree and recomputes
+    U2 = haar_unitary(m, seed=82)
+    b.set_circuit(Unitary(U2))
+    b.set_input_state(inputs[2])
+    assert np.max(np.abs(np.array(b.all_prob()) - oracle.slos_probs(U2, list(inputs[2])))) < ABS_TOL
