@@ -486,7 +486,7 @@ def main():
     ap.add_argument("--chunk-log2", type=int, default=0, help="permanent lane chunk (0 = default)")
     ap.add_argument("--variant", type=int, default=0, help="permanent kernel variant (0 = by order)")
     ap.add_argument("--samples", type=int, default=100000)
-    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--cpu-budget", type=float, default=5.0, help="seconds of CPU baseline per section")
     ap.add_argument("--sections", default=",".join(SECTIONS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baselines")

@@ -265,6 +265,16 @@ int pcvl_mis_sample_step(const float *d_logits, int B, int m, int n, int p, cons
                          uint64_t seed, uint64_t first, int32_t *d_seq, double *d_logq,
                          hipStream_t stream);
 
+/* All n steps of B proposals in one launch: the same network (W1 H x (m+n),
+ * W2 m x H, bf16; biases fp32) and the same uniform (seed, first + b, p) for
+ * step p of proposal b as the stepwise path, with the first layer's
+ * pre-activation updated in place instead of recomputed.  Writes the
+ * sequences (B x n) and the exact log q of each.  m <= 64, 8 <= H <= 256,
+ * H % 8 == 0. */
+int pcvl_mis_sample_fused(const uint16_t *d_W1, const float *d_b1, const uint16_t *d_W2, const float *d_b2,
+                          int B, int m, int n, int H, uint64_t seed, uint64_t first, int32_t *d_seq,
+                          double *d_logq, hipStream_t stream);
+
 /* n x n submatrices U[seq[b], cols] of B proposals (complex, packed). */
 int pcvl_mis_submatrices(const double *d_U, int m, const int32_t *d_cols, int n, const int32_t *d_seq,
                          int64_t B, double *d_out, hipStream_t stream);

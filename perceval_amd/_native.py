@@ -74,6 +74,8 @@ SIGNATURES = {
     "pcvl_mis_features_all": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pcvl_mis_sample_step": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_u64, c_u64, c_void_p,
                                      c_void_p, c_void_p]),
+    "pcvl_mis_sample_fused": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_u64, c_u64,
+                                      c_void_p, c_void_p, c_void_p]),
     "pcvl_mis_submatrices": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_i64, c_void_p, c_void_p]),
     "pcvl_mis_permanents": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_i64, c_void_p, c_void_p]),
     "pcvl_mis_logp": (c_int, [c_void_p, c_void_p, c_int, c_i64, c_double, c_void_p, c_void_p]),

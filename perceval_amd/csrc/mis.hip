@@ -213,65 +213,133 @@ __device__ __forceinline__ void misp_pin(cd *h, const cd &part)
     for (int cc = 0; cc < N; cc++) asm volatile("" : "+v"(h[cc].r), "+v"(h[cc].i) : "v"(part.r), "v"(part.i));
 }
 
+// ---- multiset permanents of proposals, as a balanced work list -----------
+//
+// Proposal p (a sorted mode sequence) is planned by one thread: its rows are
+// a fixed row (delta_0 = +1), nu unique rows (Gray-coded) and ng groups of
+// k_g equal rows (outer mixed-radix digits t_g in 0..k_g, weight
+// C(k_g, t_g)(-1)^t_g, column-sum offset (k_g - 2 t_g)/2 times the row).  Its
+// O * 2^nu terms are cut into units of 2^c Gray steps (units = O 2^(nu-c),
+// c <= 10 and, where possible, units >= 64), and the units into at most 64
+// slices.  Waves pull slices from one atomic cursor, so proposals of very
+// different cost (a collision-free proposal has up to 4000 times the terms
+// of a bunched one) balance over the device; each slice writes one partial,
+// and a finalize pass sums a proposal's partials in slice order
+// (deterministic).
+constexpr int MISP_REC = 8 + 2 * MISP_MAX_N;   // ints per proposal record
+constexpr int MISP_MAX_SLICES = 64;
+constexpr int MISP_MAX_C = 10;
+
+// record: [0] nu [1] ng [2] c [3] slices [4..5] units (int64) [6] nrow
+//         [8 ..] row modes (fixed, unique, groups)  [8 + MAX_N ..] group sizes
+__global__ void mis_plan_kernel(const int32_t *__restrict__ seq, int64_t B, int n, int m, int32_t *__restrict__ rec,
+                                int32_t *__restrict__ nslices, unsigned long long *__restrict__ cursor)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) *cursor = 0ull;
+    if (b >= B) return;
+    int32_t *R = rec + b * MISP_REC;
+    int gmode[MISP_MAX_N], gcnt[MISP_MAX_N];
+    int G = 0;
+    for (int r = 0; r < n; r++) {
+        int j = seq[b * n + r];
+        j = (j >= 0 && j < m) ? j : 0;
+        if (G > 0 && gmode[G - 1] == j) gcnt[G - 1]++;
+        else { gmode[G] = j; gcnt[G] = 1; G++; }
+    }
+    int fixed = -1;
+    for (int g = 0; g < G; g++) if (gcnt[g] == 1) { fixed = g; break; }
+    int nrow = 0, nu = 0, ng = 0;
+    if (fixed >= 0) {
+        R[8 + nrow++] = gmode[fixed];
+        for (int g = 0; g < G; g++)
+            if (g != fixed && gcnt[g] == 1) { R[8 + nrow++] = gmode[g]; nu++; }
+    } else {
+        R[8 + nrow++] = gmode[0];
+        gcnt[0] -= 1;   // one copy of group 0 is the fixed row
+    }
+    uint64_t O = 1;
+    for (int g = 0; g < G; g++) {
+        if (g == fixed || gcnt[g] < 1) continue;
+        if (fixed >= 0 && gcnt[g] == 1) continue;   // already a unique row
+        R[8 + nrow++] = gmode[g];
+        R[8 + MISP_MAX_N + ng++] = gcnt[g];
+        O *= (uint64_t)(gcnt[g] + 1);
+    }
+    int c = nu < MISP_MAX_C ? nu : MISP_MAX_C;
+    while (c > 0 && (O << (nu - c)) < 64) c--;
+    const uint64_t units = O << (nu - c);
+    const uint64_t sl = (units + 63) / 64;
+    const int slices = (int)(sl < (uint64_t)MISP_MAX_SLICES ? sl : (uint64_t)MISP_MAX_SLICES);
+    R[0] = nu; R[1] = ng; R[2] = c; R[3] = slices;
+    R[4] = (int32_t)(units & 0xffffffffu); R[5] = (int32_t)(units >> 32); R[6] = nrow; R[7] = 0;
+    nslices[b] = slices;
+}
+
+// Exclusive prefix of the slice counts, one block: slice0[b], total at slice0[B].
+__global__ void __launch_bounds__(1024) mis_scan_kernel(const int32_t *__restrict__ nslices, int64_t B,
+                                                        int64_t *__restrict__ slice0)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (B + 1023) / 1024;
+    const int64_t lo = t * per, hi = (lo + per < B) ? lo + per : B;
+    int64_t sum = 0;
+    for (int64_t i = lo; i < hi; i++) sum += nslices[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - sum;   // exclusive
+    for (int64_t i = lo; i < hi; i++) { slice0[i] = run; run += nslices[i]; }
+    if (t == 1023) slice0[B] = part[1023];
+}
+
 template <int N>
-__global__ void __launch_bounds__(64) mis_perm_kernel(const cd *__restrict__ U, int m, const int32_t *__restrict__ cols,
-                                                      const int32_t *__restrict__ seq, int64_t B, cd *__restrict__ out)
+__global__ void __launch_bounds__(64) mis_slice_kernel(const cd *__restrict__ U, int m, const int32_t *__restrict__ cols,
+                                                       const int32_t *__restrict__ rec, const int64_t *__restrict__ slice0,
+                                                       int64_t B, unsigned long long *__restrict__ cursor,
+                                                       cd *__restrict__ partial)
 {
     __shared__ MispShared sh;
+    __shared__ int32_t srec[MISP_REC];
     const int lane = threadIdx.x;
-    for (int64_t b = blockIdx.x; b < B; b += gridDim.x) {
-        if (lane == 0) {
-            // groups of equal modes in the sorted sequence (worked in LDS so
-            // the per-proposal bookkeeping holds no VGPR arrays)
-            int R = 0;
-            for (int r = 0; r < N; r++) {
-                int j = seq[b * N + r];
-                j = (j >= 0 && j < m) ? j : 0;
-                if (R > 0 && sh.gmode[R - 1] == j) sh.gcnt[R - 1]++;
-                else { sh.gmode[R] = j; sh.gcnt[R] = 1; R++; }
-            }
-            int fixed = -1;
-            for (int g = 0; g < R; g++) if (sh.gcnt[g] == 1) { fixed = g; break; }
-            int nrow = 0, nu = 0, ng = 0;
-            if (fixed >= 0) {
-                sh.rmode[nrow++] = sh.gmode[fixed];
-                for (int g = 0; g < R; g++)
-                    if (g != fixed && sh.gcnt[g] == 1) { sh.rmode[nrow++] = sh.gmode[g]; nu++; }
-            } else {
-                sh.rmode[nrow++] = sh.gmode[0];
-                sh.gcnt[0] -= 1;   // one copy of group 0 is the fixed row
-            }
-            for (int g = 0; g < R; g++) {
-                if (g == fixed || sh.gcnt[g] < 1) continue;
-                if (fixed >= 0 && sh.gcnt[g] == 1) continue;   // already a unique row
-                sh.rmode[nrow++] = sh.gmode[g];
-                sh.kg[ng++] = sh.gcnt[g];
-            }
-            sh.nu = nu;
-            sh.ng = ng;
+    const int64_t total = slice0[B];
+    // a wave takes at most `total` slices, so the loop is bounded whatever the cursor holds
+    for (int64_t iter = 0; iter < total; iter++) {
+        unsigned long long gk = 0;
+        if (lane == 0) gk = atomicAdd(cursor, 1ull);
+        gk = __shfl(gk, 0, 64);
+        const int64_t g = (int64_t)gk;
+        if (g >= total) break;
+        // proposal p: slice0[p] <= g < slice0[p+1] (slices are >= 1 per proposal)
+        int64_t lo = 0, hi = B - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (slice0[mid] <= g) lo = mid; else hi = mid - 1;
         }
+        const int64_t pidx = lo;
+        const int s = (int)(g - slice0[pidx]);
+        for (int e = lane; e < MISP_REC; e += 64) srec[e] = rec[pidx * MISP_REC + e];
         __syncthreads();
-        const int nu = sh.nu, ng = sh.ng, nrow = 1 + nu + ng;
-        // rows: A[r][c] = U[mode_r, cols[c]]
+        const int nu = srec[0], ng = srec[1], c = srec[2], slices = srec[3], nrow = srec[6];
+        const uint64_t units = (uint64_t)(uint32_t)srec[4] | ((uint64_t)(uint32_t)srec[5] << 32);
         for (int e = lane; e < nrow * N; e += 64) {
-            const int r = e / N, c = e % N;
-            sh.A[e] = U[(size_t)sh.rmode[r] * m + cols[c]];
+            const int r = e / N, cc = e % N;
+            sh.A[e] = U[(size_t)srec[8 + r] * m + cols[cc]];
         }
+        if (lane < ng) sh.kg[lane] = srec[8 + MISP_MAX_N + lane];
         __syncthreads();
-        // outer size and inner chunking
-        uint64_t O = 1;
-        for (int g = 0; g < ng; g++) O *= (uint64_t)(sh.kg[g] + 1);
-        const int inner_log2 = nu;
-        // steps per unit = 2^c: at least four rounds of units over the lanes
-        // while units stay long (2^6 steps) against their set-up cost, and
-        // at least one unit per lane.
-        int c = inner_log2;
-        while (c > 6 && (O << (inner_log2 - c)) < 256) c--;
-        while (c > 0 && (O << (inner_log2 - c)) < 64) c--;
-        const uint64_t nchunks = 1ull << (inner_log2 - c);
-        const uint64_t units = O * nchunks;
+        const uint64_t nchunks = 1ull << (nu - c);
+        const uint64_t per = (units + (uint64_t)slices - 1) / (uint64_t)slices;
+        const uint64_t u0 = (uint64_t)s * per;
+        const uint64_t u1 = (u0 + per < units) ? u0 + per : units;
         cd acc = {0.0, 0.0};
-        for (uint64_t unit = lane; unit < units; unit += 64) {
+        for (uint64_t unit = u0 + (uint64_t)lane; unit < u1; unit += 64) {
             uint64_t o = unit / nchunks;
             const uint64_t chunk = unit % nchunks;
             const uint64_t base = chunk << c;
@@ -289,8 +357,8 @@ __global__ void __launch_bounds__(64) mis_perm_kernel(const cd *__restrict__ U, 
             }
             // group digits of o (mixed radix k_g + 1) and the weight W_o
             double W = 1.0;
-            for (int g = 0; g < ng; g++) {
-                const int k = sh.kg[g];
+            for (int gg = 0; gg < ng; gg++) {
+                const int k = sh.kg[gg];
                 const int t = (int)(o % (uint64_t)(k + 1));
                 o /= (uint64_t)(k + 1);
                 double binom = 1.0;
@@ -299,8 +367,8 @@ __global__ void __launch_bounds__(64) mis_perm_kernel(const cd *__restrict__ U, 
                 const double f = 0.5 * (double)(k - 2 * t);
 #pragma unroll
                 for (int cc = 0; cc < N; cc++) {
-                    h[cc].r = fma(f, sh.A[(1 + nu + g) * N + cc].r, h[cc].r);
-                    h[cc].i = fma(f, sh.A[(1 + nu + g) * N + cc].i, h[cc].i);
+                    h[cc].r = fma(f, sh.A[(1 + nu + gg) * N + cc].r, h[cc].r);
+                    h[cc].i = fma(f, sh.A[(1 + nu + gg) * N + cc].i, h[cc].i);
                 }
             }
             cd part = {0.0, 0.0};
@@ -377,27 +445,39 @@ __global__ void __launch_bounds__(64) mis_perm_kernel(const cd *__restrict__ U, 
             acc.r += __shfl_xor(acc.r, off, 64);
             acc.i += __shfl_xor(acc.i, off, 64);
         }
-        if (lane == 0) out[b] = {2.0 * acc.r, 2.0 * acc.i};
-        __syncthreads();
+        if (lane == 0) partial[g] = acc;
+        __syncthreads();   // srec and sh are reused by the next slice
     }
 }
 
+// out[p] = 2 * sum of proposal p's partials, in slice order.
+__global__ void mis_finalize_kernel(const cd *__restrict__ partial, const int64_t *__restrict__ slice0, int64_t B,
+                                    cd *__restrict__ out)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    cd acc = {0.0, 0.0};
+    for (int64_t g = slice0[b]; g < slice0[b + 1]; g++) { acc.r += partial[g].r; acc.i += partial[g].i; }
+    out[b] = {2.0 * acc.r, 2.0 * acc.i};
+}
+
 template <int N>
-static int launch_mis_perm(int n, const cd *U, int m, const int32_t *cols, const int32_t *seq, int64_t B, cd *out,
-                           hipStream_t st)
+static int launch_mis_slices(int n, const cd *U, int m, const int32_t *cols, const int32_t *rec, const int64_t *slice0,
+                             int64_t B, unsigned long long *cursor, cd *partial, hipStream_t st)
 {
     if constexpr (N > MISP_MAX_N) {
         set_error("photon number above 28");
         return ERR_ARG;
     } else {
-        if (n != N) return launch_mis_perm<N + 1>(n, U, m, cols, seq, B, out, st);
+        if (n != N) return launch_mis_slices<N + 1>(n, U, m, cols, rec, slice0, B, cursor, partial, st);
         int dev = 0, cus = 0, per_cu = 0;
         PCVL_HIP_CHECK(hipGetDevice(&dev));
         PCVL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&mis_perm_kernel<N>), 64, 0));
+        PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&mis_slice_kernel<N>), 64, 0));
         if (per_cu < 1) per_cu = 1;
-        const int grid = (int)std::min<int64_t>(B, (int64_t)cus * per_cu);
-        hipLaunchKernelGGL((mis_perm_kernel<N>), dim3(grid), dim3(64), 0, st, U, m, cols, seq, B, out);
+        const int grid = (int)std::min<int64_t>(B * MISP_MAX_SLICES, (int64_t)cus * per_cu);
+        hipLaunchKernelGGL((mis_slice_kernel<N>), dim3(grid), dim3(64), 0, st, U, m, cols, rec, slice0, B, cursor,
+                           partial);
         PCVL_HIP_CHECK(hipGetLastError());
         return OK;
     }
@@ -413,8 +493,43 @@ extern "C" int pcvl_mis_permanents(const double *d_U, int m, const int32_t *d_co
     PCVL_ARG_CHECK(m >= 1 && n >= 1 && n <= MISP_MAX_N && B >= 0, "bad sizes (1 <= n <= 28)");
     if (B == 0) return OK;
     PCVL_ARG_CHECK(d_U && d_cols && d_seq && d_out, "null device buffer");
-    return launch_mis_perm<1>(n, reinterpret_cast<const cd *>(d_U), m, d_cols, d_seq, B, reinterpret_cast<cd *>(d_out),
-                              stream);
+    // stream-ordered workspace: records | slice counts | slice offsets | cursor | partials
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t b_rec = al(sizeof(int32_t) * (size_t)B * MISP_REC);
+    const size_t b_ns = al(sizeof(int32_t) * (size_t)B);
+    const size_t b_s0 = al(sizeof(int64_t) * ((size_t)B + 1));
+    const size_t b_cur = al(sizeof(unsigned long long));
+    const size_t b_part = al(sizeof(cd) * (size_t)B * MISP_MAX_SLICES);
+    char *blk = nullptr;
+    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), b_rec + b_ns + b_s0 + b_cur + b_part, stream));
+    int32_t *rec = reinterpret_cast<int32_t *>(blk);
+    int32_t *ns = reinterpret_cast<int32_t *>(blk + b_rec);
+    int64_t *s0 = reinterpret_cast<int64_t *>(blk + b_rec + b_ns);
+    unsigned long long *cur = reinterpret_cast<unsigned long long *>(blk + b_rec + b_ns + b_s0);
+    cd *part = reinterpret_cast<cd *>(blk + b_rec + b_ns + b_s0 + b_cur);
+    int rc = OK;
+    hipLaunchKernelGGL(mis_plan_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, stream, d_seq, B, n, m, rec, ns,
+                       cur);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mis_scan_kernel, dim3(1), dim3(1024), 0, stream, ns, B, s0);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("mis plan: ") + hipGetErrorString(e));
+        rc = ERR_HIP;
+    } else {
+        rc = launch_mis_slices<1>(n, reinterpret_cast<const cd *>(d_U), m, d_cols, rec, s0, B, cur, part, stream);
+    }
+    if (rc == OK) {
+        hipLaunchKernelGGL(mis_finalize_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, stream, part, s0, B,
+                           reinterpret_cast<cd *>(d_out));
+        e = hipGetLastError();
+        if (e != hipSuccess) { set_error(std::string("mis finalize: ") + hipGetErrorString(e)); rc = ERR_HIP; }
+    }
+    hipError_t fe = hipFreeAsync(blk, stream);
+    if (rc == OK && fe != hipSuccess) { set_error(std::string("hipFreeAsync(mis): ") + hipGetErrorString(fe)); rc = ERR_HIP; }
+    return rc;
 }
 
 extern "C" int pcvl_mis_seq_to_occ(const int32_t *d_seq, int64_t B, int n, int m, int32_t *d_occ, hipStream_t stream)

@@ -305,6 +305,136 @@ __global__ void sample_step_kernel(const float *__restrict__ logits, int B, int 
     logq[b] = (p == 0 ? 0.0 : logq[b]) + lp;
 }
 
+// All n autoregressive steps of a tile of proposals in one launch.  The first
+// layer's input is [counts of the modes drawn so far | one-hot of the step],
+// so its pre-activation is updated in place: + W1[:, drawn mode], and the
+// one-hot column moves from m+p to m+p+1 (no GEMM, no HBM round trip).  The
+// second layer runs from LDS (relu(h)^T in fp32, W2^T in bf16), each thread
+// computing a 4-sample x 2-logit tile.  The draw is the masked softmax of the
+// logits with the uniform (seed, first + b, p), done by the sample's 8
+// threads: group max, group sum of exp (double), ordered partial sums for the
+// inverse CDF; log q accumulates (z_pick - max) - log(sum) from the same
+// logits, so q is exactly the law of the draw.
+constexpr int FS_TB = 16;            // proposals per workgroup
+constexpr int FS_THREADS = 128;      // 8 per proposal
+constexpr int FS_MW = 64;            // logits padded to 64 modes
+constexpr int FS_MAX_H = 256;
+constexpr int FS_HPT = FS_MAX_H / 8; // h_pre entries per thread
+
+__global__ void __launch_bounds__(FS_THREADS) sample_fused_kernel(
+    const uint16_t *__restrict__ W1, const float *__restrict__ b1, const uint16_t *__restrict__ W2,
+    const float *__restrict__ b2, int B, int m, int n, int H, uint64_t seed, uint64_t first,
+    int32_t *__restrict__ seq, double *__restrict__ logq)
+{
+    extern __shared__ float s_fs[];
+    float *hT = s_fs;                                           // H x TB
+    float *zs = hT + (size_t)H * FS_TB;                         // TB x 64
+    uint16_t *W2T = reinterpret_cast<uint16_t *>(zs + FS_TB * FS_MW);   // H x 64
+    const int tid = threadIdx.x;
+    const int D = m + n;
+    for (int e = tid; e < H * FS_MW; e += FS_THREADS) {
+        const int h = e / FS_MW, j = e % FS_MW;
+        W2T[e] = j < m ? W2[(size_t)j * H + h] : (uint16_t)0;
+    }
+    // ownership for h_pre and the draw: proposal s, sub-thread sub
+    const int s = tid >> 3, sub = tid & 7;
+    const int64_t b = (int64_t)blockIdx.x * FS_TB + s;
+    const bool valid = b < B;
+    const int hpt = H / 8;
+    float hp[FS_HPT];
+#pragma unroll
+    for (int t = 0; t < FS_HPT; t++) {
+        const int h = sub + 8 * t;
+        hp[t] = (t < hpt) ? b1[h] + bf16_to_f32(W1[(size_t)h * D + m]) : 0.0f;
+    }
+    // z tile: proposals 4*sq .. 4*sq+3, logits 2*jp, 2*jp+1
+    const int sq = tid & 3, jp = tid >> 2;
+    int lo = 0;
+    double lq = 0.0;
+    __syncthreads();
+    for (int p = 0; p < n; p++) {
+#pragma unroll
+        for (int t = 0; t < FS_HPT; t++)
+            if (t < hpt) hT[(sub + 8 * t) * FS_TB + s] = hp[t] > 0.0f ? hp[t] : 0.0f;
+        __syncthreads();
+        float acc[4][2] = {};
+        for (int h = 0; h < H; h++) {
+            const float4 hv = *reinterpret_cast<const float4 *>(hT + h * FS_TB + 4 * sq);
+            const uint32_t w2 = *reinterpret_cast<const uint32_t *>(W2T + h * FS_MW + 2 * jp);
+            const float w0 = __uint_as_float(w2 << 16), w1 = __uint_as_float(w2 & 0xffff0000u);
+            acc[0][0] = fmaf(hv.x, w0, acc[0][0]); acc[0][1] = fmaf(hv.x, w1, acc[0][1]);
+            acc[1][0] = fmaf(hv.y, w0, acc[1][0]); acc[1][1] = fmaf(hv.y, w1, acc[1][1]);
+            acc[2][0] = fmaf(hv.z, w0, acc[2][0]); acc[2][1] = fmaf(hv.z, w1, acc[2][1]);
+            acc[3][0] = fmaf(hv.w, w0, acc[3][0]); acc[3][1] = fmaf(hv.w, w1, acc[3][1]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int j = 2 * jp + c;
+                zs[(4 * sq + r) * FS_MW + j] = j < m ? acc[r][c] + b2[j] : 0.0f;
+            }
+        __syncthreads();
+        // draw of proposal s by its 8 threads; thread sub owns modes 8*sub .. 8*sub+7
+        const float *z = zs + s * FS_MW;
+        float mx = -3.0e38f;
+        for (int u = 0; u < 8; u++) {
+            const int j = 8 * sub + u;
+            if (j >= lo && j < m) mx = fmaxf(mx, z[j]);
+        }
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        double part = 0.0;
+        for (int u = 0; u < 8; u++) {
+            const int j = 8 * sub + u;
+            if (j >= lo && j < m) part += exp((double)(z[j] - mx));
+        }
+        double se = part;
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) se += __shfl_xor(se, o, 64);
+        // exclusive prefix of the partial sums in sub order
+        double base = 0.0;
+        for (int v = 0; v < 7; v++) {
+            const double pv = __shfl(part, (tid & ~7) + v, 64);
+            if (v < sub) base += pv;
+        }
+        const double target = uniform01(seed, first + (uint64_t)(valid ? b : 0), (uint64_t)p) * se;
+        int cand = -1;
+        if (target >= base || sub == 0) {
+            double cum = base;
+            for (int u = 0; u < 8 && cand < 0; u++) {
+                const int j = 8 * sub + u;
+                if (j >= lo && j < m) {
+                    cum += exp((double)(z[j] - mx));
+                    if (target < cum) cand = j;
+                }
+            }
+            if (target < base) cand = -1;   // belongs to an earlier thread
+        }
+        // the first thread (in sub order) that holds the target; none -> last mode
+        int pick = m - 1;
+        for (int v = 7; v >= 0; v--) {
+            const int cv = __shfl(cand, (tid & ~7) + v, 64);
+            if (cv >= 0) pick = cv;
+        }
+        lq += (double)(z[pick] - mx) - log(se);
+        if (valid && sub == 0) seq[b * n + p] = pick;
+        lo = pick;
+        // next pre-activation: + W1[:, pick], one-hot m+p -> m+p+1
+#pragma unroll
+        for (int t = 0; t < FS_HPT; t++) {
+            if (t < hpt) {
+                const size_t row = (size_t)(sub + 8 * t) * D;
+                float v = hp[t] + bf16_to_f32(W1[row + pick]) - bf16_to_f32(W1[row + m + p]);
+                if (p + 1 < n) v += bf16_to_f32(W1[row + m + p + 1]);
+                hp[t] = v;
+            }
+        }
+        __syncthreads();   // zs and hT are rewritten by the next step
+    }
+    if (valid && sub == 0) logq[b] = lq;
+}
+
 } // namespace pcvl
 
 using namespace pcvl;
@@ -426,6 +556,22 @@ extern "C" int pcvl_mis_features_all(const int32_t *d_seq, int B, int n, int m, 
     if (B <= 0) return OK;
     hipLaunchKernelGGL(features_all_kernel, dim3(grid1((int64_t)B * n)), dim3(256), 0, stream, d_seq, B, n, m, d_x,
                        d_lo, d_target);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mis_sample_fused(const uint16_t *d_W1, const float *d_b1, const uint16_t *d_W2,
+                                     const float *d_b2, int B, int m, int n, int H, uint64_t seed, uint64_t first,
+                                     int32_t *d_seq, double *d_logq, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(m >= 1 && m <= FS_MW && n >= 1 && H >= 8 && H <= FS_MAX_H && H % 8 == 0,
+                   "fused sampler needs 1 <= m <= 64, n >= 1, 8 <= H <= 256, H % 8 == 0");
+    if (B <= 0) return OK;
+    PCVL_ARG_CHECK(d_W1 && d_b1 && d_W2 && d_b2 && d_seq && d_logq, "null device buffer");
+    const size_t lds = sizeof(float) * ((size_t)H * FS_TB + FS_TB * FS_MW) + sizeof(uint16_t) * (size_t)H * FS_MW;
+    const unsigned grid = (unsigned)(((int64_t)B + FS_TB - 1) / FS_TB);
+    hipLaunchKernelGGL(sample_fused_kernel, dim3(grid), dim3(FS_THREADS), lds, stream, d_W1, d_b1, d_W2, d_b2, B, m, n,
+                       H, seed, first, d_seq, d_logq);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
 }

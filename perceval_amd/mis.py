@@ -164,12 +164,28 @@ class ProposalNet:
                       "ce")
         return -loss.double().view(-1, self.n).sum(dim=1)
 
-    def sample(self, B: int, seed: int, first: int = 0):
-        """B sequences drawn autoregressively; returns (seq (B x n) int32, log q (B,) float64)."""
+    def fused_ok(self) -> bool:
+        """Whether the one-launch sampler (pcvl_mis_sample_fused) covers this shape."""
+        return self.m <= 64 and 8 <= self.H <= 256 and self.H % 8 == 0
+
+    def sample(self, B: int, seed: int, first: int = 0, fused: bool | None = None):
+        """B sequences drawn autoregressively; returns (seq (B x n) int32, log q (B,) float64).
+        The fused sampler runs all n steps of a tile of proposals in one launch
+        (first layer updated in place); the stepwise path runs one
+        features / GEMM / GEMM / draw sequence per step.  Both draw step p of
+        proposal b with the uniform (seed, first + b, p) and return the exact
+        log q of what they drew."""
         import torch
         lib = _lib()
         seq = torch.zeros((B, self.n), dtype=torch.int32, device="cuda")
         logq = torch.zeros(B, dtype=torch.float64, device="cuda")
+        if fused is None:
+            fused = self.fused_ok()
+        if fused:
+            _native.check(lib.pcvl_mis_sample_fused(_p(self.W1b), _p(self.b1), _p(self.W2b), _p(self.b2), B, self.m,
+                                                    self.n, self.H, int(seed), int(first), _p(seq), _p(logq), _s()),
+                          "sample_fused")
+            return seq, logq
         x = torch.empty((B, self.D), dtype=torch.float32, device="cuda")
         lo = torch.empty(B, dtype=torch.int32, device="cuda")
         for p in range(self.n):

@@ -123,6 +123,46 @@ def test_sampling_gives_exact_log_q(gpu):
     assert abs(tot - 1.0) < 1e-4
 
 
+def test_fused_sampler_matches_stepwise_and_its_own_q(gpu):
+    """The one-launch sampler draws with the same uniforms as the stepwise
+    path, so the draws coincide except where fp32 rounding of the two
+    evaluation orders moves a CDF boundary; its log q is exact for what it drew
+    and its empirical law matches q."""
+    import torch
+    from perceval_amd.mis import ProposalNet
+    m, n = 12, 5
+    net = ProposalNet(m, n, hidden=64, seed=4)
+    g = torch.Generator().manual_seed(2)
+    net.b1.copy_(torch.randn(64, generator=g).float().cuda() * 0.2)
+    net.b2.copy_(torch.randn(m, generator=g).float().cuda() * 0.5)
+    net.W2.mul_(8.0)
+    net.refresh_bf16()
+    assert net.fused_ok()
+    sf, qf = net.sample(4096, seed=21, first=7)
+    ss, qs = net.sample(4096, seed=21, first=7, fused=False)
+    same = (sf == ss).all(dim=1)
+    assert same.float().mean().item() > 0.99
+    assert torch.allclose(qf[same], qs[same], atol=1e-4)
+    assert torch.allclose(qf, net.log_q(sf), atol=2e-4)
+    # empirical law of 200k fused draws against q over the whole space (m=6, n=3: 56 states)
+    m2, n2 = 6, 3
+    net2 = ProposalNet(m2, n2, hidden=32, seed=5)
+    net2.W2.mul_(10.0)
+    net2.refresh_bf16()
+    seqs, _ = net2.sample(200000, seed=3)
+    states = oracle.fock_enumerate(m2, n2)
+    allseq = np.array([[j for j, v in enumerate(r) for _ in range(v)] for r in states], dtype=np.int32)
+    q = torch.exp(net2.log_q(torch.from_numpy(allseq).cuda())).cpu().numpy()
+    assert abs(q.sum() - 1.0) < 1e-4
+    key = {tuple(r): i for i, r in enumerate(allseq.tolist())}
+    cnt = np.zeros(len(allseq))
+    for r in seqs.cpu().numpy().tolist():
+        cnt[key[tuple(r)]] += 1
+    tvd = 0.5 * np.abs(cnt / cnt.sum() - q).sum()
+    _say(f"fused sampler TVD to q: {tvd:.4f}")
+    assert tvd < 0.01
+
+
 def test_log_p_vs_oracle(gpu):
     import torch
     from perceval_amd.mis import LearnedMIS, ProposalNet
@@ -158,6 +198,38 @@ def test_multiset_permanents_vs_oracle(gpu, m, n):
         occ = np.bincount(s, minlength=m)
         ref = oracle.perm(oracle.submatrix(U, inp, occ))
         assert abs(g - ref) <= 1e-10 * abs(ref) + 1e-14, (s, g, ref)
+    mis.close()
+
+
+def test_multiset_permanents_mixed_costs_deterministic(gpu):
+    """A batch mixing collision-free, bunched and single-mode proposals (costs
+    from 1 to 2^15 terms), against the oracle, and bit-identical when run twice."""
+    import torch
+    from perceval_amd.mis import LearnedMIS, ProposalNet
+    m, n = 40, 16
+    U = haar_unitary(m, seed=5)
+    inp = [1] * n + [0] * (m - n)
+    rng = np.random.RandomState(1)
+    seqs = []
+    for k in range(300):
+        kind = k % 4
+        if kind == 0:
+            seqs.append(np.sort(rng.choice(m, n, replace=False)))
+        elif kind == 1:
+            seqs.append(np.sort(rng.randint(0, m, size=n)))
+        elif kind == 2:
+            seqs.append(np.sort(rng.randint(0, 4, size=n)))
+        else:
+            seqs.append(np.full(n, rng.randint(m)))
+    seqs = np.array(seqs, dtype=np.int32)
+    mis = LearnedMIS(U, inp, ProposalNet(m, n, hidden=8), chunk=512)
+    d = torch.from_numpy(seqs).cuda()
+    a = mis.permanents(d).cpu().numpy()
+    b = mis.permanents(d).cpu().numpy()
+    assert np.array_equal(a, b)
+    for s_, g in list(zip(seqs, a))[::7]:
+        ref = oracle.perm(oracle.submatrix(U, inp, np.bincount(s_, minlength=m)))
+        assert abs(g - ref) <= 1e-10 * abs(ref) + 1e-14, (s_, g, ref)
     mis.close()
 
 
