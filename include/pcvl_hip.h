@@ -97,6 +97,20 @@ int pcvl_fock_rank(const int32_t *d_occ, int64_t count, int m, int nmax, int64_t
 int pcvl_fock_unrank(const int64_t *d_rank, int64_t count, int m, int n, int32_t *d_occ,
                      hipStream_t stream);
 
+/* Ranks (FSArray order) of the n-photon, m-mode states that match any of the
+ * nmask masks (AStrongSimulationBackend.set_mask / xq.FSMask,
+ * perceval/backends/_abstract_backends.py:103-146): h_cond holds nmask rows of
+ * m conditions, -1 for any count, else a count matched exactly, or reached at
+ * least on the modes where h_at_least (m flags, may be NULL) is non-zero.
+ * d_out must hold C(m+n-1, n) entries; *d_count (device) receives the number
+ * of matches.  At most 16 masks and 2^34 states.  Synchronises `stream`. */
+int pcvl_fock_mask_select(int m, int n, const int32_t *h_cond, int nmask, const int32_t *h_at_least,
+                          int64_t *d_out, int64_t *d_count, hipStream_t stream);
+
+/* d_dst[k] = d_src[d_idx[k]] (float64), e.g. the probabilities of a mask's
+ * states. */
+int pcvl_gather_f64(const double *d_src, const int64_t *d_idx, int64_t count, double *d_dst, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * SLOS -- replaces _Path._decompose/_Path.compute -> FSMap.compute_slos_layer
  * and xq.all_prob_normalize_output (perceval/backends/_slos.py:61-102,

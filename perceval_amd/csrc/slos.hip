@@ -478,6 +478,119 @@ __global__ void fock_unrank_kernel(const int64_t *__restrict__ ranks, int64_t co
     }
 }
 
+// ---------------------------------------------------------------- masks
+//
+// FSMask semantics (AStrongSimulationBackend.set_mask,
+// perceval/backends/_abstract_backends.py:103-146; tests/utils/test_mask.py:
+// 33-45): a mask holds one condition per mode, -1 for any count, else a count
+// that must match exactly, or be reached at least on the modes flagged in
+// at_least; masks are OR-ed.  Selection runs over the ranks of the n-photon
+// space: a count pass unranks each rank, tests it and counts matches per
+// block; a one-block scan turns counts into offsets; a scatter pass writes the
+// matching ranks in rank order (FSArray order).
+constexpr int MASK_THREADS = 256;
+constexpr int MASK_MAX = 16;   // masks per selection
+
+// Walks rank rk of the n-photon space and tests every mask on the way; bit k
+// of the result is set when mask k matches.
+__device__ __forceinline__ uint32_t mask_walk(uint64_t rk, int m, int n, const uint64_t *sT, int w,
+                                              const int32_t *cond, int nmask, const int32_t *at_least)
+{
+    uint32_t ok = (nmask >= 32) ? 0xffffffffu : ((1u << nmask) - 1u);
+    uint64_t rem = rk;
+    int R = n;
+    for (int q = 0; q < m; q++) {
+        int Rn = 0;
+        if (q + 1 < m) {
+            const uint64_t *tq = sT + q * w;
+            Rn = R;
+            uint64_t tv = tq[Rn];
+            while (Rn > 0 && tv > rem) { --Rn; tv = tq[Rn]; }
+            rem -= tv;
+        }
+        const int sq = R - Rn;
+        for (int k = 0; k < nmask; k++) {
+            const int c = cond[k * m + q];
+            if (c >= 0 && !(at_least[q] ? sq >= c : sq == c)) ok &= ~(1u << k);
+        }
+        R = Rn;
+    }
+    return ok;
+}
+
+// flags[i] = match of rank i; counts[b] = matches in block b.
+__global__ void __launch_bounds__(MASK_THREADS) mask_count_kernel(uint64_t total, int m, int n,
+                                                                  const uint64_t *__restrict__ gT, int w,
+                                                                  const int32_t *__restrict__ gcond, int nmask,
+                                                                  const int32_t *__restrict__ gat,
+                                                                  uint8_t *__restrict__ flags,
+                                                                  int32_t *__restrict__ counts)
+{
+    __shared__ uint64_t sT[SLOS_MAX_MODES * 64];
+    __shared__ int32_t scond[MASK_MAX * SLOS_MAX_MODES];
+    __shared__ int32_t sat[SLOS_MAX_MODES];
+    for (int e = threadIdx.x; e < m * w; e += MASK_THREADS) sT[e] = gT[e];
+    for (int e = threadIdx.x; e < nmask * m; e += MASK_THREADS) scond[e] = gcond[e];
+    for (int e = threadIdx.x; e < m; e += MASK_THREADS) sat[e] = gat[e];
+    __syncthreads();
+    const uint64_t i = (uint64_t)blockIdx.x * MASK_THREADS + threadIdx.x;
+    bool hit = false;
+    if (i < total) {
+        hit = mask_walk(i, m, n, sT, w, scond, nmask, sat) != 0u;
+        flags[i] = hit ? 1 : 0;
+    }
+    const int c = __syncthreads_count(hit);
+    if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// Exclusive prefix of the block counts (one block); offsets[nb] = total matches.
+__global__ void __launch_bounds__(1024) mask_scan_kernel(const int32_t *__restrict__ counts, int64_t nb,
+                                                         int64_t *__restrict__ offsets, int64_t *__restrict__ d_count)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t lo = t * per, hi = (lo + per < nb) ? lo + per : nb;
+    int64_t sum = 0;
+    for (int64_t i = lo; i < hi; i++) sum += counts[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - sum;
+    for (int64_t i = lo; i < hi; i++) { offsets[i] = run; run += counts[i]; }
+    if (t == 1023) { offsets[nb] = part[1023]; *d_count = part[1023]; }
+}
+
+// Matching ranks, in rank order.
+__global__ void __launch_bounds__(MASK_THREADS) mask_scatter_kernel(uint64_t total, const uint8_t *__restrict__ flags,
+                                                                    const int64_t *__restrict__ offsets,
+                                                                    int64_t *__restrict__ out)
+{
+    __shared__ int32_t wave_base[MASK_THREADS / 64];
+    const uint64_t i = (uint64_t)blockIdx.x * MASK_THREADS + threadIdx.x;
+    const bool hit = i < total && flags[i];
+    const uint64_t bal = __ballot(hit);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int before = __builtin_popcountll(bal & ((1ull << lane) - 1));
+    if (lane == 0) wave_base[wv] = __builtin_popcountll(bal);
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < wv; k++) base += wave_base[k];
+    if (hit) out[offsets[blockIdx.x] + base + before] = (int64_t)i;
+}
+
+__global__ void gather_f64_kernel(const double *__restrict__ src, const int64_t *__restrict__ idx, int64_t count,
+                                  double *__restrict__ dst)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) dst[k] = src[idx[k]];
+}
+
 struct SlosPlanImpl {
     int m = 0, n = 0, w = 0;
     std::vector<int32_t> mk;          // input mode of each added photon
@@ -1022,4 +1135,73 @@ extern "C" int64_t pcvl_fock_count(int m, int n)
     if (m < 1 || n < 0) return (m == 0 && n == 0) ? 1 : 0;
     uint64_t c = binom_u64((int64_t)m + n - 1, n);
     return c > (uint64_t)INT64_MAX ? -1 : (int64_t)c;
+}
+
+extern "C" int pcvl_fock_mask_select(int m, int n, const int32_t *h_cond, int nmask, const int32_t *h_at_least,
+                                     int64_t *d_out, int64_t *d_count, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 64]");
+    PCVL_ARG_CHECK(n >= 0 && n <= 63, "photon count outside [0, 63]");
+    PCVL_ARG_CHECK(nmask >= 1 && nmask <= MASK_MAX, "between 1 and 16 masks");
+    PCVL_ARG_CHECK(h_cond && d_out && d_count, "null buffer");
+    const uint64_t total = binom_u64((int64_t)m + n - 1, n);
+    PCVL_ARG_CHECK(total != UINT64_MAX && total < (1ull << 34), "Fock space too large (2^34 states or more)");
+    const int w = n + 1;
+    std::vector<uint64_t> T, E;
+    build_tables(m, n, T, E);
+    std::vector<int32_t> at((size_t)m, 0);
+    if (h_at_least) for (int q = 0; q < m; q++) at[(size_t)q] = h_at_least[q] ? 1 : 0;
+    const int64_t nb = (int64_t)((total + MASK_THREADS - 1) / MASK_THREADS);
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t b_T = al(sizeof(uint64_t) * T.size());
+    const size_t b_c = al(sizeof(int32_t) * (size_t)nmask * m);
+    const size_t b_a = al(sizeof(int32_t) * (size_t)m);
+    const size_t b_cnt = al(sizeof(int32_t) * (size_t)nb);
+    const size_t b_off = al(sizeof(int64_t) * ((size_t)nb + 1));
+    const size_t b_fl = al((size_t)total);
+    char *blk = nullptr;
+    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), b_T + b_c + b_a + b_cnt + b_off + b_fl, stream));
+    uint64_t *dT = reinterpret_cast<uint64_t *>(blk);
+    int32_t *dc = reinterpret_cast<int32_t *>(blk + b_T);
+    int32_t *da = reinterpret_cast<int32_t *>(blk + b_T + b_c);
+    int32_t *dcnt = reinterpret_cast<int32_t *>(blk + b_T + b_c + b_a);
+    int64_t *doff = reinterpret_cast<int64_t *>(blk + b_T + b_c + b_a + b_cnt);
+    uint8_t *dfl = reinterpret_cast<uint8_t *>(blk + b_T + b_c + b_a + b_cnt + b_off);
+    int rc = OK;
+    hipError_t e = hipMemcpyAsync(dT, T.data(), sizeof(uint64_t) * T.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dc, h_cond, sizeof(int32_t) * (size_t)nmask * m, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(da, at.data(), sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mask_count_kernel, dim3((unsigned)nb), dim3(MASK_THREADS), 0, stream, total, m, n, dT, w, dc,
+                           nmask, da, dfl, dcnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mask_scan_kernel, dim3(1), dim3(1024), 0, stream, dcnt, nb, doff, d_count);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mask_scatter_kernel, dim3((unsigned)nb), dim3(MASK_THREADS), 0, stream, total, dfl, doff,
+                           d_out);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) { set_error(std::string("fock mask select: ") + hipGetErrorString(e)); rc = ERR_HIP; }
+    // the host tables must outlive their copies
+    hipError_t se = hipStreamSynchronize(stream);
+    hipError_t fe = hipFreeAsync(blk, stream);
+    if (rc == OK && se != hipSuccess) { set_error(std::string("fock mask select: ") + hipGetErrorString(se)); rc = ERR_HIP; }
+    if (rc == OK && fe != hipSuccess) { set_error(std::string("hipFreeAsync(mask): ") + hipGetErrorString(fe)); rc = ERR_HIP; }
+    return rc;
+}
+
+extern "C" int pcvl_gather_f64(const double *d_src, const int64_t *d_idx, int64_t count, double *d_dst,
+                               hipStream_t stream)
+{
+    PCVL_ARG_CHECK(count >= 0, "count < 0");
+    if (count == 0) return OK;
+    PCVL_ARG_CHECK(d_src && d_idx && d_dst, "null device buffer");
+    hipLaunchKernelGGL(gather_f64_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, d_src, d_idx,
+                       count, d_dst);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
 }
