@@ -49,6 +49,12 @@ if ROOT not in sys.path:
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 vector, FMA = 2 flops
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the permanent kernel on the headline workload (4096
+# matrices of order 20), measured with rocprofv3 --pmc on MI355X: FETCH_SIZE
+# 30,798 KiB x 2.0 + WRITE_SIZE 1,313 KiB x 1.0, the factors that map a plain
+# 1 GiB device copy onto 1 GiB read and 1 GiB written.  bench.py cannot read
+# hardware counters itself, so the measured value is carried here.
+PERM_N20_B4096_HBM_BYTES = (30798 * 2 + 1313) * 1024
 SECTIONS = ("perm", "mixed", "slos", "slos_tree", "gradient", "sampling", "sampling_multi", "gurvits", "copy")
 
 
@@ -526,7 +532,12 @@ def main():
         "config": {"workload": f"batched Glynn permanents, {args.batch} matrices of order {args.order}",
                    "batch": args.batch, "order": args.order, "parallelism": "single GPU"},
         "roofline": {"bound": "fp64-valu", "achieved": perm["tflops"], "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": perm["tflops"] / FP64_PEAK_TFLOPS, "traffic": None},
+                     "unit": "TFLOP/s", "frac": perm["tflops"] / FP64_PEAK_TFLOPS,
+                     "traffic": PERM_N20_B4096_HBM_BYTES if (args.order, args.batch) == (20, 4096) else None,
+                     "compute": "FP64 vector ALU (no MFMA: Glynn's products are not a dense contraction); the peak "
+                                "is MI355X's FP64 vector figure, equal to its dense FP64 MFMA figure",
+                     "attainable_fp64_fma_tflops": perm.get("fp64_fma_probe_tflops"),
+                     "frac_of_attainable": perm.get("frac_of_probe")},
         "cpu_baseline": res.get("perm_cpu"),
     }
     for k, v in res.items():

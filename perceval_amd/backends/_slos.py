@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from ..slos import COEFS, PROBS, SlosPlan, SlosTree
+from ..slos import COEFS, PROBS, SlosPlan, SlosTree, fock_mask_select, gather_f64
 from ..utils.states import BSDistribution, FockState, StateVector, fock_space
 from ._abstract_backends import AStrongSimulationBackend, _as_circuit
 
@@ -122,11 +122,19 @@ class SLOSBackend(AStrongSimulationBackend):
                 self._results[key] = plan.run(self._umat, mode).cpu().numpy()
         return self._results[key]
 
-    def _masked_indices(self, istate):
-        rows = fock_space(istate.m, istate.n)
+    def _masked_indices_device(self, istate):
+        """Ranks of the output states the mask keeps (CUDA int64), selected on
+        the GPU and cached per photon number, or None without a mask."""
         if self._mask is None:
             return None
-        return np.array([i for i, r in enumerate(rows) if self._mask.match(FockState(r))], dtype=np.int64)
+        key = ("mask_idx", istate.m, istate.n)
+        if key not in self._results:
+            self._results[key] = fock_mask_select(istate.m, istate.n, self._masks_str, self._no_limit_modes)
+        return self._results[key]
+
+    def _masked_indices(self, istate):
+        idx = self._masked_indices_device(istate)
+        return None if idx is None else idx.cpu().numpy()
 
     def prob_amplitude(self, output_state) -> complex:
         output_state = FockState(output_state)
@@ -141,9 +149,11 @@ class SLOSBackend(AStrongSimulationBackend):
         """SLOS specific signature, to enhance optimization in some computations"""
         if input_state is not None:
             self.set_input_state(input_state)
-        probs = self._compute(self._input_state, PROBS)
-        sel = self._masked_indices(self._input_state)
-        return (probs if sel is None else probs[sel]).tolist()
+        if self._mask is None:
+            return self._compute(self._input_state, PROBS).tolist()
+        # masked: select on the GPU and copy back only the kept probabilities
+        idx = self._masked_indices_device(self._input_state)
+        return gather_f64(self.all_prob_tensor(), idx).cpu().numpy().tolist()
 
     def all_prob_tensor(self, input_state=None):
         """Probabilities of every output state as a CUDA float64 tensor (no host copy)."""

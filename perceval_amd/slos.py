@@ -172,3 +172,43 @@ def fock_unrank(ranks, m: int, n: int) -> np.ndarray:
     _native.check(lib.pcvl_fock_unrank(_native.ptr(d_r), len(ranks), m, n, _native.ptr(d_out),
                                        _native.stream_handle()), "pcvl_fock_unrank")
     return d_out.cpu().numpy()
+
+
+def mask_conditions(masks, m: int) -> np.ndarray:
+    """FSMask strings -> (nmask, m) int32 conditions (-1 = any count)."""
+    if isinstance(masks, str):
+        masks = [masks]
+    rows = []
+    for mk in masks:
+        mk = mk.replace("*", " ")
+        if len(mk) != m:
+            raise ValueError("mask length must equal the number of modes")
+        rows.append([-1 if ch == " " else int(ch) for ch in mk])
+    return np.ascontiguousarray(np.asarray(rows, dtype=np.int32))
+
+
+def fock_mask_select(m: int, n: int, masks, at_least_modes=None):
+    """Ranks (FSArray order) of the n-photon states matching the masks, as a
+    CUDA int64 tensor, selected on the GPU (pcvl_fock_mask_select)."""
+    import torch
+    lib = _native.load()
+    cond = mask_conditions(masks, m)
+    at = np.zeros(m, dtype=np.int32)
+    for q in (at_least_modes or []):
+        at[int(q)] = 1
+    total = fock_count(m, n)
+    out = torch.empty(max(1, total), dtype=torch.int64, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _native.check(lib.pcvl_fock_mask_select(m, n, cond.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cond.shape[0],
+                                            at.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _native.ptr(out),
+                                            _native.ptr(cnt), _native.stream_handle()), "pcvl_fock_mask_select")
+    return out[:int(cnt.item())]
+
+
+def gather_f64(src, idx):
+    """src[idx] for a CUDA float64 tensor and CUDA int64 indices (pcvl_gather_f64)."""
+    import torch
+    out = torch.empty(max(1, idx.numel()), dtype=torch.float64, device=src.device)
+    _native.check(_native.load().pcvl_gather_f64(_native.ptr(src), _native.ptr(idx), idx.numel(), _native.ptr(out),
+                                                 _native.stream_handle()), "pcvl_gather_f64")
+    return out[:idx.numel()]
