@@ -248,12 +248,25 @@ def bench_mixed(args, res):
     mout = torch.empty(len(mo), dtype=torch.complex128, device="cuda")
     mplan = PermanentPlan(mo, max_chunk_log2=args.chunk_log2)
     mms = time_plan(mplan, md, mout, max(3, args.steps // 4), 1)
+    # the same launch captured in a HIP graph and replayed
+    from perceval_amd.graphs import graphed_permanents
+    g = graphed_permanents(mplan, md, mout)
+    with Timer() as t:
+        for _ in range(3):
+            g.replay()
+    graph_ms = t.ms / 3
     mplan.close()
+    # the project's autotuner on this batch shape (chunk length), cached per device
+    from perceval_amd import autotune
+    tuned = autotune.tune_permanent_mixed(mo, reps=2)
     mflops = sum(glynn_flops(int(k)) for k in mo)
     res["mixed_n12_24"] = {"perms_per_s": len(mo) / (mms * 1e-3), "ms_per_step": mms,
                            "tflops": mflops / (mms * 1e-3) / 1e12,
-                           "frac_fp64_peak": mflops / (mms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
-    log(f"[bench] mixed n=12..24: {mms:.3f} ms/step, {res['mixed_n12_24']['tflops']:.2f} TFLOP/s")
+                           "frac_fp64_peak": mflops / (mms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                           "ms_graph_replay": graph_ms, "autotuned_chunk_log2": tuned["chunk_log2"],
+                           "autotuned_ms": tuned["ms"]}
+    log(f"[bench] mixed n=12..24: {mms:.3f} ms/step, {res['mixed_n12_24']['tflops']:.2f} TFLOP/s; HIP-graph replay "
+        f"{graph_ms:.3f} ms; autotuned chunk 2^{tuned['chunk_log2']}: {tuned['ms']:.3f} ms")
 
 
 def bench_slos(args, res):
@@ -268,9 +281,10 @@ def bench_slos(args, res):
     out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
     reps = 5
     per_kernel = {}
-    # 1 per state, 2 incremental runs, 3 per state with 32-bit ranks, 4 per
-    # state in two phases, 0 the library's own pick (the headline number)
-    for kernel in (1, 2, 3, 4, 0):
+    # 1 per state, 2 incremental runs, 3 per state with 32-bit ranks, 4 / 5 per
+    # state in two phases (4 / 8 parent loads in flight), 0 the library's own
+    # pick (the headline number)
+    for kernel in (1, 2, 3, 4, 5, 0):
         plan.set_kernel(kernel)
         plan.run(dU, PROBS, out)
         with Timer() as t:
@@ -285,7 +299,8 @@ def bench_slos(args, res):
                            "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,
                            "sum_probs": total, "ms_by_kernel": {"per_state": per_kernel[1], "incremental": per_kernel[2],
                                                  "per_state_32bit": per_kernel[3],
-                                                 "two_phase": per_kernel[4]}}
+                                                 "two_phase": per_kernel[4],
+                                                 "two_phase_8_loads": per_kernel[5]}}
     log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}")
     if not args.no_cpu:
         import oracle

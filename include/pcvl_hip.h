@@ -164,8 +164,9 @@ int pcvl_slos_tree_destroy(pcvl_slos_tree plan);
  * consecutive output states (layers below 2^32 states and 64 KiB of LDS
  * only), 3 one thread per output state with 32-bit ranks (layers below 2^32
  * states), 4 one thread per output state with the walk and the parent gather
- * in two phases (parent layers below 2^26 states).  Kernels not applicable to
- * the plan give -1.  For comparisons. */
+ * in two phases (parent layers below 2^26 states), 5 as 4 with eight parent
+ * loads in flight instead of four.  Kernels not applicable to the plan give
+ * -1.  For comparisons. */
 int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel);
 
 /* ---------------------------------------------------------------------------

@@ -2,10 +2,11 @@
 
 For a batch shape (order n, batch size) it times every kernel variant (row 1
 of the Gray code in VGPRs or in LDS; row flips as a separate phase or fused
-into the product) and lane-chunk length that the plan API
-accepts, on the current GPU, and keeps the fastest.  Results are cached in a
-JSON file keyed by device name, order and batch size, so tuning runs once per
-machine.
+into the product) and lane-chunk length that the plan API accepts, on the
+current GPU, and keeps the fastest; for a mixed-order batch it tunes the
+chunk length; for a SLOS plan it times every applicable layer kernel.
+Results are cached in a JSON file keyed by device name and shape, so tuning
+runs once per machine.
 """
 from __future__ import annotations
 
@@ -84,9 +85,109 @@ def tune_permanent(n: int, batch: int = 4096, reps: int = 5, use_cache: bool = T
 
 
 def tuned_plan(orders, batch_hint: int | None = None) -> PermanentPlan:
-    """A PermanentPlan for `orders`, tuned when every order is the same and >= 9."""
+    """A PermanentPlan for `orders`: variant and chunk tuned when every order is
+    the same and >= 9, chunk tuned for a mixed batch with orders >= 9."""
     orders = np.asarray(orders, dtype=np.int32).ravel()
     if orders.size and orders.min() == orders.max() and orders[0] >= 9:
         t = tune_permanent(int(orders[0]), batch_hint or int(orders.size))
         return PermanentPlan(orders, max_chunk_log2=t["chunk_log2"], variant=t["variant"])
+    if orders.size and orders.max() >= 9:
+        t = tune_permanent_mixed(orders)
+        return PermanentPlan(orders, max_chunk_log2=t["chunk_log2"])
     return PermanentPlan(orders)
+
+
+def _orders_key(orders) -> str:
+    import hashlib
+    o = np.asarray(orders, dtype=np.int32).ravel()
+    return hashlib.sha1(o.tobytes()).hexdigest()[:16]
+
+
+def tune_permanent_mixed(orders, reps: int = 3, use_cache: bool = True) -> dict:
+    """Fastest chunk length for a mixed-order batch (one persistent launch).
+    Returns {"chunk_log2", "ms", "trials": [...]}."""
+    import torch
+    orders = np.asarray(orders, dtype=np.int32).ravel()
+    cache = _load() if use_cache else {}
+    key = f"{torch.cuda.get_device_name(0)}|perm-mixed|{_orders_key(orders)}"
+    if key in cache:
+        return cache[key]
+    rng = np.random.RandomState(int(orders.sum()) % (2 ** 31))
+    flat = np.concatenate([((rng.randn(k, k) + 1j * rng.randn(k, k)) / np.sqrt(2 * max(k, 1))).ravel()
+                           for k in orders]) if orders.size else np.zeros(1, complex)
+    d = torch.from_numpy(flat).cuda()
+    out = torch.empty(max(1, orders.size), dtype=torch.complex128, device="cuda")
+    trials = []
+    for c in CHUNKS:
+        plan = PermanentPlan(orders, max_chunk_log2=c)
+        try:
+            plan(d, out)
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                plan(d, out)
+            b.record()
+            torch.cuda.synchronize()
+            trials.append({"chunk_log2": c, "ms": a.elapsed_time(b) / reps})
+        finally:
+            plan.close()
+    best = min(trials, key=lambda t: t["ms"])
+    res = {"chunk_log2": best["chunk_log2"], "ms": best["ms"], "trials": trials}
+    cache[key] = res
+    if use_cache:
+        _save(cache)
+    return res
+
+
+SLOS_KERNELS = (1, 2, 3, 4, 5)
+
+
+def tune_slos(in_occ, reps: int = 3, use_cache: bool = True) -> dict:
+    """Fastest SLOS layer kernel for an input state (SlosPlan.set_kernel ids),
+    timed on a Haar unitary.  Returns {"kernel", "ms", "trials": [...]}."""
+    import torch
+    from .slos import PROBS, SlosPlan
+    from .utils.workloads import haar_unitary
+    in_occ = [int(v) for v in in_occ]
+    cache = _load() if use_cache else {}
+    key = f"{torch.cuda.get_device_name(0)}|slos|{in_occ}"
+    if key in cache:
+        return cache[key]
+    plan = SlosPlan(in_occ)
+    dU = torch.from_numpy(haar_unitary(plan.m, seed=1)).cuda()
+    out = torch.empty(max(1, plan.count), dtype=torch.float64, device="cuda")
+    trials = []
+    try:
+        for k in SLOS_KERNELS:
+            try:
+                plan.set_kernel(k)
+            except ValueError:
+                continue   # not applicable to this plan's layer sizes
+            plan.run(dU, PROBS, out)
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                plan.run(dU, PROBS, out)
+            b.record()
+            torch.cuda.synchronize()
+            trials.append({"kernel": k, "ms": a.elapsed_time(b) / reps})
+    finally:
+        plan.close()
+    best = min(trials, key=lambda t: t["ms"])
+    res = {"kernel": best["kernel"], "ms": best["ms"], "trials": trials}
+    cache[key] = res
+    if use_cache:
+        _save(cache)
+    return res
+
+
+def tuned_slos_plan(in_occ):
+    """A SlosPlan running the layer kernel the tuner found fastest."""
+    from .slos import SlosPlan
+    plan = SlosPlan(in_occ)
+    plan.set_kernel(tune_slos(in_occ)["kernel"])
+    return plan

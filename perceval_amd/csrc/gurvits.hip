@@ -12,11 +12,12 @@
 //
 // Sample s of matrix b takes its signs from the 64 bits rand64(seed,
 // stream0 + b, s) (bit j -> x_j = -1), so the oracle draws the same samples
-// and a batch can be split across calls.  One wavefront per matrix: A^T is
-// staged in LDS, zero-padded to NP x NP (rows i >= n keep y_i = 1), and lane l
-// takes samples l, l + 64, ...; per sample the column sums y = A x are built
-// column by column (2 FMAs per entry, column reads broadcast from LDS), then
-// multiplied as four independent chains.
+// and a batch can be split across calls.  A matrix's samples are split into
+// parts so that a small batch still fills the device; one wavefront per
+// (matrix, part) stages A in LDS (zero-padded to NP x NP), lane l takes
+// samples l, l + 64, ... of the part, forms y_i = sum_j x_j a_ij row by row
+// (2 FMAs per broadcast LDS read) and multiplies the y_i into four chains by
+// i mod 4; a finalize pass sums each matrix's parts in order.
 #include "common.h"
 #include "../../include/pcvl_hip.h"
 
@@ -47,39 +48,55 @@ __device__ __forceinline__ cd gurvits_prod(const cd *y)
     return cmul(cmul(p0, p1), cmul(p2, p3));
 }
 
+// One wavefront per (matrix, part): part q of a matrix takes samples
+// [q * per, min(iters, (q + 1) * per)), lane l of them l, l + 64, ...; it
+// writes (sum Re X, sum Im X, sum |X|^2) to its slot, and the finalize kernel
+// sums a matrix's parts in order (deterministic for a given split).
 template <int NP>
 __global__ void __launch_bounds__(64) gurvits_kernel(const cd *__restrict__ mats, const GurvitsItem *__restrict__ items,
-                                                     int64_t count, int64_t iters, uint64_t seed, uint64_t stream0,
-                                                     cd *__restrict__ est, double *__restrict__ err)
+                                                     int64_t count, int parts, int64_t iters, int64_t per,
+                                                     uint64_t seed, uint64_t stream0, double *__restrict__ partial)
 {
-    __shared__ cd sAt[NP * NP];
+    __shared__ cd sA[NP * NP];
     const int lane = threadIdx.x;
-    for (int64_t b = blockIdx.x; b < count; b += gridDim.x) {
+    const int64_t nwork = count * parts;
+    for (int64_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const int64_t b = wi / parts;
+        const int64_t q = wi % parts;
         const int n = __builtin_amdgcn_readfirstlane(items[b].n);
         const int64_t off = items[b].off;
         for (int e = lane; e < NP * NP; e += 64) {
-            const int j = e / NP, i = e % NP;
-            sAt[e] = (i < n && j < n) ? mats[off + (int64_t)i * n + j] : cd{0.0, 0.0};
+            const int i = e / NP, j = e % NP;
+            sA[e] = (i < n && j < n) ? mats[off + (int64_t)i * n + j] : cd{0.0, 0.0};
         }
         __syncthreads();
         const uint64_t mask = n >= 64 ? ~0ull : ((1ull << n) - 1);
+        const int64_t s_lo = q * per;
+        const int64_t s_hi = (s_lo + per < iters) ? s_lo + per : iters;
         cd sum = {0.0, 0.0};
         double sq = 0.0;
-        for (int64_t s = lane; s < iters; s += 64) {
+        for (int64_t s = s_lo + lane; s < s_hi; s += 64) {
             const uint64_t bits = rand64(seed, stream0 + (uint64_t)b, (uint64_t)s);
-            cd y[NP];
+            double x[NP];
 #pragma unroll
-            for (int i = 0; i < NP; i++) y[i] = {i < n ? 0.0 : 1.0, 0.0};
-            for (int j = 0; j < n; j++) {
-                const double x = ((bits >> j) & 1) ? -1.0 : 1.0;
-                const cd *col = sAt + j * NP;
+            for (int j = 0; j < NP; j++) x[j] = ((bits >> j) & 1) ? -1.0 : 1.0;
+            cd p0 = {1.0, 0.0}, p1 = {1.0, 0.0}, p2 = {1.0, 0.0}, p3 = {1.0, 0.0};
+            for (int i = 0; i < n; i++) {
+                const cd *row = sA + i * NP;
+                cd y = {0.0, 0.0};
 #pragma unroll
-                for (int i = 0; i < NP; i++) {
-                    y[i].r = fma(x, col[i].r, y[i].r);
-                    y[i].i = fma(x, col[i].i, y[i].i);
+                for (int j = 0; j < NP; j++) {   // zero padding beyond n adds nothing
+                    const cd a = row[j];
+                    y.r = fma(x[j], a.r, y.r);
+                    y.i = fma(x[j], a.i, y.i);
                 }
+                const int c = i & 3;
+                if (c == 0) p0 = cmul(p0, y);
+                else if (c == 1) p1 = cmul(p1, y);
+                else if (c == 2) p2 = cmul(p2, y);
+                else p3 = cmul(p3, y);
             }
-            cd X = gurvits_prod<NP>(y);
+            cd X = cmul(cmul(p0, p1), cmul(p2, p3));
             if (__builtin_popcountll(bits & mask) & 1) X = {-X.r, -X.i};
             sum.r += X.r;
             sum.i += X.i;
@@ -92,19 +109,36 @@ __global__ void __launch_bounds__(64) gurvits_kernel(const cd *__restrict__ mats
             sq += __shfl_xor(sq, o, 64);
         }
         if (lane == 0) {
-            const double inv = 1.0 / (double)iters;
-            const cd mean = {sum.r * inv, sum.i * inv};
-            const double var = sq * inv - (mean.r * mean.r + mean.i * mean.i);
-            est[b] = mean;
-            err[b] = 2.5758293035489004 * sqrt((var > 0.0 ? var : 0.0) * inv);
+            partial[3 * wi] = sum.r;
+            partial[3 * wi + 1] = sum.i;
+            partial[3 * wi + 2] = sq;
         }
         __syncthreads();
     }
 }
 
+__global__ void gurvits_finalize_kernel(const double *__restrict__ partial, int64_t count, int parts, int64_t iters,
+                                        cd *__restrict__ est, double *__restrict__ err)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= count) return;
+    double sr = 0.0, si = 0.0, sq = 0.0;
+    for (int q = 0; q < parts; q++) {
+        const double *pp = partial + 3 * (b * parts + q);
+        sr += pp[0];
+        si += pp[1];
+        sq += pp[2];
+    }
+    const double inv = 1.0 / (double)iters;
+    const cd mean = {sr * inv, si * inv};
+    const double var = sq * inv - (mean.r * mean.r + mean.i * mean.i);
+    est[b] = mean;
+    err[b] = 2.5758293035489004 * sqrt((var > 0.0 ? var : 0.0) * inv);
+}
+
 template <int NP>
-static int launch_gurvits(const cd *mats, const GurvitsItem *items, int64_t count, int64_t iters, uint64_t seed,
-                          uint64_t stream0, cd *est, double *err, hipStream_t st)
+static int launch_gurvits(const cd *mats, const GurvitsItem *items, int64_t count, int parts, int64_t iters,
+                          int64_t per, uint64_t seed, uint64_t stream0, double *partial, hipStream_t st)
 {
     int dev = 0, cus = 0, per_cu = 0;
     PCVL_HIP_CHECK(hipGetDevice(&dev));
@@ -112,9 +146,9 @@ static int launch_gurvits(const cd *mats, const GurvitsItem *items, int64_t coun
     PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&gurvits_kernel<NP>),
                                                                 64, 0));
     if (per_cu < 1) per_cu = 1;
-    const int grid = (int)std::min<int64_t>(count, (int64_t)cus * per_cu);
-    hipLaunchKernelGGL((gurvits_kernel<NP>), dim3(grid), dim3(64), 0, st, mats, items, count, iters, seed, stream0, est,
-                       err);
+    const int grid = (int)std::min<int64_t>(count * parts, (int64_t)cus * per_cu);
+    hipLaunchKernelGGL((gurvits_kernel<NP>), dim3(grid), dim3(64), 0, st, mats, items, count, parts, iters, per, seed,
+                       stream0, partial);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
 }
@@ -138,8 +172,18 @@ extern "C" int pcvl_gurvits_batch(const double *d_mats, const int32_t *h_n, cons
         items[(size_t)b] = GurvitsItem{h_offsets[b], h_n[b], 0};
         nmax = std::max(nmax, (int)h_n[b]);
     }
-    GurvitsItem *d_items = nullptr;
-    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&d_items), sizeof(GurvitsItem) * items.size(), stream));
+    // samples of a matrix are split into parts of at least 64 * 16 samples so
+    // that the batch fills the device (4 waves per SIMD of 256 CUs)
+    int parts = (int)std::max<int64_t>(1, std::min<int64_t>((4096 + count - 1) / count, iters / 1024));
+    const int64_t per = (iters + parts - 1) / parts;
+    parts = (int)((iters + per - 1) / per);
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t b_items = al(sizeof(GurvitsItem) * items.size());
+    const size_t b_part = al(sizeof(double) * 3 * (size_t)count * parts);
+    char *blk = nullptr;
+    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), b_items + b_part, stream));
+    GurvitsItem *d_items = reinterpret_cast<GurvitsItem *>(blk);
+    double *d_part = reinterpret_cast<double *>(blk + b_items);
     hipError_t e = hipMemcpyAsync(d_items, items.data(), sizeof(GurvitsItem) * items.size(), hipMemcpyHostToDevice,
                                   stream);
     int rc = OK;
@@ -148,17 +192,22 @@ extern "C" int pcvl_gurvits_batch(const double *d_mats, const int32_t *h_n, cons
         rc = ERR_HIP;
     } else {
         const cd *mats = reinterpret_cast<const cd *>(d_mats);
-        cd *est = reinterpret_cast<cd *>(d_est);
-        if (nmax <= 8) rc = launch_gurvits<8>(mats, d_items, count, iters, seed, stream0, est, d_err, stream);
-        else if (nmax <= 16) rc = launch_gurvits<16>(mats, d_items, count, iters, seed, stream0, est, d_err, stream);
-        else if (nmax <= 24) rc = launch_gurvits<24>(mats, d_items, count, iters, seed, stream0, est, d_err, stream);
-        else if (nmax <= 32) rc = launch_gurvits<32>(mats, d_items, count, iters, seed, stream0, est, d_err, stream);
-        else if (nmax <= 40) rc = launch_gurvits<40>(mats, d_items, count, iters, seed, stream0, est, d_err, stream);
-        else rc = launch_gurvits<48>(mats, d_items, count, iters, seed, stream0, est, d_err, stream);
+        if (nmax <= 8) rc = launch_gurvits<8>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
+        else if (nmax <= 16) rc = launch_gurvits<16>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
+        else if (nmax <= 24) rc = launch_gurvits<24>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
+        else if (nmax <= 32) rc = launch_gurvits<32>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
+        else if (nmax <= 40) rc = launch_gurvits<40>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
+        else rc = launch_gurvits<48>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
+        if (rc == OK) {
+            hipLaunchKernelGGL(gurvits_finalize_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
+                               d_part, count, parts, iters, reinterpret_cast<cd *>(d_est), d_err);
+            e = hipGetLastError();
+            if (e != hipSuccess) { set_error(std::string("gurvits finalize: ") + hipGetErrorString(e)); rc = ERR_HIP; }
+        }
     }
     // the table must outlive the kernel: wait for it, then release it
     hipError_t se = hipStreamSynchronize(stream);
-    hipError_t fe = hipFreeAsync(d_items, stream);
+    hipError_t fe = hipFreeAsync(blk, stream);
     if (rc == OK && se != hipSuccess) { set_error(std::string("gurvits: ") + hipGetErrorString(se)); rc = ERR_HIP; }
     if (rc == OK && fe != hipSuccess) { set_error(std::string("hipFree(gurvits items): ") + hipGetErrorString(fe)); rc = ERR_HIP; }
     return rc;

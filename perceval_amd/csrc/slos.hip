@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(SLOS_THREADS) slos_layer32_kernel(
 // mode), so the results agree with it to the last bit for non-zero sums.
 constexpr int SLOS4_THREADS = 256;
 
-template <int MODE>
+template <int MODE, int G>
 __global__ void __launch_bounds__(SLOS4_THREADS) slos_layer2p_kernel(
     const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
     int k, uint32_t nchild, const uint64_t *__restrict__ gT, const uint64_t *__restrict__ gE, int w,
@@ -253,17 +253,17 @@ __global__ void __launch_bounds__(SLOS4_THREADS) slos_layer2p_kernel(
             if (q + 1 < m) pe += sE[q * w + Rn];
             R = Rn;
         }
-        // Gather four parents at a time with unconditional loads (indices
-        // clamped to the layer), so the four are in flight together; an
+        // Gather G parents at a time with unconditional loads (indices
+        // clamped to the layer), so the G are in flight together; an
         // entry past the depth, or an index outside the parent layer (never
         // for a consistent rank), contributes an exact zero.
         cd acc = {0.0, 0.0};
-        for (int d0 = 0; d0 < depth; d0 += 4) {
-            cd pv[4];
-            int qv[4];
-            bool ok[4];
+        for (int d0 = 0; d0 < depth; d0 += G) {
+            cd pv[G];
+            int qv[G];
+            bool ok[G];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
+            for (int t = 0; t < G; t++) {
                 const int d = d0 + t;
                 const uint32_t w32 = d < depth ? slot[d * SLOS4_THREADS + tid] : 0u;
                 const uint32_t pidx = w32 >> 6;
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(SLOS4_THREADS) slos_layer2p_kernel(
                 qv[t] = (int)(w32 & 63u);
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
+            for (int t = 0; t < G; t++) {
                 const cd u0 = su[qv[t]];
                 const cd u = {ok[t] ? u0.r : 0.0, ok[t] ? u0.i : 0.0};
                 acc.r = fma(pv[t].r, u.r, acc.r);
@@ -757,7 +757,8 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     const bool small = all_layers_32bit(p);
     const bool use_run = (p.kernel == 2);
     const bool use32 = (p.kernel == 3);
-    const bool use2p = (p.kernel == 4) || (p.kernel == 0 && twophase_ok(p));
+    const bool use2p = (p.kernel == 4) || (p.kernel == 5) || (p.kernel == 0 && twophase_ok(p));
+    const bool g8 = (p.kernel == 5);
     if ((use32 && !small) || (use2p && !twophase_ok(p))) {
         set_error("SLOS kernel not applicable to this plan's layer sizes");
         return ERR_ARG;
@@ -786,14 +787,10 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         } else if (use2p) {
             const int grid = (int)std::min<uint64_t>((nc + SLOS4_THREADS - 1) / SLOS4_THREADS, (uint64_t)p.grid_cap);
             const size_t lds2p = twophase_lds(p.m, p.w, k);
-            if (probs)
-                hipLaunchKernelGGL(slos_layer2p_kernel<1>, dim3(grid), dim3(SLOS4_THREADS), lds2p, stream, prev,
-                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
-                                   p.d_E, p.w, p.inv_prod_in);
-            else
-                hipLaunchKernelGGL(slos_layer2p_kernel<0>, dim3(grid), dim3(SLOS4_THREADS), lds2p, stream, prev,
-                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
-                                   p.d_E, p.w, p.inv_prod_in);
+            auto kp = probs ? (g8 ? slos_layer2p_kernel<1, 8> : slos_layer2p_kernel<1, 4>)
+                            : (g8 ? slos_layer2p_kernel<0, 8> : slos_layer2p_kernel<0, 4>);
+            hipLaunchKernelGGL(kp, dim3(grid), dim3(SLOS4_THREADS), lds2p, stream, prev, (uint32_t)p.counts[k - 1], dst,
+                               U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T, p.d_E, p.w, p.inv_prod_in);
         } else if (use32) {
             const int grid = (int)std::min<uint64_t>((nc + SLOS_THREADS - 1) / SLOS_THREADS, (uint64_t)p.grid_cap);
             const size_t lds32 = sizeof(cd) * p.m + sizeof(double) * p.w + 2 * sizeof(uint32_t) * p.m * p.w;
@@ -823,10 +820,10 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
 extern "C" int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel)
 {
     PCVL_ARG_CHECK(plan != nullptr, "plan is null");
-    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 4,
-                   "kernel must be 0 (auto), 1 (per state), 2 (incremental), 3 (per state, 32-bit) "
-                   "or 4 (per state, two phases)");
-    if (kernel == 4) PCVL_ARG_CHECK(twophase_ok(plan->impl), "two-phase kernel needs parent layers below 2^26 states");
+    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 5,
+                   "kernel must be 0 (auto), 1 (per state), 2 (incremental), 3 (per state, 32-bit), "
+                   "4 (per state, two phases) or 5 (two phases, eight loads in flight)");
+    if (kernel >= 4) PCVL_ARG_CHECK(twophase_ok(plan->impl), "two-phase kernel needs parent layers below 2^26 states");
     if (kernel == 2) PCVL_ARG_CHECK(run_kernel_ok(plan->impl), "incremental kernel not applicable to this plan");
     if (kernel == 3) PCVL_ARG_CHECK(all_layers_32bit(plan->impl), "32-bit kernel needs layers below 2^32 states");
     plan->impl.kernel = kernel;

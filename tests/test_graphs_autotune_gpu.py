@@ -59,3 +59,34 @@ def test_autotune_picks_a_measured_config(gpu, tmp_path, monkeypatch):
     plan = at.tuned_plan([16] * 512)
     assert plan.info()["single_order"] == 16
     plan.close()
+
+
+def test_autotune_mixed_and_slos(gpu, tmp_path, monkeypatch):
+    import numpy as np
+    import torch
+    import oracle
+    import perceval_amd.autotune as at
+    from perceval_amd.permanent import pack
+    from perceval_amd.slos import PROBS
+    from perceval_amd.utils.workloads import haar_unitary
+    monkeypatch.setattr(at, "_CACHE_PATH", str(tmp_path / "tune.json"))
+    orders = np.array([9, 12, 14, 10, 13, 11] * 20, dtype=np.int32)
+    res = at.tune_permanent_mixed(orders, reps=2)
+    assert res["chunk_log2"] in at.CHUNKS and res["ms"] == min(t["ms"] for t in res["trials"])
+    rng = np.random.RandomState(3)
+    mats = [rng.randn(k, k) + 1j * rng.randn(k, k) for k in orders[:12]]
+    o, flat = pack(mats)
+    plan = at.tuned_plan(o)
+    got = plan(torch.from_numpy(flat).cuda()).cpu().numpy()
+    plan.close()
+    ref = oracle.perm_batch(mats, precise=True)
+    assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-10
+    inp = [1, 1, 1, 0, 0, 1, 0, 0, 0, 0]
+    t = at.tune_slos(inp, reps=2)
+    assert t["kernel"] in at.SLOS_KERNELS and t["ms"] == min(x["ms"] for x in t["trials"])
+    assert at.tune_slos(inp)["kernel"] == t["kernel"]          # cached
+    sp = at.tuned_slos_plan(inp)
+    U = haar_unitary(10, seed=8)
+    p = sp.run(U, PROBS).cpu().numpy()
+    sp.close()
+    assert np.max(np.abs(p - oracle.slos_probs(U, inp))) < 1e-12

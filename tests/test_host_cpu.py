@@ -144,3 +144,40 @@ def test_no_gpu_means_loud_failure():
     b.set_circuit(BS.H())
     with pytest.raises(_native.NativeUnavailable):
         b.set_input_state(BasicState([1, 1]))
+
+
+def test_entry_points_reject_bad_arguments_before_touching_the_device():
+    """Argument checks return -1 with a message and never reach HIP, so they run
+    without a GPU (the header's error convention)."""
+    import ctypes
+    from perceval_amd import _native
+    lib = _native.load(require_gpu=False)
+
+    def err(rc):
+        assert rc == -1, rc
+        return lib.pcvl_last_error().decode()
+
+    cond = (ctypes.c_int32 * 4)(-1, 0, 1, -1)
+    out = ctypes.c_void_p(1)
+    assert "masks" in err(lib.pcvl_fock_mask_select(4, 2, cond, 0, None, out, out, None))
+    assert "mode count" in err(lib.pcvl_fock_mask_select(65, 2, cond, 1, None, out, out, None))
+    assert "photon count" in err(lib.pcvl_fock_mask_select(4, 64, cond, 1, None, out, out, None))
+    n = (ctypes.c_int32 * 1)(49)
+    offs = (ctypes.c_int64 * 1)(0)
+    assert "order" in err(lib.pcvl_gurvits_batch(out, n, offs, 1, 100, 1, 0, out, out, None))
+    assert "iterations" in err(lib.pcvl_gurvits_batch(out, n, offs, 1, 0, 1, 0, out, out, None))
+    inputs = (ctypes.c_int32 * 3)(1, 1, 0)
+    counts = (ctypes.c_int64 * 1)(-1)
+    assert "negative" in err(lib.pcvl_cc_samples_multi(out, 3, inputs, counts, 1, 1, 0, out, None))
+    assert "mode count" in err(lib.pcvl_cc_samples_multi(out, 0, inputs, counts, 1, 1, 0, out, None))
+    plan = ctypes.c_void_p()
+    assert "no input" in err(lib.pcvl_slos_tree_create(3, inputs, 0, ctypes.byref(plan)))
+    bad = (ctypes.c_int32 * 3)(1, -1, 0)
+    assert "negative" in err(lib.pcvl_slos_tree_create(3, bad, 1, ctypes.byref(plan)))
+    assert "fused sampler" in err(lib.pcvl_mis_sample_fused(out, out, out, out, 10, 65, 3, 64, 1, 0, out, out, None))
+    assert "fused sampler" in err(lib.pcvl_mis_sample_fused(out, out, out, out, 10, 8, 3, 12, 1, 0, out, out, None))
+    f = ctypes.c_int64()
+    assert "negative" in err(lib.pcvl_mlp_wgrad_workspace(-1, 3, 3, ctypes.byref(f)))
+    assert "workspace" in err(lib.pcvl_mlp_wgrad(out, 3, out, 3, 100, 3, 3, out, out, out, 0, None))
+    assert "probe" in err(lib.pcvl_probe_fp64_fma(0, 10, out, ctypes.byref(f), None))
+    assert "1 <= n <= 28" in err(lib.pcvl_mis_permanents(out, 10, out, 29, out, 5, out, None))

@@ -44,7 +44,7 @@ def test_fock_rank_unrank_m24_n10_sampled(gpu):
     assert (bad == -1).all()
 
 
-KERNELS = (1, 2, 3, 4)   # per output state; incremental runs; 32-bit ranks; two-phase gather
+KERNELS = (1, 2, 3, 4, 5)   # per state; incremental runs; 32-bit ranks; two-phase gather (4 / 8 loads in flight)
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -104,7 +104,7 @@ def test_slos_kernels_agree(gpu, inp):
         assert np.max(np.abs(got[kernel] - ref)) < ABS_TOL, kernel
         assert np.max(np.abs(got[kernel] - got[1])) < 1e-14, kernel
     assert np.array_equal(got[1], got[3])   # same walk, same summation order
-    assert np.max(np.abs(got[1] - got[4])) == 0.0
+    assert np.max(np.abs(got[1] - got[4])) == 0.0 and np.max(np.abs(got[1] - got[5])) == 0.0
     plan.close()
 
 
@@ -112,7 +112,7 @@ def test_slos_set_kernel_rejects_bad_values(gpu):
     from perceval_amd.slos import SlosPlan
     plan = SlosPlan([1, 1, 0])
     with pytest.raises(ValueError):
-        plan.set_kernel(5)
+        plan.set_kernel(6)
     plan.close()
 
 
