@@ -141,7 +141,7 @@ def tune_permanent_mixed(orders, reps: int = 3, use_cache: bool = True) -> dict:
     return res
 
 
-SLOS_KERNELS = (1, 2, 3, 4, 5)
+SLOS_KERNELS = (1, 2, 3, 4, 5, 6, 7)
 
 
 def tune_slos(in_occ, reps: int = 3, use_cache: bool = True) -> dict:

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 namespace pcvl {
@@ -48,6 +49,20 @@ static uint64_t binom_u64(int64_t a, int64_t b)
 
 // Rank tables for (m, nmax): T[q*(nmax+1)+x] and E[q*(nmax+1)+x] for
 // q in [0, m), x in [0, nmax]; T/E of q = m-1 are zero (no term).
+// Packed walk table of the wave-prefix kernel: P[q][x] = T[q][x] | E[q][x] << 32
+// for x = 0..nmax+1 (one column more than T, for the block-size lookup).
+static void build_packed(int m, int nmax, std::vector<uint64_t> &P)
+{
+    const int wp = nmax + 2;
+    P.assign((size_t)m * wp, 0);
+    for (int q = 0; q + 1 < m; q++)
+        for (int x = 0; x <= nmax + 1; x++) {
+            const uint64_t t = binom_u64((int64_t)x + m - q - 2, m - q - 1);
+            const uint64_t e = binom_u64((int64_t)x + m - q - 3, m - q - 2);
+            P[(size_t)q * wp + x] = (t & 0xffffffffull) | (e << 32);
+        }
+}
+
 static void build_tables(int m, int nmax, std::vector<uint64_t> &T, std::vector<uint64_t> &E)
 {
     const int w = nmax + 1;
@@ -289,6 +304,331 @@ __global__ void __launch_bounds__(SLOS4_THREADS) slos_layer2p_kernel(
     }
 }
 
+// Successor kernel with a transposed gather.  Thread t of a block unranks
+// child blk0 + t*RUN once and steps through the RUN - 1 children after it in
+// O(1) each (in descending lexicographic order only the end of the
+// occupied-mode list changes: the last occupied mode i below m-1 gives one
+// photon to mode i+1, and the photons of mode m-1 join it there), writing
+// every child's parent slots to LDS.  The gather then gives consecutive
+// lanes consecutive children, so each load and store instruction covers a
+// contiguous run of parents and outputs.  A slot holds
+// (parent rank << 6) | mode, so the slots of the next child are the current
+// ones plus 64 except at the end of the list.  Slots, gather and summation
+// order are the two-phase kernel's, so the results are bit-identical to it.
+// Slot columns are padded by one word per 32 so that both the per-thread
+// writes (stride RUN) and the per-lane reads avoid LDS bank conflicts.
+// The walk reads the packed table P[q][x] = T[q][x] | E[q][x] << 32.
+// Parent layers carry one zero coefficient after their last state (index
+// nparent): gathers past a child's depth read it, so they need no masking.
+// Intermediate layers (sentinel != 0) write that zero for the next layer.
+// PFI: the product of factorials of a child's occupations is kept as an
+// exact 32-bit integer (k <= 12), else as a double.
+constexpr int SLOS7_THREADS = 256;
+
+__device__ __forceinline__ uint32_t succ_pad(uint32_t c) { return c + (c >> 5); }
+
+template <int MODE, int RUN, bool PFI, int G>
+__global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ_kernel(
+    const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
+    int k, uint32_t nchild, const uint64_t *__restrict__ gP, int wp, double inv_prod_in, int sentinel)
+{
+    constexpr uint32_t TILE = (uint32_t)RUN * SLOS7_THREADS;
+    constexpr uint32_t TPAD = TILE + TILE / 32;
+    using pf_t = typename std::conditional<PFI, uint32_t, double>::type;
+    extern __shared__ uint64_t s_dyn7[];
+    // layout: P (m*wp u64) | u (m cd) | fact (wp double) | pf (TILE pf_t, probabilities only) |
+    //         slots (k*TPAD u32) | cnt ((k+1)*THREADS u8) | dep (TILE u8)
+    uint64_t *sP = s_dyn7;
+    cd *su = reinterpret_cast<cd *>(sP + (size_t)m * wp);
+    double *sf = reinterpret_cast<double *>(su + m);
+    pf_t *spf = reinterpret_cast<pf_t *>(sf + wp);
+    uint32_t *slot = reinterpret_cast<uint32_t *>(spf + (MODE == 1 ? TILE : 0));
+    uint8_t *cnt = reinterpret_cast<uint8_t *>(slot + (size_t)k * TPAD);
+    uint8_t *dep = cnt + (size_t)(k + 1) * SLOS7_THREADS;
+    for (int e = threadIdx.x; e < m * wp; e += blockDim.x) sP[e] = gP[e];
+    for (int e = threadIdx.x; e < m; e += blockDim.x) su[e] = U[(size_t)e * m + mk];
+    if (threadIdx.x == 0) {
+        double f = 1.0;
+        for (int v = 0; v < wp; v++) {
+            if (v > 1) f *= (double)v;
+            sf[v] = f;
+        }
+        if (sentinel && blockIdx.x == 0) reinterpret_cast<cd *>(out)[nchild] = cd{0.0, 0.0};
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const uint32_t sent_word = nparent << 6;
+#define CNT(d) cnt[(size_t)(d) * SLOS7_THREADS + tid]
+    for (uint64_t blk = (uint64_t)blockIdx.x * TILE; blk < nchild; blk += (uint64_t)gridDim.x * TILE) {
+        const uint32_t blk0 = (uint32_t)blk;
+        // phase 1: this thread's RUN consecutive children
+        const uint32_t c0 = (uint32_t)tid * RUN;
+        const uint32_t first = blk0 + c0;
+        const uint32_t p0 = succ_pad(c0);
+        int depth = 0;
+        pf_t pfact = (pf_t)1;
+        if (first < nchild) {
+            uint32_t rem = first, pe = 0;
+            int R = k;
+            for (int q = 0; q < m && R > 0; q++) {
+                int Rn = 0;
+                uint64_t e = 0;
+                if (q + 1 < m) {
+                    const uint64_t *tq = sP + q * wp;
+                    Rn = R;
+                    e = tq[Rn];
+                    while (Rn > 0 && (uint32_t)e > rem) { --Rn; e = tq[Rn]; }
+                    rem -= (uint32_t)e;
+                }
+                const int sq = R - Rn;
+                if (sq > 0) {
+                    slot[depth * TPAD + p0] = ((first - pe) << 6) | (uint32_t)q;
+                    CNT(depth) = (uint8_t)sq;
+                    depth++;
+                    if (MODE == 1) pfact *= (pf_t)sf[sq];
+                }
+                pe += (uint32_t)(e >> 32);
+                R = Rn;
+            }
+        }
+        dep[c0] = first < nchild ? (uint8_t)depth : (uint8_t)0;
+        if (MODE == 1) spf[c0] = pfact;
+        for (int r = 1; r < RUN; r++) {
+            const uint32_t c = c0 + (uint32_t)r, idx = first + (uint32_t)r;
+            if (idx >= nchild) { dep[c] = 0; continue; }
+            // idx - 1 is not the last state, so after dropping a mode-(m-1)
+            // entry the list still has an entry below m-1.
+            const uint32_t pp = succ_pad(c - 1), pc = succ_pad(c);
+            for (int d = 0; d < depth; d++) slot[d * TPAD + pc] = slot[d * TPAD + pp] + 64u;
+            int top = depth - 1;
+            uint32_t ts = slot[top * TPAD + pc];
+            int tail = 0;
+            if ((int)(ts & 63u) == m - 1) {
+                tail = CNT(top);
+                top--;
+                ts = slot[top * TPAD + pc];
+            }
+            const int i = (int)(ts & 63u);
+            const int tc = CNT(top);
+            const uint32_t pe_top = idx - (ts >> 6);
+            if (tc == 1) {
+                depth = top;
+            } else {
+                CNT(top) = (uint8_t)(tc - 1);
+                depth = top + 1;
+            }
+            const uint32_t pe_new = pe_top + (uint32_t)(sP[i * wp + tail + 1] >> 32);
+            slot[depth * TPAD + pc] = ((idx - pe_new) << 6) | (uint32_t)(i + 1);
+            CNT(depth) = (uint8_t)(tail + 1);
+            depth++;
+            dep[c] = (uint8_t)depth;
+            if (MODE == 1) {
+                pf_t pf = (pf_t)1;
+                for (int d = 0; d < depth; d++) pf *= (pf_t)sf[CNT(d)];
+                spf[c] = pf;
+            }
+        }
+        __syncthreads();
+        // phase 2: lane-consecutive children
+        for (int r = 0; r < RUN; r++) {
+            const uint32_t c = (uint32_t)r * SLOS7_THREADS + (uint32_t)tid;
+            const uint32_t idx = blk0 + c;
+            if (idx >= nchild) break;
+            const int dd = dep[c];
+            const uint32_t pc = succ_pad(c);
+            cd acc = {0.0, 0.0};
+            for (int d0 = 0; d0 < dd; d0 += G) {
+                cd pv[G];
+                int qv[G];
+#pragma unroll
+                for (int t = 0; t < G; t++) {
+                    const int d = d0 + t;
+                    const uint32_t w32 = d < dd ? slot[d * TPAD + pc] : sent_word;
+                    pv[t] = parent[min(w32 >> 6, nparent)];
+                    qv[t] = (int)(w32 & 63u);
+                }
+#pragma unroll
+                for (int t = 0; t < G; t++) {
+                    const cd u = su[qv[t]];
+                    acc.r = fma(pv[t].r, u.r, acc.r);
+                    acc.r = fma(-pv[t].i, u.i, acc.r);
+                    acc.i = fma(pv[t].r, u.i, acc.i);
+                    acc.i = fma(pv[t].i, u.r, acc.i);
+                }
+            }
+            if (MODE == 0) {
+                reinterpret_cast<cd *>(out)[idx] = acc;
+            } else {
+                reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * (double)spf[c] * inv_prod_in;
+            }
+        }
+        __syncthreads();
+    }
+#undef CNT
+}
+
+// Successor kernel, compact form.  As slos_layer_succ_kernel, but a thread
+// stores the full slot list only for its first child (slot0); child s of
+// the run is the first child's first keep_s entries (parent ranks + s) plus
+// an own tail of at most s entries, since each successor step rewrites only
+// the end of the list.  That cuts the LDS per child to about a quarter, so
+// runs of 4-8 children fit with 4-5 blocks per CU.  Results are
+// bit-identical to the two-phase kernel.
+template <int MODE, int RUN, bool PFI, int G>
+__global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ2_kernel(
+    const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
+    int k, uint32_t nchild, const uint64_t *__restrict__ gP, int wp, double inv_prod_in, int sentinel)
+{
+    constexpr uint32_t TILE = (uint32_t)RUN * SLOS7_THREADS;
+    constexpr int OWN_ROWS = RUN * (RUN - 1) / 2;
+    using pf_t = typename std::conditional<PFI, uint32_t, double>::type;
+    extern __shared__ uint64_t s_dyn8[];
+    // layout: P (m*wp u64) | u (m cd) | fact (wp double) | pf (RUN*THREADS pf_t, probabilities only) |
+    //         slot0 (k*THREADS u32) | own (OWN_ROWS*THREADS u32) | cnt ((k+1)*THREADS u8) |
+    //         keep (RUN*THREADS u8) | dep (RUN*THREADS u8)
+    uint64_t *sP = s_dyn8;
+    cd *su = reinterpret_cast<cd *>(sP + (size_t)m * wp);
+    double *sf = reinterpret_cast<double *>(su + m);
+    pf_t *spf = reinterpret_cast<pf_t *>(sf + wp);
+    uint32_t *slot0 = reinterpret_cast<uint32_t *>(spf + (MODE == 1 ? TILE : 0));
+    uint32_t *own = slot0 + (size_t)k * SLOS7_THREADS;
+    uint8_t *cnt = reinterpret_cast<uint8_t *>(own + (size_t)OWN_ROWS * SLOS7_THREADS);
+    uint8_t *skeep = cnt + (size_t)(k + 1) * SLOS7_THREADS;
+    uint8_t *sdep = skeep + TILE;
+    for (int e = threadIdx.x; e < m * wp; e += blockDim.x) sP[e] = gP[e];
+    for (int e = threadIdx.x; e < m; e += blockDim.x) su[e] = U[(size_t)e * m + mk];
+    if (threadIdx.x == 0) {
+        double f = 1.0;
+        for (int v = 0; v < wp; v++) {
+            if (v > 1) f *= (double)v;
+            sf[v] = f;
+        }
+        if (sentinel && blockIdx.x == 0) reinterpret_cast<cd *>(out)[nchild] = cd{0.0, 0.0};
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const uint32_t sent_word = nparent << 6;
+#define CNT(d) cnt[(size_t)(d) * SLOS7_THREADS + tid]
+#define SLOT0(d, t) slot0[(size_t)(d) * SLOS7_THREADS + (t)]
+#define OWN(s, j, t) own[(size_t)((s) * ((s) - 1) / 2 + (j)) * SLOS7_THREADS + (t)]
+    for (uint64_t blk = (uint64_t)blockIdx.x * TILE; blk < nchild; blk += (uint64_t)gridDim.x * TILE) {
+        const uint32_t blk0 = (uint32_t)blk;
+        // phase 1: this thread's RUN consecutive children
+        const uint32_t first = blk0 + (uint32_t)tid * RUN;
+        int depth = 0;
+        pf_t pfact = (pf_t)1;
+        if (first < nchild) {
+            uint32_t rem = first, pe = 0;
+            int R = k;
+            for (int q = 0; q < m && R > 0; q++) {
+                int Rn = 0;
+                uint64_t e = 0;
+                if (q + 1 < m) {
+                    const uint64_t *tq = sP + q * wp;
+                    Rn = R;
+                    e = tq[Rn];
+                    while (Rn > 0 && (uint32_t)e > rem) { --Rn; e = tq[Rn]; }
+                    rem -= (uint32_t)e;
+                }
+                const int sq = R - Rn;
+                if (sq > 0) {
+                    SLOT0(depth, tid) = ((first - pe) << 6) | (uint32_t)q;
+                    CNT(depth) = (uint8_t)sq;
+                    depth++;
+                    if (MODE == 1) pfact *= (pf_t)sf[sq];
+                }
+                pe += (uint32_t)(e >> 32);
+                R = Rn;
+            }
+        }
+        int kp = depth;
+        skeep[tid] = (uint8_t)depth;
+        sdep[tid] = first < nchild ? (uint8_t)depth : (uint8_t)0;
+        if (MODE == 1) spf[tid] = pfact;
+#pragma unroll
+        for (int s = 1; s < RUN; s++) {
+            const uint32_t idx = first + (uint32_t)s;
+            const int cs = s * SLOS7_THREADS + tid;
+            if (idx >= nchild) { sdep[cs] = 0; skeep[cs] = 0; continue; }
+            // slot word of entry d of child s-1 (parent ranks of child s-1)
+            auto word = [&](int d) -> uint32_t {
+                return d < kp ? SLOT0(d, tid) + 64u * (uint32_t)(s - 1) : OWN(s - 1, d - kp, tid);
+            };
+            int top = depth - 1;
+            uint32_t ts = word(top) + 64u;
+            int tail = 0;
+            if ((int)(ts & 63u) == m - 1) {   // idx - 1 is not the last state: an entry below m-1 remains
+                tail = CNT(top);
+                top--;
+                ts = word(top) + 64u;
+            }
+            const int i = (int)(ts & 63u);
+            const int tc = CNT(top);
+            const uint32_t pe_top = idx - (ts >> 6);
+            int newlen = top;
+            if (tc > 1) {
+                CNT(top) = (uint8_t)(tc - 1);
+                newlen = top + 1;
+            }
+            const int nkp = min(kp, newlen);
+            for (int d = nkp; d < newlen; d++) OWN(s, d - nkp, tid) = word(d) + 64u;
+            const uint32_t pe_new = pe_top + (uint32_t)(sP[i * wp + tail + 1] >> 32);
+            OWN(s, newlen - nkp, tid) = ((idx - pe_new) << 6) | (uint32_t)(i + 1);
+            CNT(newlen) = (uint8_t)(tail + 1);
+            depth = newlen + 1;
+            kp = nkp;
+            skeep[cs] = (uint8_t)kp;
+            sdep[cs] = (uint8_t)depth;
+            if (MODE == 1) {
+                pf_t pf = (pf_t)1;
+                for (int d = 0; d < depth; d++) pf *= (pf_t)sf[CNT(d)];
+                spf[cs] = pf;
+            }
+        }
+        __syncthreads();
+        // phase 2: lane-consecutive children; child c is step c % RUN of thread c / RUN
+        for (int r = 0; r < RUN; r++) {
+            const uint32_t c = (uint32_t)r * SLOS7_THREADS + (uint32_t)tid;
+            const uint32_t idx = blk0 + c;
+            if (idx >= nchild) break;
+            const uint32_t o = c / RUN, s = c % RUN;
+            const int cs = (int)(s * SLOS7_THREADS + o);
+            const int dd = sdep[cs], kk = skeep[cs];
+            cd acc = {0.0, 0.0};
+            for (int d0 = 0; d0 < dd; d0 += G) {
+                cd pv[G];
+                int qv[G];
+#pragma unroll
+                for (int t = 0; t < G; t++) {
+                    const int d = d0 + t;
+                    uint32_t w32 = sent_word;
+                    if (d < kk) w32 = SLOT0(d, o) + 64u * s;
+                    else if (d < dd) w32 = own[(size_t)(s * (s - 1) / 2 + (uint32_t)(d - kk)) * SLOS7_THREADS + o];
+                    pv[t] = parent[min(w32 >> 6, nparent)];
+                    qv[t] = (int)(w32 & 63u);
+                }
+#pragma unroll
+                for (int t = 0; t < G; t++) {
+                    const cd u = su[qv[t]];
+                    acc.r = fma(pv[t].r, u.r, acc.r);
+                    acc.r = fma(-pv[t].i, u.i, acc.r);
+                    acc.i = fma(pv[t].r, u.i, acc.i);
+                    acc.i = fma(pv[t].i, u.r, acc.i);
+                }
+            }
+            if (MODE == 0) {
+                reinterpret_cast<cd *>(out)[idx] = acc;
+            } else {
+                reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * (double)spf[cs] * inv_prod_in;
+            }
+        }
+        __syncthreads();
+    }
+#undef OWN
+#undef SLOT0
+#undef CNT
+}
+
 // ------------------------------------------------------------------------
 // Incremental layer kernel: each thread walks a run of RUN consecutive
 // children.  Only the first child of a run is unranked; the next child in
@@ -427,6 +767,9 @@ __global__ void slos_vacuum_kernel(double *dst, int as_prob)
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         dst[0] = 1.0;
         if (!as_prob) dst[1] = 0.0;
+        // as_prob == 2: coefficient layer in the workspace, followed by the
+        // zero coefficient the successor kernel's gathers read past a depth
+        if (as_prob == 2) { dst[1] = 0.0; dst[2] = 0.0; dst[3] = 0.0; }
     }
 }
 
@@ -598,9 +941,11 @@ struct SlosPlanImpl {
     double inv_prod_in = 1.0;         // 1 / prod t!
     uint64_t max_inter = 1;           // largest intermediate layer (k < n)
     uint64_t *d_T = nullptr, *d_E = nullptr;
+    uint64_t *d_P = nullptr;          // packed table of the wave-prefix kernel (inside the d_T allocation)
     int grid_cap = 0;
     int kernel = 0;                   // 0 auto, 1 per state, 2 incremental runs, 3 per state 32-bit,
-                                      // 4 per state in two phases
+                                      // 4 per state in two phases, 5 the same with eight loads in flight,
+                                      // 6 / 7 successor runs with a transposed gather
 };
 
 static size_t run_kernel_lds(int m, int w, int k)
@@ -621,6 +966,35 @@ static bool twophase_ok(const SlosPlanImpl &p)
     for (int k = 0; k < p.n; k++)
         if (p.counts[k] >= (1ull << 26)) return false;
     return p.counts[p.n] < (1ull << 32) && twophase_lds(p.m, p.w, p.n) <= 64 * 1024;
+}
+
+static size_t succ2_lds(int m, int n, int k, int run, bool probs)
+{
+    const size_t rows = (size_t)k + (size_t)run * (run - 1) / 2;
+    const size_t pf = probs ? (k <= 12 ? sizeof(uint32_t) : sizeof(double)) * run * SLOS7_THREADS : 0;
+    return sizeof(uint64_t) * (size_t)m * (n + 2) + sizeof(cd) * m + sizeof(double) * (n + 2) + pf +
+           sizeof(uint32_t) * rows * SLOS7_THREADS + (size_t)(k + 1) * SLOS7_THREADS +
+           2 * (size_t)run * SLOS7_THREADS;
+}
+
+static size_t succ_lds(int m, int n, int k, int run, bool probs)
+{
+    const size_t tile = (size_t)run * SLOS7_THREADS, tpad = tile + tile / 32;
+    const size_t pf = probs ? (k <= 12 ? sizeof(uint32_t) : sizeof(double)) * tile : 0;
+    return sizeof(uint64_t) * (size_t)m * (n + 2) + sizeof(cd) * m + sizeof(double) * (n + 2) + pf +
+           sizeof(uint32_t) * (size_t)k * tpad + (size_t)(k + 1) * SLOS7_THREADS + tile;
+}
+
+// Successor kernels: 6 compact slot lists, runs of 4 children per thread;
+// 7 full slot lists, runs of 2 with eight loads in flight.
+static size_t succ_kernel_lds(const SlosPlanImpl &p, int kernel, int k, bool probs)
+{
+    return kernel == 6 ? succ2_lds(p.m, p.n, k, 4, probs) : succ_lds(p.m, p.n, k, 2, probs);
+}
+
+static bool succ_ok(const SlosPlanImpl &p, int kernel)
+{
+    return twophase_ok(p) && succ_kernel_lds(p, kernel, p.n, true) <= 64 * 1024;
 }
 
 static bool all_layers_32bit(const SlosPlanImpl &p)
@@ -691,14 +1065,17 @@ extern "C" int pcvl_slos_plan_create(int m, const int32_t *h_in_occ, pcvl_slos_p
         p.counts.push_back(c);
         if (k < n && c > p.max_inter) p.max_inter = c;
     }
-    std::vector<uint64_t> T, E;
+    std::vector<uint64_t> T, E, P;
     build_tables(m, n, T, E);
+    build_packed(m, n, P);
     size_t bytes = sizeof(uint64_t) * T.size();
-    hipError_t e = hipMalloc(&p.d_T, 2 * bytes);
+    hipError_t e = hipMalloc(&p.d_T, 2 * bytes + sizeof(uint64_t) * P.size());
     if (e != hipSuccess) { delete plan; set_error(std::string("hipMalloc(slos tables): ") + hipGetErrorString(e)); return ERR_ALLOC; }
     p.d_E = p.d_T + T.size();
+    p.d_P = p.d_E + E.size();
     e = hipMemcpy(p.d_T, T.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p.d_E, E.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p.d_P, P.data(), sizeof(uint64_t) * P.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(p.d_T);
         delete plan;
@@ -719,7 +1096,8 @@ extern "C" int pcvl_slos_plan_info(pcvl_slos_plan plan, int *n, int64_t *out_cou
     if (n) *n = p.n;
     if (out_count) *out_count = (int64_t)p.counts[p.n];
     // two ping-pong buffers of the largest intermediate layer (complex)
-    if (work_bytes) *work_bytes = p.n >= 2 ? 2 * p.max_inter * sizeof(cd) : sizeof(cd) * 2;
+    // two layer buffers, each with one zero coefficient after its states
+    if (work_bytes) *work_bytes = p.n >= 2 ? 2 * (p.max_inter + 1) * sizeof(cd) : sizeof(cd) * 2;
     if (h_path) for (int k = 0; k < p.n; k++) h_path[k] = p.mk[k];
     return OK;
 }
@@ -736,7 +1114,7 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     PCVL_ARG_CHECK(d_work != nullptr && work_bytes >= need, "workspace missing or smaller than pcvl_slos_plan_info reports");
     const cd *U = reinterpret_cast<const cd *>(d_U);
     cd *bufA = reinterpret_cast<cd *>(d_work);
-    cd *bufB = bufA + p.max_inter;
+    cd *bufB = bufA + p.max_inter + 1;
     const size_t lds = 2 * sizeof(uint64_t) * (size_t)p.m * p.w;
 
     if (p.n == 0) {
@@ -746,20 +1124,23 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         return OK;
     }
     // Layer 0 is the vacuum coefficient 1; layer 1 reads it from bufA[0].
-    hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(bufA), 0);
+    hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(bufA), 2);
     PCVL_HIP_CHECK(hipGetLastError());
     const cd *prev = bufA;
-    // Auto takes the two-phase per-state kernel where its packed parent
-    // ranks fit (parent layers below 2^26 states), else the per-state kernel.
-    // The incremental kernel is slower on MI355X (7.8 ms against 3.4 ms at
-    // m=24, n=10): its lanes gather parents at a stride of RUN elements, and
-    // the resulting L2 misses read 41.5 GB for the last layer.
+    // Auto takes the compact successor kernel where its packed parent ranks
+    // and LDS fit (parent layers below 2^26 states), else the two-phase
+    // kernel where its ranks fit, else the per-state kernel.  At m=24, n=10 on MI355X: successor 2.0 ms, two-phase 3.1 ms,
+    // per state 3.4 ms, incremental 7.8 ms (its lanes gather parents at a
+    // stride of RUN elements, and the resulting L2 misses read 41.5 GB for
+    // the last layer).
     const bool small = all_layers_32bit(p);
-    const bool use_run = (p.kernel == 2);
-    const bool use32 = (p.kernel == 3);
-    const bool use2p = (p.kernel == 4) || (p.kernel == 5) || (p.kernel == 0 && twophase_ok(p));
-    const bool g8 = (p.kernel == 5);
-    if ((use32 && !small) || (use2p && !twophase_ok(p))) {
+    const int kernel = p.kernel != 0 ? p.kernel : succ_ok(p, 6) ? 6 : twophase_ok(p) ? 4 : 1;
+    const bool use_run = (kernel == 2);
+    const bool use32 = (kernel == 3);
+    const bool usesucc = (kernel == 6 || kernel == 7);
+    const bool use2p = (kernel == 4) || (kernel == 5);
+    const bool g8 = (kernel == 5);
+    if ((use32 && !small) || (use2p && !twophase_ok(p)) || (usesucc && !succ_ok(p, kernel))) {
         set_error("SLOS kernel not applicable to this plan's layer sizes");
         return ERR_ARG;
     }
@@ -772,7 +1153,20 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         const bool last = (k == p.n);
         void *dst = last ? static_cast<void *>(d_out) : static_cast<void *>(prev == bufA ? bufB : bufA);
         const bool probs = last && mode == PCVL_SLOS_PROBS;
-        if (use_run) {
+        if (usesucc) {
+            const int run = kernel == 6 ? 4 : 2;
+            const uint64_t tile = (uint64_t)run * SLOS7_THREADS;
+            const int grid = (int)std::min<uint64_t>((nc + tile - 1) / tile, (uint64_t)p.grid_cap);
+            const bool pfi = k <= 12;
+            auto kp = kernel == 6
+                          ? (probs ? (pfi ? slos_layer_succ2_kernel<1, 4, true, 4> : slos_layer_succ2_kernel<1, 4, false, 4>)
+                                   : slos_layer_succ2_kernel<0, 4, true, 4>)
+                          : (probs ? (pfi ? slos_layer_succ_kernel<1, 2, true, 8> : slos_layer_succ_kernel<1, 2, false, 8>)
+                                   : slos_layer_succ_kernel<0, 2, true, 8>);
+            hipLaunchKernelGGL(kp, dim3(grid), dim3(SLOS7_THREADS), succ_kernel_lds(p, kernel, k, probs), stream, prev,
+                               (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_P, p.n + 2,
+                               p.inv_prod_in, last ? 0 : 1);
+        } else if (use_run) {
             const uint64_t per_block = (uint64_t)SLOS2_RUN * SLOS2_THREADS;
             const int grid = (int)std::min<uint64_t>((nc + per_block - 1) / per_block, (uint64_t)p.grid_cap * 2);
             const size_t lds2 = run_kernel_lds(p.m, p.w, k);
@@ -820,10 +1214,12 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
 extern "C" int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel)
 {
     PCVL_ARG_CHECK(plan != nullptr, "plan is null");
-    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 5,
+    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 7,
                    "kernel must be 0 (auto), 1 (per state), 2 (incremental), 3 (per state, 32-bit), "
-                   "4 (per state, two phases) or 5 (two phases, eight loads in flight)");
+                   "4 (per state, two phases), 5 (two phases, eight loads in flight), 6 (successor runs of 4, "
+                   "compact slots) or 7 (successor runs of 2, full slots)");
     if (kernel >= 4) PCVL_ARG_CHECK(twophase_ok(plan->impl), "two-phase kernel needs parent layers below 2^26 states");
+    if (kernel >= 6) PCVL_ARG_CHECK(succ_ok(plan->impl, kernel), "successor kernel needs its LDS below 64 KiB");
     if (kernel == 2) PCVL_ARG_CHECK(run_kernel_ok(plan->impl), "incremental kernel not applicable to this plan");
     if (kernel == 3) PCVL_ARG_CHECK(all_layers_32bit(plan->impl), "32-bit kernel needs layers below 2^32 states");
     plan->impl.kernel = kernel;

@@ -42,7 +42,11 @@ class SlosPlan:
     def set_kernel(self, kernel: int):
         """0 = pick, 1 = one thread per output state, 2 = incremental runs,
         3 = one thread per output state with 32-bit ranks, 4 = one thread per
-        output state, walk and gather in two phases."""
+        output state, walk and gather in two phases, 5 = the same with eight
+        loads in flight, 6 = runs of 4 consecutive states per thread stepped
+        by successor, gathered lane-consecutively (compact slot lists; the
+        default where it fits), 7 = the same with runs of 2 and full slot
+        lists."""
         _native.check(self._lib.pcvl_slos_plan_set_kernel(self._plan, int(kernel)), "pcvl_slos_plan_set_kernel")
 
     def _workspace(self, device):

@@ -44,7 +44,9 @@ def test_fock_rank_unrank_m24_n10_sampled(gpu):
     assert (bad == -1).all()
 
 
-KERNELS = (1, 2, 3, 4, 5)   # per state; incremental runs; 32-bit ranks; two-phase gather (4 / 8 loads in flight)
+KERNELS = (1, 2, 3, 4, 5, 6, 7)   # per state; incremental runs; 32-bit ranks; two-phase gather (4 / 8 loads
+#                                   in flight); successor runs of 4 (compact slots) / 2 (full slots) with a
+#                                   transposed gather
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -105,6 +107,26 @@ def test_slos_kernels_agree(gpu, inp):
         assert np.max(np.abs(got[kernel] - got[1])) < 1e-14, kernel
     assert np.array_equal(got[1], got[3])   # same walk, same summation order
     assert np.max(np.abs(got[1] - got[4])) == 0.0 and np.max(np.abs(got[1] - got[5])) == 0.0
+    for kernel in (6, 7):
+        assert np.array_equal(got[4], got[kernel]), kernel   # same slots and summation order
+    plan.close()
+
+
+@pytest.mark.parametrize("inp", [[1] * 8 + [0] * 8, [2, 0, 3, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0]])
+def test_slos_successor_kernels_bit_identical(gpu, inp):
+    """The successor kernels (6, 7) against the two-phase kernel, on every
+    coefficient and probability of a 490,314-state layer (a last tile of 10
+    states, successor runs cut by the end of the layer)."""
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan
+    U = haar_unitary(16, seed=7)
+    plan = SlosPlan(inp)
+    assert plan.count == 490314 and plan.count % 64 == 10
+    out = {}
+    for kernel in (4, 6, 7):
+        plan.set_kernel(kernel)
+        out[kernel] = (plan.run(U, COEFS).cpu().numpy(), plan.run(U, PROBS).cpu().numpy())
+    for kernel in (6, 7):
+        assert np.array_equal(out[4][0], out[kernel][0]) and np.array_equal(out[4][1], out[kernel][1]), kernel
     plan.close()
 
 
@@ -112,7 +134,7 @@ def test_slos_set_kernel_rejects_bad_values(gpu):
     from perceval_amd.slos import SlosPlan
     plan = SlosPlan([1, 1, 0])
     with pytest.raises(ValueError):
-        plan.set_kernel(6)
+        plan.set_kernel(8)
     plan.close()
 
 
