@@ -283,7 +283,7 @@ def bench_slos(args, res):
     per_kernel = {}
     # 1 per state, 2 incremental runs, 3 per state with 32-bit ranks, 4 / 5 per
     # state in two phases (4 / 8 parent loads in flight), 6 / 7 successor runs
-    # of 4 / 2 children per thread with a transposed gather (compact / full slot
+    # of 3 / 2 children per thread with a transposed gather (compact / full slot
     # lists), 0 the library's own pick (the headline number)
     for kernel in (1, 2, 3, 4, 5, 6, 7, 0):
         plan.set_kernel(kernel)
@@ -302,7 +302,7 @@ def bench_slos(args, res):
                                                  "per_state_32bit": per_kernel[3],
                                                  "two_phase": per_kernel[4],
                                                  "two_phase_8_loads": per_kernel[5],
-                                                 "successor_run4_compact": per_kernel[6],
+                                                 "successor_run3_compact": per_kernel[6],
                                                  "successor_run2": per_kernel[7]}}
     log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}; "
         f"by kernel " + ", ".join(f"{k}: {v:.3f}" for k, v in per_kernel.items()))

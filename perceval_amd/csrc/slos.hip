@@ -49,18 +49,41 @@ static uint64_t binom_u64(int64_t a, int64_t b)
 
 // Rank tables for (m, nmax): T[q*(nmax+1)+x] and E[q*(nmax+1)+x] for
 // q in [0, m), x in [0, nmax]; T/E of q = m-1 are zero (no term).
-// Packed walk table of the wave-prefix kernel: P[q][x] = T[q][x] | E[q][x] << 32
-// for x = 0..nmax+1 (one column more than T, for the block-size lookup).
+// Walk tables of the successor kernels, one array of 64-bit words: the packed
+// table P[q][x] = T[q][x] | E[q][x] << 32 for x = 0..nmax+1, then the 32-bit
+// cumulative columns CT[R][q] = sum_{j<q} T[j][R] and CE[R][q] (the same
+// for E) for R = 0..nmax, q = 0..m+3, with CT = 0xffffffff for q >= m.
+// Values above 2^32 are truncated; the kernels run only where every layer
+// has fewer than 2^32 states, which bounds all of them.
 static void build_packed(int m, int nmax, std::vector<uint64_t> &P)
 {
-    const int wp = nmax + 2;
-    P.assign((size_t)m * wp, 0);
+    const int wp = nmax + 2, cw = m + 4;
+    const size_t nc = (size_t)(nmax + 1) * cw;
+    P.assign((size_t)m * wp + nc, 0);   // nc u64 words hold the 2 * nc u32 of CT and CE
     for (int q = 0; q + 1 < m; q++)
         for (int x = 0; x <= nmax + 1; x++) {
             const uint64_t t = binom_u64((int64_t)x + m - q - 2, m - q - 1);
             const uint64_t e = binom_u64((int64_t)x + m - q - 3, m - q - 2);
             P[(size_t)q * wp + x] = (t & 0xffffffffull) | (e << 32);
         }
+    uint32_t *ct = reinterpret_cast<uint32_t *>(P.data() + (size_t)m * wp);
+    uint32_t *ce = ct + nc;
+    for (int R = 0; R <= nmax; R++) {
+        uint64_t at = 0, ae = 0;
+        for (int q = 0; q < cw; q++) {
+            if (q < m) {
+                ct[(size_t)R * cw + q] = (uint32_t)at;
+                ce[(size_t)R * cw + q] = (uint32_t)ae;
+                if (q + 1 < m) {
+                    at += binom_u64((int64_t)R + m - q - 2, m - q - 1);
+                    ae += binom_u64((int64_t)R + m - q - 3, m - q - 2);
+                }
+            } else {
+                ct[(size_t)R * cw + q] = 0xffffffffu;
+                ce[(size_t)R * cw + q] = 0;
+            }
+        }
+    }
 }
 
 static void build_tables(int m, int nmax, std::vector<uint64_t> &T, std::vector<uint64_t> &E)
@@ -327,6 +350,51 @@ constexpr int SLOS7_THREADS = 256;
 
 __device__ __forceinline__ uint32_t succ_pad(uint32_t c) { return c + (c >> 5); }
 
+// Unranks `first` in the k-photon layer and calls emit(q, s_q, pe) for each
+// occupied mode q in increasing order, pe being the rank offset of modes
+// below q (parent(c - e_q) = first - pe).  Empty modes are skipped four at
+// a time: with CT[R][q] = sum_{j<q} T[j][R] (CE the same for E; entries
+// q >= m hold 0xffffffff), the walk at mode q with R photons left and rank
+// rem in its block next stops at the first q' >= q with
+// CT[R][q'+1] > rem + CT[R][q], and CT is increasing in q, so one compare
+// per table entry counts the empty modes.  Equivalent to the mode-by-mode
+// walk T[q][Rn] <= rem < T[q][Rn+1].
+template <typename Emit>
+__device__ __forceinline__ void skip_walk(uint32_t first, int m, int k, const uint64_t *__restrict__ sP, int wp,
+                                          const uint32_t *__restrict__ sCT, const uint32_t *__restrict__ sCE, Emit emit)
+{
+    const int cw = m + 4;
+    uint32_t rem = first, pe = 0;
+    int R = k, q = 0;
+    while (R > 0) {
+        const uint32_t *ct = sCT + R * cw;
+        const uint32_t X = rem + ct[q];
+        int qn = q;
+        for (;;) {
+            const int c = (int)(ct[qn + 1] <= X) + (int)(ct[qn + 2] <= X) + (int)(ct[qn + 3] <= X) +
+                          (int)(ct[qn + 4] <= X);
+            qn += c;
+            if (c < 4) break;
+        }
+        rem = X - ct[qn];
+        pe += sCE[R * cw + qn] - sCE[R * cw + q];
+        q = qn;
+        int Rn = 0;
+        uint64_t e = 0;
+        if (q + 1 < m) {   // mode q holds at least one photon
+            const uint64_t *tq = sP + q * wp;
+            Rn = R - 1;
+            e = tq[Rn];
+            while (Rn > 0 && (uint32_t)e > rem) { --Rn; e = tq[Rn]; }
+            rem -= (uint32_t)e;
+        }
+        emit(q, R - Rn, pe);
+        pe += (uint32_t)(e >> 32);
+        R = Rn;
+        q++;
+    }
+}
+
 template <int MODE, int RUN, bool PFI, int G>
 __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ_kernel(
     const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
@@ -339,13 +407,19 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ_kernel(
     // layout: P (m*wp u64) | u (m cd) | fact (wp double) | pf (TILE pf_t, probabilities only) |
     //         slots (k*TPAD u32) | cnt ((k+1)*THREADS u8) | dep (TILE u8)
     uint64_t *sP = s_dyn7;
-    cd *su = reinterpret_cast<cd *>(sP + (size_t)m * wp);
+    uint32_t *sCT = reinterpret_cast<uint32_t *>(sP + (size_t)m * wp);
+    uint32_t *sCE = sCT + (size_t)(wp - 1) * (m + 4);
+    cd *su = reinterpret_cast<cd *>(sCE + (size_t)(wp - 1) * (m + 4));
     double *sf = reinterpret_cast<double *>(su + m);
     pf_t *spf = reinterpret_cast<pf_t *>(sf + wp);
     uint32_t *slot = reinterpret_cast<uint32_t *>(spf + (MODE == 1 ? TILE : 0));
     uint8_t *cnt = reinterpret_cast<uint8_t *>(slot + (size_t)k * TPAD);
     uint8_t *dep = cnt + (size_t)(k + 1) * SLOS7_THREADS;
     for (int e = threadIdx.x; e < m * wp; e += blockDim.x) sP[e] = gP[e];
+    {
+        const uint32_t *gC = reinterpret_cast<const uint32_t *>(gP + (size_t)m * wp);
+        for (int e = threadIdx.x; e < 2 * (wp - 1) * (m + 4); e += blockDim.x) sCT[e] = gC[e];
+    }
     for (int e = threadIdx.x; e < m; e += blockDim.x) su[e] = U[(size_t)e * m + mk];
     if (threadIdx.x == 0) {
         double f = 1.0;
@@ -368,28 +442,12 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ_kernel(
         int depth = 0;
         pf_t pfact = (pf_t)1;
         if (first < nchild) {
-            uint32_t rem = first, pe = 0;
-            int R = k;
-            for (int q = 0; q < m && R > 0; q++) {
-                int Rn = 0;
-                uint64_t e = 0;
-                if (q + 1 < m) {
-                    const uint64_t *tq = sP + q * wp;
-                    Rn = R;
-                    e = tq[Rn];
-                    while (Rn > 0 && (uint32_t)e > rem) { --Rn; e = tq[Rn]; }
-                    rem -= (uint32_t)e;
-                }
-                const int sq = R - Rn;
-                if (sq > 0) {
-                    slot[depth * TPAD + p0] = ((first - pe) << 6) | (uint32_t)q;
-                    CNT(depth) = (uint8_t)sq;
-                    depth++;
-                    if (MODE == 1) pfact *= (pf_t)sf[sq];
-                }
-                pe += (uint32_t)(e >> 32);
-                R = Rn;
-            }
+            skip_walk(first, m, k, sP, wp, sCT, sCE, [&](int q, int sq, uint32_t pe) {
+                slot[depth * TPAD + p0] = ((first - pe) << 6) | (uint32_t)q;
+                CNT(depth) = (uint8_t)sq;
+                depth++;
+                if (MODE == 1) pfact *= (pf_t)sf[sq];
+            });
         }
         dep[c0] = first < nchild ? (uint8_t)depth : (uint8_t)0;
         if (MODE == 1) spf[c0] = pfact;
@@ -471,8 +529,8 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ_kernel(
 // stores the full slot list only for its first child (slot0); child s of
 // the run is the first child's first keep_s entries (parent ranks + s) plus
 // an own tail of at most s entries, since each successor step rewrites only
-// the end of the list.  That cuts the LDS per child to about a quarter, so
-// runs of 4-8 children fit with 4-5 blocks per CU.  Results are
+// the end of the list.  That cuts the LDS per child to a half or less, so
+// runs of several children fit with 5-6 blocks per CU.  Results are
 // bit-identical to the two-phase kernel.
 template <int MODE, int RUN, bool PFI, int G>
 __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ2_kernel(
@@ -487,7 +545,9 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ2_kernel(
     //         slot0 (k*THREADS u32) | own (OWN_ROWS*THREADS u32) | cnt ((k+1)*THREADS u8) |
     //         keep (RUN*THREADS u8) | dep (RUN*THREADS u8)
     uint64_t *sP = s_dyn8;
-    cd *su = reinterpret_cast<cd *>(sP + (size_t)m * wp);
+    uint32_t *sCT = reinterpret_cast<uint32_t *>(sP + (size_t)m * wp);
+    uint32_t *sCE = sCT + (size_t)(wp - 1) * (m + 4);
+    cd *su = reinterpret_cast<cd *>(sCE + (size_t)(wp - 1) * (m + 4));
     double *sf = reinterpret_cast<double *>(su + m);
     pf_t *spf = reinterpret_cast<pf_t *>(sf + wp);
     uint32_t *slot0 = reinterpret_cast<uint32_t *>(spf + (MODE == 1 ? TILE : 0));
@@ -496,6 +556,10 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ2_kernel(
     uint8_t *skeep = cnt + (size_t)(k + 1) * SLOS7_THREADS;
     uint8_t *sdep = skeep + TILE;
     for (int e = threadIdx.x; e < m * wp; e += blockDim.x) sP[e] = gP[e];
+    {
+        const uint32_t *gC = reinterpret_cast<const uint32_t *>(gP + (size_t)m * wp);
+        for (int e = threadIdx.x; e < 2 * (wp - 1) * (m + 4); e += blockDim.x) sCT[e] = gC[e];
+    }
     for (int e = threadIdx.x; e < m; e += blockDim.x) su[e] = U[(size_t)e * m + mk];
     if (threadIdx.x == 0) {
         double f = 1.0;
@@ -518,28 +582,12 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ2_kernel(
         int depth = 0;
         pf_t pfact = (pf_t)1;
         if (first < nchild) {
-            uint32_t rem = first, pe = 0;
-            int R = k;
-            for (int q = 0; q < m && R > 0; q++) {
-                int Rn = 0;
-                uint64_t e = 0;
-                if (q + 1 < m) {
-                    const uint64_t *tq = sP + q * wp;
-                    Rn = R;
-                    e = tq[Rn];
-                    while (Rn > 0 && (uint32_t)e > rem) { --Rn; e = tq[Rn]; }
-                    rem -= (uint32_t)e;
-                }
-                const int sq = R - Rn;
-                if (sq > 0) {
-                    SLOT0(depth, tid) = ((first - pe) << 6) | (uint32_t)q;
-                    CNT(depth) = (uint8_t)sq;
-                    depth++;
-                    if (MODE == 1) pfact *= (pf_t)sf[sq];
-                }
-                pe += (uint32_t)(e >> 32);
-                R = Rn;
-            }
+            skip_walk(first, m, k, sP, wp, sCT, sCE, [&](int q, int sq, uint32_t pe) {
+                SLOT0(depth, tid) = ((first - pe) << 6) | (uint32_t)q;
+                CNT(depth) = (uint8_t)sq;
+                depth++;
+                if (MODE == 1) pfact *= (pf_t)sf[sq];
+            });
         }
         int kp = depth;
         skeep[tid] = (uint8_t)depth;
@@ -972,24 +1020,27 @@ static size_t succ2_lds(int m, int n, int k, int run, bool probs)
 {
     const size_t rows = (size_t)k + (size_t)run * (run - 1) / 2;
     const size_t pf = probs ? (k <= 12 ? sizeof(uint32_t) : sizeof(double)) * run * SLOS7_THREADS : 0;
-    return sizeof(uint64_t) * (size_t)m * (n + 2) + sizeof(cd) * m + sizeof(double) * (n + 2) + pf +
-           sizeof(uint32_t) * rows * SLOS7_THREADS + (size_t)(k + 1) * SLOS7_THREADS +
-           2 * (size_t)run * SLOS7_THREADS;
+    return sizeof(uint64_t) * (size_t)m * (n + 2) + 2 * sizeof(uint32_t) * (size_t)(n + 1) * (m + 4) +
+           sizeof(cd) * m + sizeof(double) * (n + 2) + pf + sizeof(uint32_t) * rows * SLOS7_THREADS +
+           (size_t)(k + 1) * SLOS7_THREADS + 2 * (size_t)run * SLOS7_THREADS;
 }
 
 static size_t succ_lds(int m, int n, int k, int run, bool probs)
 {
     const size_t tile = (size_t)run * SLOS7_THREADS, tpad = tile + tile / 32;
     const size_t pf = probs ? (k <= 12 ? sizeof(uint32_t) : sizeof(double)) * tile : 0;
-    return sizeof(uint64_t) * (size_t)m * (n + 2) + sizeof(cd) * m + sizeof(double) * (n + 2) + pf +
-           sizeof(uint32_t) * (size_t)k * tpad + (size_t)(k + 1) * SLOS7_THREADS + tile;
+    return sizeof(uint64_t) * (size_t)m * (n + 2) + 2 * sizeof(uint32_t) * (size_t)(n + 1) * (m + 4) +
+           sizeof(cd) * m + sizeof(double) * (n + 2) + pf + sizeof(uint32_t) * (size_t)k * tpad +
+           (size_t)(k + 1) * SLOS7_THREADS + tile;
 }
 
-// Successor kernels: 6 compact slot lists, runs of 4 children per thread;
-// 7 full slot lists, runs of 2 with eight loads in flight.
+// Successor kernels: 6 compact slot lists, runs of 3 children per thread;
+// 7 full slot lists, runs of 2 with eight loads in flight.  At m=24, n=10 runs
+// of 2 / 3 / 4 / 5 (compact) take 1.80 / 1.71 / 1.86 / 1.98 ms: longer runs
+// save unranking work but need more LDS per block, so fewer waves.
 static size_t succ_kernel_lds(const SlosPlanImpl &p, int kernel, int k, bool probs)
 {
-    return kernel == 6 ? succ2_lds(p.m, p.n, k, 4, probs) : succ_lds(p.m, p.n, k, 2, probs);
+    return kernel == 6 ? succ2_lds(p.m, p.n, k, 3, probs) : succ_lds(p.m, p.n, k, 2, probs);
 }
 
 static bool succ_ok(const SlosPlanImpl &p, int kernel)
@@ -1129,10 +1180,10 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     const cd *prev = bufA;
     // Auto takes the compact successor kernel where its packed parent ranks
     // and LDS fit (parent layers below 2^26 states), else the two-phase
-    // kernel where its ranks fit, else the per-state kernel.  At m=24, n=10 on MI355X: successor 2.0 ms, two-phase 3.1 ms,
-    // per state 3.4 ms, incremental 7.8 ms (its lanes gather parents at a
-    // stride of RUN elements, and the resulting L2 misses read 41.5 GB for
-    // the last layer).
+    // kernel where its ranks fit, else the per-state kernel.  At m=24, n=10
+    // on MI355X: successor 1.7 ms, two-phase 3.1 ms, per state 3.4 ms,
+    // incremental 7.8 ms (its lanes gather parents at a stride of RUN
+    // elements, and the resulting L2 misses read 41.5 GB for the last layer).
     const bool small = all_layers_32bit(p);
     const int kernel = p.kernel != 0 ? p.kernel : succ_ok(p, 6) ? 6 : twophase_ok(p) ? 4 : 1;
     const bool use_run = (kernel == 2);
@@ -1154,13 +1205,13 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         void *dst = last ? static_cast<void *>(d_out) : static_cast<void *>(prev == bufA ? bufB : bufA);
         const bool probs = last && mode == PCVL_SLOS_PROBS;
         if (usesucc) {
-            const int run = kernel == 6 ? 4 : 2;
+            const int run = kernel == 6 ? 3 : 2;
             const uint64_t tile = (uint64_t)run * SLOS7_THREADS;
             const int grid = (int)std::min<uint64_t>((nc + tile - 1) / tile, (uint64_t)p.grid_cap);
             const bool pfi = k <= 12;
             auto kp = kernel == 6
-                          ? (probs ? (pfi ? slos_layer_succ2_kernel<1, 4, true, 4> : slos_layer_succ2_kernel<1, 4, false, 4>)
-                                   : slos_layer_succ2_kernel<0, 4, true, 4>)
+                          ? (probs ? (pfi ? slos_layer_succ2_kernel<1, 3, true, 4> : slos_layer_succ2_kernel<1, 3, false, 4>)
+                                   : slos_layer_succ2_kernel<0, 3, true, 4>)
                           : (probs ? (pfi ? slos_layer_succ_kernel<1, 2, true, 8> : slos_layer_succ_kernel<1, 2, false, 8>)
                                    : slos_layer_succ_kernel<0, 2, true, 8>);
             hipLaunchKernelGGL(kp, dim3(grid), dim3(SLOS7_THREADS), succ_kernel_lds(p, kernel, k, probs), stream, prev,
@@ -1216,7 +1267,7 @@ extern "C" int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel)
     PCVL_ARG_CHECK(plan != nullptr, "plan is null");
     PCVL_ARG_CHECK(kernel >= 0 && kernel <= 7,
                    "kernel must be 0 (auto), 1 (per state), 2 (incremental), 3 (per state, 32-bit), "
-                   "4 (per state, two phases), 5 (two phases, eight loads in flight), 6 (successor runs of 4, "
+                   "4 (per state, two phases), 5 (two phases, eight loads in flight), 6 (successor runs of 3, "
                    "compact slots) or 7 (successor runs of 2, full slots)");
     if (kernel >= 4) PCVL_ARG_CHECK(twophase_ok(plan->impl), "two-phase kernel needs parent layers below 2^26 states");
     if (kernel >= 6) PCVL_ARG_CHECK(succ_ok(plan->impl, kernel), "successor kernel needs its LDS below 64 KiB");

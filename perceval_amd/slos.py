@@ -43,7 +43,7 @@ class SlosPlan:
         """0 = pick, 1 = one thread per output state, 2 = incremental runs,
         3 = one thread per output state with 32-bit ranks, 4 = one thread per
         output state, walk and gather in two phases, 5 = the same with eight
-        loads in flight, 6 = runs of 4 consecutive states per thread stepped
+        loads in flight, 6 = runs of 3 consecutive states per thread stepped
         by successor, gathered lane-consecutively (compact slot lists; the
         default where it fits), 7 = the same with runs of 2 and full slot
         lists."""

@@ -45,7 +45,7 @@ def test_fock_rank_unrank_m24_n10_sampled(gpu):
 
 
 KERNELS = (1, 2, 3, 4, 5, 6, 7)   # per state; incremental runs; 32-bit ranks; two-phase gather (4 / 8 loads
-#                                   in flight); successor runs of 4 (compact slots) / 2 (full slots) with a
+#                                   in flight); successor runs of 3 (compact slots) / 2 (full slots) with a
 #                                   transposed gather
 
 
