@@ -350,6 +350,23 @@ constexpr int SLOS7_THREADS = 256;
 
 __device__ __forceinline__ uint32_t succ_pad(uint32_t c) { return c + (c >> 5); }
 
+// Tiles of a layer in XCD-contiguous order.  Workgroups are dispatched to
+// the 8 XCDs round robin (block b on XCD b % 8), each XCD with its own L2, so
+// XCD x is given the contiguous tile range [x*T8, (x+1)*T8): neighbouring
+// tiles gather many of the same parent lines, and this keeps them in one L2
+// (at m=24, n=10 it cuts the last layer's L2 misses from 6.1 to 4.7 GB; the
+// time is unchanged, the kernel being bound by instruction issue).
+// gridDim.x is a multiple of 8; any tile order gives the same results.
+struct XcdTiles {
+    uint32_t first, end, step;
+};
+__device__ __forceinline__ XcdTiles xcd_tiles(uint32_t ntiles)
+{
+    const uint32_t x = blockIdx.x & 7u, j = blockIdx.x >> 3, t8 = (ntiles + 7u) / 8u;
+    const uint32_t lo = min(x * t8, ntiles), hi = min(lo + t8, ntiles);
+    return {lo + j, hi, gridDim.x >> 3};
+}
+
 // Unranks `first` in the k-photon layer and calls emit(q, s_q, pe) for each
 // occupied mode q in increasing order, pe being the rank offset of modes
 // below q (parent(c - e_q) = first - pe).  Empty modes are skipped four at
@@ -433,8 +450,9 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ_kernel(
     const int tid = threadIdx.x;
     const uint32_t sent_word = nparent << 6;
 #define CNT(d) cnt[(size_t)(d) * SLOS7_THREADS + tid]
-    for (uint64_t blk = (uint64_t)blockIdx.x * TILE; blk < nchild; blk += (uint64_t)gridDim.x * TILE) {
-        const uint32_t blk0 = (uint32_t)blk;
+    const XcdTiles xt = xcd_tiles((uint32_t)(((uint64_t)nchild + TILE - 1) / TILE));
+    for (uint32_t tile = xt.first; tile < xt.end; tile += xt.step) {
+        const uint32_t blk0 = tile * TILE;
         // phase 1: this thread's RUN consecutive children
         const uint32_t c0 = (uint32_t)tid * RUN;
         const uint32_t first = blk0 + c0;
@@ -575,8 +593,9 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ2_kernel(
 #define CNT(d) cnt[(size_t)(d) * SLOS7_THREADS + tid]
 #define SLOT0(d, t) slot0[(size_t)(d) * SLOS7_THREADS + (t)]
 #define OWN(s, j, t) own[(size_t)((s) * ((s) - 1) / 2 + (j)) * SLOS7_THREADS + (t)]
-    for (uint64_t blk = (uint64_t)blockIdx.x * TILE; blk < nchild; blk += (uint64_t)gridDim.x * TILE) {
-        const uint32_t blk0 = (uint32_t)blk;
+    const XcdTiles xt = xcd_tiles((uint32_t)(((uint64_t)nchild + TILE - 1) / TILE));
+    for (uint32_t tile = xt.first; tile < xt.end; tile += xt.step) {
+        const uint32_t blk0 = tile * TILE;
         // phase 1: this thread's RUN consecutive children
         const uint32_t first = blk0 + (uint32_t)tid * RUN;
         int depth = 0;
@@ -1207,7 +1226,8 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         if (usesucc) {
             const int run = kernel == 6 ? 3 : 2;
             const uint64_t tile = (uint64_t)run * SLOS7_THREADS;
-            const int grid = (int)std::min<uint64_t>((nc + tile - 1) / tile, (uint64_t)p.grid_cap);
+            // a multiple of 8 workgroups, for the XCD-contiguous tile order
+            const int grid = (int)((std::min<uint64_t>((nc + tile - 1) / tile, (uint64_t)p.grid_cap) + 7) / 8 * 8);
             const bool pfi = k <= 12;
             auto kp = kernel == 6
                           ? (probs ? (pfi ? slos_layer_succ2_kernel<1, 3, true, 4> : slos_layer_succ2_kernel<1, 3, false, 4>)
