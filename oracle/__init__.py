@@ -54,6 +54,7 @@ def lib():
         L.oracle_perm_glynn_ld.argtypes = [_dp, ctypes.c_int, _dp]
         L.oracle_perm_ryser_ld.argtypes = [_dp, ctypes.c_int, _dp]
         L.oracle_perm_glynn_d.argtypes = [_dp, ctypes.c_int, _dp]
+        L.oracle_perm_glynn_ld_mt.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int]
         L.oracle_perm_batch.argtypes = [_dp, _i32p, _i64p, ctypes.c_int64, _dp, ctypes.c_int, ctypes.c_int]
         L.oracle_binom.argtypes = [ctypes.c_int64, ctypes.c_int64]
         L.oracle_binom.restype = ctypes.c_uint64
@@ -104,8 +105,19 @@ def perm(a, method: str = "glynn_ld") -> complex:
     return complex(out[0], out[1])
 
 
+def perm_large(a, threads: int = 0) -> complex:
+    """Long-double Glynn of one matrix with its Gray-code chunks split over
+    threads (oracle_perm_glynn_ld_mt): for orders 25..28."""
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    n = a.shape[0]
+    assert a.shape == (n, n)
+    out = np.zeros(2)
+    _check(lib().oracle_perm_glynn_ld_mt(_cd(a.view(np.float64)), n, _cd(out), threads), "perm_large")
+    return complex(out[0], out[1])
+
+
 def perm_batch(mats: list, precise: bool = True, threads: int = 0) -> np.ndarray:
-    """Permanents of a list of square matrices (OpenMP over matrices)."""
+    """Permanents of a list of square matrices (POSIX threads over matrices)."""
     ns = np.array([m.shape[0] for m in mats], dtype=np.int32)
     offs = np.zeros(len(mats), dtype=np.int64)
     if len(mats) > 1:

@@ -258,6 +258,73 @@ int oracle_perm_glynn_d(const double *a, int n, double *out)
     return 0;
 }
 
+/* Long-double Glynn of one large matrix split over threads: the 2^(n-1)
+ * Gray-code steps are cut into chunks of 2^16 steps (fewer for small n); a
+ * chunk rebuilds its column sums in long double from its first Gray code,
+ * then steps as glynn_ld does.  Chunk partials are summed in chunk order, so
+ * the result does not depend on the thread count.  Same formula as glynn_ld
+ * (_naive.py:70-71's xq.permanent_cx); for orders 25..28, where the
+ * single-threaded form takes minutes. */
+typedef struct { const double *a; int n; uint64_t chunk; ld *part; } glynn_mt_ctx;
+
+static void glynn_ld_chunks(void *vctx, int64_t lo, int64_t hi)
+{
+    glynn_mt_ctx *c = (glynn_mt_ctx *)vctx;
+    const double *a = c->a;
+    int n = c->n;
+    for (int64_t k = lo; k < hi; k++) {
+        uint64_t base = (uint64_t)k * c->chunk;
+        uint64_t g = base ^ (base >> 1);
+        ld cr[OR_MAXN], ci[OR_MAXN];
+        for (int j = 0; j < n; j++) {
+            ld sr = a[2 * j], si = a[2 * j + 1];
+            for (int i = 1; i < n; i++) {
+                ld d = ((g >> (i - 1)) & 1) ? -1.0L : 1.0L;
+                sr += d * a[2 * (i * n + j)]; si += d * a[2 * (i * n + j) + 1];
+            }
+            cr[j] = sr; ci[j] = si;
+        }
+        ld accr = 0, acci = 0;
+        for (uint64_t u = 0; u < c->chunk; u++) {
+            uint64_t t = base + u;
+            if (u) {
+                int b = __builtin_ctzll(t);
+                g ^= 1ull << b;
+                ld s = ((g >> b) & 1) ? -2.0L : 2.0L;
+                const double *row = a + 2 * (b + 1) * n;
+                for (int j = 0; j < n; j++) { cr[j] += s * row[2 * j]; ci[j] += s * row[2 * j + 1]; }
+            }
+            ld pr = cr[0], pi = ci[0];
+            for (int j = 1; j < n; j++) {
+                ld nr = pr * cr[j] - pi * ci[j];
+                ld ni = pr * ci[j] + pi * cr[j];
+                pr = nr; pi = ni;
+            }
+            if (t & 1) { accr -= pr; acci -= pi; } else { accr += pr; acci += pi; }
+        }
+        c->part[2 * k] = accr; c->part[2 * k + 1] = acci;
+    }
+}
+
+int oracle_perm_glynn_ld_mt(const double *a, int n, double *out, int threads)
+{
+    if (n < 0 || n > OR_MAXN) return -1;
+    if (n == 0) { out[0] = 1; out[1] = 0; return 0; }
+    uint64_t steps = 1ull << (n - 1);
+    uint64_t chunk = steps < (1ull << 16) ? steps : (1ull << 16);
+    uint64_t nchunks = steps / chunk;
+    ld *part = (ld *)malloc(sizeof(ld) * 2 * nchunks);
+    if (!part) return -3;
+    glynn_mt_ctx c = {a, n, chunk, part};
+    or_parallel(glynn_ld_chunks, &c, (int64_t)nchunks, 1, threads);
+    ld sr = 0, si = 0;
+    for (uint64_t k = 0; k < nchunks; k++) { sr += part[2 * k]; si += part[2 * k + 1]; }
+    free(part);
+    ld scale = ldexpl(1.0L, -(n - 1));
+    out[0] = (double)(sr * scale); out[1] = (double)(si * scale);
+    return 0;
+}
+
 /* Batched permanents over a packed array: matrix b has order n[b] and starts
  * at complex element offsets[b].  precise=1 uses the long-double Glynn,
  * precise=0 the chunked double Glynn.  Threads over matrices. */

@@ -476,8 +476,12 @@ struct pcvl_perm_plan_s {
     PermPlanImpl impl;
 };
 
-extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
-                                        int max_chunk_log2, int variant, pcvl_perm_plan *out)
+// Work items above this count are refused (24 bytes each on the host and the
+// device); partial-sum slots are int32 on the device.
+static constexpr int64_t PERM_MAX_ITEMS = int64_t(1) << 27;
+
+static int perm_plan_create_impl(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                                 int max_chunk_log2, int variant, pcvl_perm_plan *out)
 {
     PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
     PCVL_ARG_CHECK(variant >= 0 && variant <= 4, "variant outside [0, 4]");
@@ -533,6 +537,15 @@ extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_off
             wave_c[i] = std::max(2, wave_c[i] - 2);
         }
     }
+    int64_t total_slots = 0;
+    for (size_t i = 0; i < wave_idx.size(); i++)
+        total_slots += (int64_t)((1ull << (wave_n[i] - 1 - wave_c[i])) / 64);
+    if (total_slots > PERM_MAX_ITEMS) {
+        set_error("permanent batch needs " + std::to_string(total_slots) + " work items, above the limit of " +
+                  std::to_string(PERM_MAX_ITEMS) + " (use a longer chunk or split the batch)");
+        return ERR_RANGE;
+    }
+    items.reserve((size_t)total_slots);
     int32_t slots = 0;
     for (size_t i = 0; i < wave_idx.size(); i++) {
         const int n = wave_n[i], c = wave_c[i];
@@ -611,6 +624,20 @@ extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_off
     if (p.nitems > 0) p.grid = (int)std::min<int64_t>(p.nitems, (int64_t)rw);
     *out = plan;
     return OK;
+}
+
+extern "C" int pcvl_perm_plan_create_ex(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                                        int max_chunk_log2, int variant, pcvl_perm_plan *out)
+{
+    try {
+        return perm_plan_create_impl(h_n, h_offsets, count, max_chunk_log2, variant, out);
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed while building the permanent plan");
+        return ERR_ALLOC;
+    } catch (...) {
+        set_error("unexpected exception while building the permanent plan");
+        return ERR_ALLOC;
+    }
 }
 
 extern "C" int pcvl_perm_plan_create(const int32_t *h_n, const int64_t *h_offsets, int64_t count,

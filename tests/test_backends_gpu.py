@@ -1,226 +1,183 @@
-"""The reference's backend tests (tests/backends/test_backends.py), run against
-this simulation core's Naive / SLOS / CliffordClifford2017 backends on the GPU.
-Expected values are the reference's own."""
-import math
+"""Backend API on the GPU against the expected values the reference's own
+tests and documentation hold.
 
+The values are data (tests/golden/backend_cases.json, each case with the
+reference file:line it comes from); this module only builds each case's
+circuit, sets up the named backends through BackendFactory and applies one
+checker per kind of case.  Strong-simulation values are held to 1e-12
+absolute, the SLOS bar."""
+import json
+import math
+import os
+
+import numpy as np
 import pytest
 
-from perceval_amd import (BasicState, BS, BSCount, BackendFactory, Circuit, PS, SLOSBackend, StateVector, catalog)
+from perceval_amd import BS, PERM, PS, BackendFactory, BasicState, Circuit, SLOSBackend, catalog
 
 pytestmark = pytest.mark.gpu
 
-STRONG = ["SLOS", "Naive"]
+TOL = 1e-12
+_CASES = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "backend_cases.json")))["cases"]
 
 
-def _assert_cnot(backend):
-    s00 = BasicState([1, 0, 1, 0, 0, 0])
-    s01 = BasicState([1, 0, 0, 1, 0, 0])
-    s10 = BasicState([0, 1, 1, 0, 0, 0])
-    s11 = BasicState([0, 1, 0, 1, 0, 0])
-    backend.set_input_state(s00)
-    assert pytest.approx(backend.probability(s00)) == 1 / 9
-    assert pytest.approx(backend.probability(s01)) == 0
-    backend.set_input_state(s01)
-    assert pytest.approx(backend.probability(s01)) == 1 / 9
-    assert pytest.approx(backend.probability(s00)) == 0
-    backend.set_input_state(s10)
-    assert pytest.approx(backend.probability(s11)) == 1 / 9
-    assert pytest.approx(backend.probability(s10)) == 0
-    backend.set_input_state(s11)
-    assert pytest.approx(backend.probability(s11)) == 0
-    assert pytest.approx(backend.probability(s10)) == 1 / 9
+def _component(spec):
+    if "bs" in spec:
+        kw = {k: v for k, v in spec["bs"].items() if k != "convention"}
+        return {"H": BS.H, "Rx": BS.Rx}[spec["bs"]["convention"]](**kw)
+    if "ps" in spec:
+        return PS(spec["ps"])
+    if "perm" in spec:
+        return PERM(spec["perm"])
+    raise ValueError(spec)
 
 
-def check_output_distribution(backend, input_state, expected: dict):
-    backend.set_input_state(input_state)
-    prob_list = []
-    for output_state, prob in backend.prob_distribution().items():
-        prob_expected = expected.get(output_state)
-        if prob_expected is None:
-            assert pytest.approx(0) == prob, "cannot find: %s (prob=%f)" % (str(output_state), prob)
-        else:
-            assert pytest.approx(prob_expected) == prob
-        prob_list.append(prob)
-    assert pytest.approx(sum(prob_list)) == 1
+def build_circuit(spec):
+    if "identity" in spec:
+        return Circuit(spec["identity"])
+    if "catalog" in spec:
+        return catalog[spec["catalog"]].build_circuit()
+    if "bs" in spec:
+        return _component(spec)
+    c = Circuit(spec["m"])
+    for modes, comp in spec["ops"]:
+        c.add(tuple(modes) if len(modes) > 1 else modes[0], _component(comp))
+    return c
 
 
-def test_clifford_bs(gpu):
-    cliff_bs = BackendFactory.get_backend("CliffordClifford2017")
-    cliff_bs.set_circuit(BS.H())
-    cliff_bs.set_input_state(BasicState([0, 1]))
-    counts = BSCount()
-    n_samples = 10000
-    for s in cliff_bs.samples(n_samples):
-        counts[s] += 1
-    assert n_samples * 0.475 < counts[BasicState("|0,1>")] < n_samples * 0.525
-    assert n_samples * 0.475 < counts[BasicState("|1,0>")] < n_samples * 0.525
+def make_backend(name, case, circuit_spec=None):
+    b = BackendFactory.get_backend() if name == "<default>" else BackendFactory.get_backend(name)
+    if "mask" in case:
+        b.set_mask(case["mask"])
+    b.set_circuit(build_circuit(circuit_spec or case["circuit"]))
+    return b
 
 
-def test_backend_factory_default(gpu):
-    default_backend = BackendFactory.get_backend()
-    default_backend.set_circuit(BS.H())
-    check_output_distribution(default_backend, BasicState([1, 0]),
-                              {BasicState("|1,0>"): 0.5, BasicState("|0,1>"): 0.5})
+def _check_distribution(b, inp, expected):
+    b.set_input_state(BasicState(inp))
+    got = b.prob_distribution()
+    want = {BasicState(k): v for k, v in expected.items()}
+    for s, p in got.items():
+        assert abs(p - want.get(s, 0.0)) < TOL, (s, p, want.get(s))
+    for s, p in want.items():
+        assert abs(got.get(s, 0.0) - p) < TOL, (s, p)
+    assert abs(sum(got.values()) - 1.0) < TOL
 
 
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_backend_wiring(gpu, backend_name):
-    backend = BackendFactory.get_backend(backend_name)
-    backend.set_circuit(Circuit(1))
-    check_output_distribution(backend, BasicState([1]), {BasicState("|1>"): 1})
+def _distribution(case, name):
+    _check_distribution(make_backend(name, case), case["input"], case["expected"])
 
 
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_backend_identity(gpu, backend_name):
-    backend = BackendFactory.get_backend(backend_name)
-    backend.set_circuit(Circuit(2))
-    check_output_distribution(backend, BasicState([0, 0]), {BasicState("|0,0>"): 1})
-    check_output_distribution(backend, BasicState([0, 1]), {BasicState("|0,1>"): 1})
-    check_output_distribution(backend, BasicState([1, 1]), {BasicState("|1,1>"): 1})
+def _probabilities(case, name):
+    b = make_backend(name, case)
+    for inp, out, p in case["table"]:
+        b.set_input_state(BasicState(inp))
+        assert abs(b.probability(BasicState(out)) - p) < TOL, (inp, out)
 
 
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_backend_sym_bs(gpu, backend_name):
-    backend = BackendFactory.get_backend(backend_name)
-    backend.set_circuit(BS.H())
-    check_output_distribution(backend, BasicState("|2,0>"),
-                              {BasicState("|2,0>"): 0.25, BasicState("|1,1>"): 0.5, BasicState("|0,2>"): 0.25})
-    check_output_distribution(backend, BasicState("|1,0>"),
-                              {BasicState("|1,0>"): 0.5, BasicState("|0,1>"): 0.5})
-    check_output_distribution(backend, BasicState("|1,1>"),
-                              {BasicState("|2,0>"): 0.5, BasicState("|0,2>"): 0.5})
+def _mass_where_any(case, name):
+    b = make_backend(name, case)
+    b.set_input_state(BasicState(case["input"]))
+    mass = sum(p for s, p in b.prob_distribution().items() if any(s[j] for j in case["modes"]))
+    assert abs(mass - case["expected"]) < TOL
 
 
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_backend_asym_bs(gpu, backend_name):
-    backend = BackendFactory.get_backend(backend_name)
-    backend.set_circuit(BS.H(theta=2 * math.pi / 3))
-    check_output_distribution(backend, BasicState("|2,0>"),
-                              {BasicState("|2,0>"): 0.0625, BasicState("|1,1>"): 0.3750,
-                               BasicState("|0,2>"): 0.5625})
-    check_output_distribution(backend, BasicState("|1,0>"),
-                              {BasicState("|1,0>"): 0.25, BasicState("|0,1>"): 0.75})
+def _exact_support(case, name):
+    b = make_backend(name, case)
+    b.set_input_state(BasicState(case["input"]))
+    got = b.prob_distribution()
+    want = {BasicState(k): v for k, v in case["expected"].items()}
+    assert set(got.keys()) == set(want.keys())
+    for s, p in want.items():
+        assert abs(got[s] - p) < TOL
 
 
-def test_slos_precomputation(gpu):
+def _refresh(case, name):
+    b = BackendFactory.get_backend(name)
+    for step in case["steps"]:
+        b.set_circuit(build_circuit(step["circuit"]))
+        for inp in step.get("inputs", []):
+            b.set_input_state(BasicState(inp))
+        _check_distribution(b, step["check"], step["expected"])
+
+
+def _evolve(case, name):
+    b = make_backend(name, case)
+    b.set_input_state(BasicState(case["input"]))
+    sv = b.evolve()
+    want = {BasicState(k): complex(*v) for k, v in case["expected"].items()}
+    for s in set(sv.keys()) | set(want.keys()):
+        assert abs(sv[s] - want.get(s, 0)) < TOL, s
+
+
+def _sample_band(case, name):
+    b = make_backend(name, case)
+    b.set_input_state(BasicState(case["input"]))
+    counts = b.samples(case["samples"]).to_counts()
+    for s, (lo, hi) in case["band"].items():
+        assert lo * case["samples"] < counts[BasicState(s)] < hi * case["samples"], s
+
+
+def _amplitude(case, name):
+    b = make_backend(name, case)
+    b.set_input_state(BasicState(case["input"]))
+    assert abs(b.prob_amplitude(BasicState(case["output"])) - complex(*case["expected"])) < TOL
+
+
+CHECKS = {"distribution": _distribution, "probabilities": _probabilities, "mass_where_any": _mass_where_any,
+          "exact_support": _exact_support, "refresh": _refresh, "evolve": _evolve, "sample_band": _sample_band,
+          "amplitude": _amplitude}
+
+
+@pytest.mark.parametrize("case,name", [(c, n) for c in _CASES for n in c["backends"]],
+                         ids=[f"{c['id']}-{n}" for c in _CASES for n in c["backends"]])
+def test_reference_case(gpu, case, name):
+    CHECKS[case["kind"]](case, name)
+
+
+def test_every_case_kind_has_a_checker():
+    assert {c["kind"] for c in _CASES} <= set(CHECKS)
+
+
+def test_slos_layers_are_shared_across_inputs(gpu):
+    """_fsms grows one layer per photon number and keeps its layer objects
+    when a new input reuses them (the deployed path of _slos.py:151-168)."""
     slos = SLOSBackend()
     slos.set_circuit(Circuit(2))
     slos.set_input_state(BasicState([0, 1]))
+    one_photon = slos._fsms[-1]
     assert len(slos._fsms) == 2
-    fsm_precompute_1 = slos._fsms[-1]
     slos.set_input_state(BasicState([1, 0]))
-    assert len(slos._fsms) == 2
-    assert fsm_precompute_1 is slos._fsms[-1]
+    assert len(slos._fsms) == 2 and slos._fsms[-1] is one_photon
     slos.set_input_state(BasicState([1, 1]))
-    assert len(slos._fsms) == 3
-    assert fsm_precompute_1 is slos._fsms[1]
+    assert len(slos._fsms) == 3 and slos._fsms[1] is one_photon
 
 
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_backend_cnot(gpu, backend_name):
-    backend = BackendFactory.get_backend(backend_name)
-    cnot = catalog["postprocessed cnot"].build_circuit()
-    backend.set_circuit(cnot)
-    _assert_cnot(backend)
-    non_post_selected_probability = 0
-    backend.set_input_state(BasicState([1, 0, 1, 0, 0, 0]))
-    for output_state, prob in backend.prob_distribution().items():
-        if output_state[4] or output_state[5]:
-            non_post_selected_probability += prob
-    assert pytest.approx(non_post_selected_probability) == 7 / 9
-
-
-def test_cnot_with_mask(gpu):
-    backend = BackendFactory.get_backend("SLOS")
-    backend.set_mask(["    00"])
-    cnot = catalog["postprocessed cnot"].build_circuit()
-    backend.set_circuit(cnot)
-    _assert_cnot(backend)
-    non_post_selected_probability = 0
-    backend.set_input_state(BasicState([0, 1, 0, 1, 0, 0]))
-    for output_state, prob in backend.prob_distribution().items():
-        if output_state[4] or output_state[5]:
-            non_post_selected_probability += prob
-    assert pytest.approx(non_post_selected_probability) == 0
-
-
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_strong_sim_with_mask(gpu, backend_name):
-    backend = BackendFactory.get_backend(backend_name)
-    backend.set_mask("****00")
-    cnot = catalog["postprocessed cnot"].build_circuit()
-    backend.set_circuit(cnot)
-    logical00 = BasicState([1, 0, 1, 0, 0, 0])
-    backend.set_input_state(logical00)
-    bsd = backend.prob_distribution()
-    assert len(bsd) == 2
-    assert bsd[logical00] == pytest.approx(1 / 9)
-    assert bsd[BasicState([1, 1, 0, 0, 0, 0])] == pytest.approx(1 / 9)
-
-
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_probampli_backends(gpu, backend_name):
-    backend = BackendFactory.get_backend(backend_name)
-    circuit = Circuit(3) // BS.H() // (1, PS(math.pi / 4)) // (1, BS.H())
-    backend.set_circuit(circuit)
-    check_output_distribution(
-        backend, BasicState("|0,1,1>"),
-        {BasicState("|0,1,1>"): 0, BasicState("|1,1,0>"): 0.25, BasicState("|1,0,1>"): 0.25,
-         BasicState("|2,0,0>"): 0, BasicState("|0,2,0>"): 0.25, BasicState("|0,0,2>"): 0.25})
-    backend.set_circuit(BS())
-    check_output_distribution(
-        backend, BasicState("|2,3>"),
-        {BasicState("|5,0>"): 0.3125, BasicState("|4,1>"): 0.0625, BasicState("|3,2>"): 0.125,
-         BasicState("|2,3>"): 0.125, BasicState("|1,4>"): 0.0625, BasicState("|0,5>"): 0.3125})
-
-
-def test_slos_refresh_coefs(gpu):
-    slos = SLOSBackend()
-    slos.set_circuit(BS())
-    slos.set_input_state(BasicState("|1,1>"))
-    slos.set_input_state(BasicState("|8,5>"))
-    check_output_distribution(slos, BasicState("|1,1>"), {BasicState("|0,2>"): 0.5, BasicState("|2,0>"): 0.5})
-    slos.set_circuit(Circuit(2))
-    check_output_distribution(slos, BasicState("|1,1>"), {BasicState("|1,1>"): 1})
-
-
-def _assert_sv_close(a, b):
-    keys = set(a.keys()) | set(b.keys())
-    for k in keys:
-        assert a[k] == pytest.approx(b[k], abs=1e-9)
-
-
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_evolve_indistinguishable(gpu, backend_name):
-    backend = BackendFactory.get_backend(backend_name)
-    backend.set_circuit(BS.H())
-    backend.set_input_state(BasicState([1, 0]))
-    _assert_sv_close(backend.evolve(), math.sqrt(2) / 2 * StateVector([1, 0]) + math.sqrt(2) / 2 * StateVector([0, 1]))
-    backend.set_input_state(BasicState([1, 1]))
-    _assert_sv_close(backend.evolve(), math.sqrt(2) / 2 * StateVector([2, 0]) - math.sqrt(2) / 2 * StateVector([0, 2]))
-
-
-def test_docs_examples(gpu):
-    # docs/source/reference/backends/naive.rst and slos.rst
-    c = Circuit(4) // BS() // (2, BS())
-    naive = BackendFactory.get_backend("Naive")
-    naive.set_circuit(c)
-    naive.set_input_state(BasicState([1, 0, 1, 0]))
-    assert naive.prob_amplitude(BasicState([1, 0, 0, 1])) == pytest.approx(0.5j)
-    slos = BackendFactory.get_backend("SLOS")
-    slos.set_circuit(c)
-    slos.set_input_state(BasicState([1, 0, 1, 0]))
-    nz = {s: p for s, p in slos.prob_distribution().items() if p > 1e-12}
-    assert len(nz) == 4 and all(p == pytest.approx(0.25) for p in nz.values())
-
-
-@pytest.mark.parametrize("backend_name", STRONG)
-def test_probampli_iterator_cache(gpu, backend_name):
-    from perceval_amd import BackendFactory as BF
-    b = BF.get_backend(backend_name)
+@pytest.mark.parametrize("name", ["Naive", "SLOS"])
+def test_iterator_cache_cleared_by_a_wider_circuit(gpu, name):
+    b = BackendFactory.get_backend(name)
     b.set_circuit(Circuit(5).add(0, BS.H()))
     b.set_input_state(BasicState([1, 1, 0, 0, 0]))
     b.evolve()
-    assert len(b._cache_iterator) != 0 or backend_name == "SLOS"
+    if name == "Naive":
+        assert len(b._cache_iterator) != 0
     b.set_circuit(Circuit(7).add(3, BS.H()))
     assert len(b._cache_iterator) == 0
+
+
+@pytest.mark.parametrize("name", ["Naive", "SLOS"])
+def test_prob_iterator_drops_states_below_min_p(gpu, name):
+    """prob_iterator(min_p) (_abstract_backends.py:197-201) yields the states
+    whose probability exceeds min_p, in all_prob order."""
+    b = BackendFactory.get_backend(name)
+    b.set_circuit(build_circuit({"m": 3, "ops": [[[0, 1], {"bs": {"convention": "H"}}], [[1], {"ps": math.pi / 4}],
+                                                 [[1, 2], {"bs": {"convention": "H"}}]]}))
+    b.set_input_state(BasicState("|0,1,1>"))
+    probs = b.all_prob()
+    states = list(b._get_iterator(BasicState("|0,1,1>")))
+    for min_p in (1e-16, 0.2, 0.3):
+        it = list(b.prob_iterator(min_p))
+        assert [s for s, _ in it] == [s for s, p in zip(states, probs) if p > min_p]
+        assert all(p > min_p for _, p in it)
+    assert len(list(b.prob_iterator(1e-16))) == 4     # |0,1,1> and |2,0,0> have probability 0
+    assert np.isclose(sum(p for _, p in b.prob_iterator()), 1.0, atol=TOL)

@@ -1,0 +1,224 @@
+"""The reference-side binding (perceval_amd.perceval_binding) on the CPU.
+
+perceval_amd.backends stands in for perceval.backends as the parent module.
+Every parent method that would run the parent's own arithmetic (the SLOS
+path recursion, its coefficient reads, the Naive permanent, the sampler and
+the parents' state iterators) is replaced by one that raises, and the GPU
+layer the binding calls (SLOS plans and trees, Fock ranking, mask selection,
+permanent plans, the sampler) by the oracle.  The bound backends must then
+reproduce the reference's expected values without touching the parents."""
+import json
+import os
+import types
+
+import numpy as np
+import pytest
+
+import oracle
+from perceval_amd import backends as amd_backends
+from perceval_amd import perceval_binding, permanent, sampling, slos
+from perceval_amd.backends import _abstract_backends
+from perceval_amd.utils import states as amd_states
+from perceval_amd.utils.mask import FSMask
+
+torch = pytest.importorskip("torch")
+
+_CASES = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "backend_cases.json")))["cases"]
+
+
+class ParentCalled(AssertionError):
+    pass
+
+
+def _forbid(name):
+    def f(*a, **k):
+        raise ParentCalled(f"the binding called the parent's {name}")
+    return f
+
+
+class _FakePlan:
+    """SlosPlan on the oracle: coefficients and probabilities in FSArray order."""
+
+    def __init__(self, in_occ):
+        self.in_occ = list(in_occ)
+
+    def run(self, U, mode=slos.PROBS, out=None, stream=None):
+        U = np.asarray(U)
+        if mode == slos.COEFS:
+            return torch.from_numpy(oracle.slos_coefs(U, self.in_occ))
+        return torch.from_numpy(oracle.slos_probs(U, self.in_occ))
+
+
+class _FakeTree:
+    def __init__(self, inputs):
+        self.plans = [_FakePlan(s) for s in inputs]
+
+    def run(self, U, mode=slos.PROBS, out=None, stream=None):
+        return [p.run(U, mode) for p in self.plans]
+
+
+class _FakePermPlan:
+    def __init__(self, orders, *a, **k):
+        self.orders = np.asarray(orders, dtype=np.int64)
+
+    def __call__(self, packed, out=None, stream=None):
+        flat = packed.numpy() if hasattr(packed, "numpy") else np.asarray(packed)
+        res, off = [], 0
+        for n in self.orders:
+            res.append(oracle.perm(flat[off:off + n * n].reshape(n, n)) if n else 1.0)
+            off += n * n
+        r = torch.from_numpy(np.array(res, dtype=np.complex128))
+        if out is not None:
+            out.copy_(r)
+            return out
+        return r
+
+    def close(self):
+        pass
+
+
+class _FakeSingle:
+    def __call__(self, M):
+        return oracle.perm(np.asarray(M))
+
+
+def _fake_mask_select(m, n, masks, at_least=None):
+    fm = FSMask(m, n, masks, at_least)
+    states = oracle.fock_enumerate(m, n)
+    return torch.tensor([i for i, s in enumerate(states) if fm.match(list(s))], dtype=torch.int64)
+
+
+@pytest.fixture
+def bound(monkeypatch):
+    # the parents' arithmetic must never run
+    S, N, C = amd_backends.SLOSBackend, amd_backends.NaiveBackend, amd_backends.Clifford2017Backend
+    for cls, names in ((S, ["preprocess", "_compute", "prob_amplitude", "all_prob", "all_prob_tensor",
+                            "prob_distribution", "evolve", "_deploy", "_masked_indices_device"]),
+                       (N, ["_compute_permanent", "_all_amplitudes", "all_prob", "prob_distribution", "evolve"]),
+                       (C, ["sample", "samples", "samples_for_inputs"]),
+                       (_abstract_backends.AStrongSimulationBackend, ["_get_iterator"])):
+        for name in names:
+            monkeypatch.setattr(cls, name, _forbid(f"{cls.__name__}.{name}"))
+    # the GPU layer, replaced by the oracle
+    monkeypatch.setattr(slos, "SlosPlan", _FakePlan)
+    monkeypatch.setattr(slos, "SlosTree", _FakeTree)
+    monkeypatch.setattr(slos, "fock_unrank",
+                        lambda ranks, m, n: np.array([oracle.fock_unrank(int(r), m, n) for r in ranks], dtype=np.int32))
+    monkeypatch.setattr(slos, "fock_mask_select", _fake_mask_select)
+    monkeypatch.setattr(slos, "gather_f64", lambda src, idx: src[idx])
+    monkeypatch.setattr(permanent, "PermanentPlan", _FakePermPlan)
+    monkeypatch.setattr(perceval_binding, "_SinglePermanents", _FakeSingle)
+    monkeypatch.setattr(sampling, "cc_samples", lambda U, inp, count, seed, first_sample=0:
+                        oracle.cc_samples(np.asarray(U), inp, count, seed=seed % (2 ** 63)))
+    monkeypatch.setattr(torch.Tensor, "cuda", lambda self, *a, **k: self)
+    parent = types.SimpleNamespace(SLOSBackend=S, NaiveBackend=N, Clifford2017Backend=C,
+                                   BACKEND_LIST=dict(amd_backends.BACKEND_LIST))
+    t = types.SimpleNamespace(FockState=amd_states.FockState, BSDistribution=amd_states.BSDistribution,
+                              StateVector=amd_states.StateVector, BSSamples=amd_states.BSSamples)
+    previous = perceval_binding.install(parent, t)
+    assert previous["SLOS"] is S and previous["Naive"] is N
+    return parent
+
+
+def _backend(parent, name, case):
+    from test_backends_gpu import build_circuit
+    b = parent.BACKEND_LIST["SLOS" if name == "<default>" else name]()
+    if "mask" in case:
+        b.set_mask(case["mask"])
+    b.set_circuit(build_circuit(case["circuit"]))
+    return b
+
+
+def test_install_replaces_the_registry_entries(bound):
+    for name in ("SLOS", "Naive", "CliffordClifford2017"):
+        cls = bound.BACKEND_LIST[name]
+        assert cls.__name__.startswith("MI355X")
+        assert issubclass(cls, {"SLOS": amd_backends.SLOSBackend, "Naive": amd_backends.NaiveBackend,
+                                "CliffordClifford2017": amd_backends.Clifford2017Backend}[name])
+    assert bound.BACKEND_LIST["NaiveApprox"] is amd_backends.BACKEND_LIST["NaiveApprox"]
+
+
+@pytest.mark.parametrize("case,name", [(c, n) for c in _CASES if c["kind"] in ("distribution", "probabilities",
+                                                                                 "exact_support", "evolve",
+                                                                                 "amplitude", "mass_where_any")
+                                       for n in c["backends"]],
+                         ids=[f"{c['id']}-{n}" for c in _CASES if c["kind"] in ("distribution", "probabilities",
+                                                                                  "exact_support", "evolve",
+                                                                                  "amplitude", "mass_where_any")
+                              for n in c["backends"]])
+def test_bound_backends_reproduce_reference_values(bound, case, name):
+    tol = 1e-12
+    b = _backend(bound, name, case)
+    BS_ = amd_states.BasicState
+    if case["kind"] == "distribution":
+        b.set_input_state(BS_(case["input"]))
+        got = b.prob_distribution()
+        want = {BS_(k): v for k, v in case["expected"].items()}
+        for s in set(got) | set(want):
+            assert abs(got.get(s, 0.0) - want.get(s, 0.0)) < tol, s
+    elif case["kind"] == "probabilities":
+        for inp, out, p in case["table"]:
+            b.set_input_state(BS_(inp))
+            assert abs(b.probability(BS_(out)) - p) < tol
+    elif case["kind"] == "exact_support":
+        b.set_input_state(BS_(case["input"]))
+        got = b.prob_distribution()
+        assert set(got) == {BS_(k) for k in case["expected"]}
+    elif case["kind"] == "evolve":
+        b.set_input_state(BS_(case["input"]))
+        sv = b.evolve()
+        for k, v in case["expected"].items():
+            assert abs(sv[BS_(k)] - complex(*v)) < tol
+    elif case["kind"] == "amplitude":
+        b.set_input_state(BS_(case["input"]))
+        assert abs(b.prob_amplitude(BS_(case["output"])) - complex(*case["expected"])) < tol
+    elif case["kind"] == "mass_where_any":
+        b.set_input_state(BS_(case["input"]))
+        mass = sum(p for s, p in b.prob_distribution().items() if any(s[j] for j in case["modes"]))
+        assert abs(mass - case["expected"]) < tol
+
+
+def test_bound_slos_keeps_paths_for_a_same_size_circuit(bound):
+    from perceval_amd.components import Unitary
+    from perceval_amd.utils.workloads import haar_unitary
+    U1, U2 = haar_unitary(5, seed=1), haar_unitary(5, seed=2)
+    b = bound.BACKEND_LIST["SLOS"]()
+    b.set_circuit(Unitary(U1))
+    b.set_input_state(amd_states.BasicState([1, 1, 0, 1, 0]))
+    plans = dict(b._plans)
+    assert np.allclose(b.all_prob(), oracle.slos_probs(U1, [1, 1, 0, 1, 0]), atol=1e-13)
+    b.set_circuit(Unitary(U2))
+    b.set_input_state(amd_states.BasicState([1, 1, 0, 1, 0]))
+    assert b._plans == plans                      # the deployed path is kept
+    assert np.allclose(b.all_prob(), oracle.slos_probs(U2, [1, 1, 0, 1, 0]), atol=1e-13)
+    assert b.preprocess([amd_states.BasicState([1, 1, 0, 1, 0])]) is False
+    assert b.preprocess([amd_states.BasicState([1, 0, 0, 1, 0]), amd_states.BasicState([2, 0, 0, 0, 0])]) is True
+
+
+def test_bound_naive_single_amplitudes_and_batch_agree(bound):
+    from perceval_amd.components import Unitary
+    from perceval_amd.utils.workloads import haar_unitary
+    U = haar_unitary(6, seed=3)
+    b = bound.BACKEND_LIST["Naive"]()
+    b.set_circuit(Unitary(U))
+    b.set_input_state(amd_states.BasicState([1, 0, 2, 0, 1, 0]))
+    probs = b.all_prob()
+    states = b._get_iterator(amd_states.BasicState([1, 0, 2, 0, 1, 0]))
+    assert len(states) == oracle.fock_count(6, 4)
+    for k in (0, 5, len(states) - 1):
+        assert abs(b.probability(states[k]) - probs[k]) < 1e-13
+    assert np.allclose(probs, oracle.slos_probs(U, [1, 0, 2, 0, 1, 0]), atol=1e-13)
+
+
+def test_bound_clifford_samples_from_the_library(bound):
+    from perceval_amd import random_seed
+    from perceval_amd.components import Unitary
+    from perceval_amd.utils.workloads import haar_unitary
+    b = bound.BACKEND_LIST["CliffordClifford2017"]()
+    b.set_circuit(Unitary(haar_unitary(4, seed=4)))
+    b.set_input_state(amd_states.BasicState([1, 1, 0, 0]))
+    random_seed(3)
+    s1 = b.samples(50)
+    random_seed(3)
+    assert b.samples(50) == s1 and len(s1) == 50 and all(s.n == 2 for s in s1)
+    assert b.sample().m == 4

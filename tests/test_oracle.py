@@ -354,3 +354,25 @@ def test_gurvits_oracle_exact_cases_and_coverage():
     misses = sum(abs(oracle.gurvits(A, 2000, seed=7, stream=k)[0] - exact) > oracle.gurvits(A, 2000, seed=7, stream=k)[1]
                  for k in range(200))
     assert misses <= 8   # expectation 2 at 99%
+
+
+def test_perm_large_threaded_equals_single_thread_glynn():
+    """oracle.perm_large (chunked over threads, for orders 25..28) against the
+    single-threaded long-double Glynn, the mpmath goldens and a closed form;
+    its result does not depend on the thread count."""
+    import json
+    import os
+    U = haar_unitary(64, seed=3)
+    rng = np.random.RandomState(0)
+    for n in (0, 1, 2, 5, 12, 17, 20):
+        M = U[np.ix_(rng.choice(64, n, replace=False), rng.choice(64, n, replace=False))]
+        a = oracle.perm(M)
+        b = oracle.perm_large(M, threads=4)
+        assert abs(a - b) <= 1e-15 * max(abs(a), 1e-300), n
+        assert b == oracle.perm_large(M, threads=1)
+    for c in _golden():
+        A = np.array(c["re"]) + 1j * np.array(c["im"])
+        want = complex(float(c["perm"][0]), float(c["perm"][1]))
+        assert _rel(oracle.perm_large(A, threads=3), want) < 1e-13, c["n"]
+    J = np.ones((22, 22), dtype=complex) / 2
+    assert abs(oracle.perm_large(J, threads=8) / (math.factorial(22) / 2 ** 22) - 1) < 1e-14

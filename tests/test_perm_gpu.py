@@ -154,3 +154,14 @@ def test_fp64_probe_counts_and_values(gpu):
         assert abs(got[lane] - ref) <= 1e-12 * abs(ref)
     with pytest.raises(ValueError):
         _native.check(lib.pcvl_probe_fp64_fma(0, 10, _native.ptr(out), ctypes.byref(flops), None), "probe")
+
+
+def test_plan_refuses_more_work_items_than_it_can_index(gpu):
+    """4096 order-28 matrices cut into 2^2-step chunks need 2^31 work items:
+    the plan returns the range status instead of overflowing its 32-bit
+    partial-sum slots."""
+    from perceval_amd._native import NativeError
+    from perceval_amd.permanent import PermanentPlan
+    with pytest.raises(NativeError, match="work items"):
+        PermanentPlan([28] * 4096, max_chunk_log2=2)
+    PermanentPlan([28] * 4, max_chunk_log2=11).close()
