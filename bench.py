@@ -81,6 +81,18 @@ def cpu_threads() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (first processor of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 class Timer:
     """HIP events on torch's current stream, which is the stream every
     entry point of this library is launched on."""
@@ -138,6 +150,35 @@ def fp64_probe_tflops() -> float:
     return flops.value / (t.ms / 3 * 1e-3) / 1e12
 
 
+def fp64_mix_probe() -> dict:
+    """FP64 matrix cores beside the vector ALU (pcvl_probe_fp64_mix): TFLOP/s
+    of v_mfma_f64_16x16x4f64 alone, v_fma_f64 alone, and both in one loop."""
+    import ctypes
+    import torch
+    from perceval_amd import _native
+    lib = _native.load()
+    blocks = torch.cuda.get_device_properties(0).multi_processor_count * 8
+    out = torch.empty(blocks * 256, dtype=torch.float64, device="cuda")
+    fm, fv = ctypes.c_int64(), ctypes.c_int64()
+    iters = 20000
+    res = {}
+    for mode, name in ((0, "mfma"), (1, "valu"), (2, "both")):
+        def launch():
+            _native.check(lib.pcvl_probe_fp64_mix(blocks, iters, mode, _native.ptr(out), ctypes.byref(fm),
+                                                  ctypes.byref(fv), _native.stream_handle()), "mix probe")
+        launch()
+        with Timer() as t:
+            for _ in range(3):
+                launch()
+        sec = t.ms / 3 * 1e-3
+        res[name] = {"ms": t.ms / 3, "mfma_tflops": fm.value / sec / 1e12, "valu_tflops": fv.value / sec / 1e12}
+    if not bool(torch.isfinite(out).all()):
+        raise RuntimeError("FP64 mix probe produced non-finite values")
+    b = res["both"]
+    res["combined_tflops"] = b["mfma_tflops"] + b["valu_tflops"]
+    return res
+
+
 def bench_copy(args, res):
     """Plain 1 GiB device-to-device copy, three times: the reference point that
     scales FETCH_SIZE / WRITE_SIZE counter readings to bytes, and the HBM rate
@@ -193,7 +234,7 @@ def bench_gurvits(args, res):
         t0 = time.perf_counter()
         oracle.gurvits(mats[0], k, seed=5)
         dt = time.perf_counter() - t0
-        res["gurvits_cpu"] = {"value": k / dt, "unit": "samples/s", "cores": 1, "kind": "port",
+        res["gurvits_cpu"] = {"value": k / dt, "unit": "samples/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
                               "sample": f"{k} samples of one order-30 matrix, oracle (oracle/pcvl_oracle.c), 1 thread"}
         log(f"[bench] Gurvits CPU: {k / dt:.0f} samples/s on 1 thread")
 
@@ -214,6 +255,18 @@ def bench_perm(args, res):
     plan.close()
     achieved = len(o) * glynn_flops(n) / (ms * 1e-3) / 1e12
     res["perm"] = {"ms": ms, "perms_per_s": len(o) / (ms * 1e-3), "tflops": achieved}
+    # the timed launch's own output, checked on a spread sample against the
+    # oracle's long-double Glynn (outside the timed region)
+    import oracle
+    pick = np.unique(np.linspace(0, len(mats) - 1, 24).astype(int))
+    got = out.cpu().numpy()[pick]
+    ref = oracle.perm_batch([mats[i] for i in pick], precise=True, threads=cpu_threads())
+    rel = float(np.max(np.abs(got - ref) / np.abs(ref)))
+    res["perm"]["checked"] = int(len(pick))
+    res["perm"]["max_rel_err"] = rel
+    log(f"[bench] perm: {len(pick)} of the timed permanents against the oracle: max rel err {rel:.2e}")
+    if not rel <= 1e-10:
+        raise RuntimeError(f"timed permanents differ from the oracle (max rel err {rel:.3e})")
     log(f"[bench] perm: {ms:.3f} ms/step, {len(o) / (ms * 1e-3):.1f} perms/s, {achieved:.2f} TFLOP/s "
         f"({100 * achieved / FP64_PEAK_TFLOPS:.1f}% of FP64 peak)")
     probe = fp64_probe_tflops()
@@ -221,6 +274,11 @@ def bench_perm(args, res):
     res["perm"]["frac_of_probe"] = achieved / probe
     log(f"[bench] FP64 FMA probe: {probe:.2f} TFLOP/s attainable; permanent kernel at {100 * achieved / probe:.1f}% "
         f"of it (instruction ceiling of Glynn at n={n}: {100 * (8 * n - 4) / (2 * (6 * n - 4)):.1f}%)")
+    mix = fp64_mix_probe()
+    res["perm"]["fp64_matrix_core_probe"] = mix
+    log(f"[bench] FP64 matrix cores: alone {mix['mfma']['mfma_tflops']:.1f} TFLOP/s; vector FMA alone "
+        f"{mix['valu']['valu_tflops']:.1f}; both in one loop {mix['both']['mfma_tflops']:.1f} + "
+        f"{mix['both']['valu_tflops']:.1f} = {mix['combined_tflops']:.1f} TFLOP/s")
     if not args.no_cpu:
         import oracle
         th = cpu_threads()
@@ -232,7 +290,7 @@ def bench_perm(args, res):
             if dt >= args.cpu_budget or k >= len(mats):
                 break
             k = min(len(mats), k * 2)
-        res["perm_cpu"] = {"value": k / dt, "unit": "perms/s", "cores": th, "kind": "port",
+        res["perm_cpu"] = {"value": k / dt, "unit": "perms/s", "cores": th, "kind": "port", "cpu_model": cpu_model(),
                            "sample": f"{k} of the {len(mats)} order-{n} matrices, oracle chunked-double Glynn "
                                      f"(oracle/pcvl_oracle.c), {th} POSIX threads, {dt:.2f} s wall"}
         log(f"[bench] perm CPU: {k / dt:.1f} perms/s on {th} threads")
@@ -281,38 +339,50 @@ def bench_slos(args, res):
     out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
     reps = 5
     per_kernel = {}
-    # 1 per state, 2 incremental runs, 3 per state with 32-bit ranks, 4 / 5 per
-    # state in two phases (4 / 8 parent loads in flight), 6 / 7 successor runs
-    # of 3 / 2 children per thread with a transposed gather (compact / full slot
-    # lists), 0 the library's own pick (the headline number)
-    for kernel in (1, 2, 3, 4, 5, 6, 7, 0):
+    # 1 per state, 4 per state in two phases, 6 successor runs of 3 children
+    # per thread with a transposed gather, 8 prefix/suffix blocks, 0 the
+    # library's own pick (the headline number)
+    for kernel in (1, 4, 6, 8, 0):
         plan.set_kernel(kernel)
         plan.run(dU, PROBS, out)
         with Timer() as t:
             for _ in range(reps):
                 plan.run(dU, PROBS, out)
         per_kernel[kernel] = t.ms / reps
+    # suffix width of the block kernel
+    default_w = plan.block_width
+    per_width = {}
+    plan.set_kernel(8)
+    for w in args.slos_widths:
+        plan.set_block_width(w)
+        plan.run(dU, PROBS, out)
+        with Timer() as t:
+            for _ in range(reps):
+                plan.run(dU, PROBS, out)
+        per_width[w] = t.ms / reps
+    plan.set_block_width(default_w)
+    plan.set_kernel(0)
+    plan.run(dU, PROBS, out)
     ms = per_kernel[0]
     total = float(out.sum().item())
     plan.close()
     gbytes = slos_bytes(m, n) / 1e9
     res["slos_m24_n10"] = {"ms": ms, "outputs": plan.count, "algorithmic_GB": gbytes,
                            "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,
-                           "sum_probs": total, "ms_by_kernel": {"per_state": per_kernel[1], "incremental": per_kernel[2],
-                                                 "per_state_32bit": per_kernel[3],
-                                                 "two_phase": per_kernel[4],
-                                                 "two_phase_8_loads": per_kernel[5],
-                                                 "successor_run3_compact": per_kernel[6],
-                                                 "successor_run2": per_kernel[7]}}
+                           "sum_probs": total, "block_width": default_w,
+                           "ms_by_kernel": {"per_state": per_kernel[1], "two_phase": per_kernel[4],
+                                            "successor_run3": per_kernel[6], "prefix_suffix_blocks": per_kernel[8]},
+                           "ms_by_block_width": {str(w): v for w, v in per_width.items()}}
     log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}; "
-        f"by kernel " + ", ".join(f"{k}: {v:.3f}" for k, v in per_kernel.items()))
+        f"by kernel " + ", ".join(f"{k}: {v:.3f}" for k, v in per_kernel.items()) +
+        "; block kernel by width " + ", ".join(f"{w}: {v:.3f}" for w, v in per_width.items()))
     if not args.no_cpu:
         import oracle
         th = cpu_threads()
         t0 = time.perf_counter()
         oracle.slos_coefs(U, inp, "gather", threads=th)
         dt = time.perf_counter() - t0
-        res["slos_cpu"] = {"value": dt * 1e3, "unit": "ms per all_prob (coefficients)", "cores": th, "kind": "port",
+        res["slos_cpu"] = {"value": dt * 1e3, "unit": "ms per all_prob (coefficients)", "cores": th, "kind": "port", "cpu_model": cpu_model(),
                            "sample": f"full m=24 n=10 layer recursion, oracle gather form, {th} POSIX threads"}
         log(f"[bench] SLOS CPU: {dt:.2f} s on {th} threads")
 
@@ -366,7 +436,7 @@ def bench_slos_tree(args, res):
             oracle.slos_coefs(U, s, "gather", threads=th)
         dt = time.perf_counter() - t0
         res["slos_tree_cpu"] = {"value": dt * 1e3, "unit": "ms for all inputs (coefficients)", "cores": th,
-                                "kind": "port", "sample": f"{len(inputs)} inputs, oracle gather form, one run each"}
+                                "kind": "port", "cpu_model": cpu_model(), "sample": f"{len(inputs)} inputs, oracle gather form, one run each"}
         log(f"[bench] SLOS tree CPU: {dt * 1e3:.1f} ms on {th} threads")
 
 
@@ -437,15 +507,45 @@ def bench_gradient(args, res):
                 c = ph[b].copy(); c[k] -= 1e-6
                 _ = (loss(a) - loss(c)) / 2e-6
         dt = time.perf_counter() - t0
-        res["gradient_cpu"] = {"value": len(ph) / dt, "unit": "sets/s", "cores": 1, "kind": "port",
+        res["gradient_cpu"] = {"value": len(ph) / dt, "unit": "sets/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
                                "sample": "2 parameter sets, central differences over 132 phases with oracle "
                                          "permanents, 1 thread"}
         log(f"[bench] gradient CPU (FD): {len(ph) / dt:.2f} sets/s")
 
 
+def _tvd_curve(occ, exact, grid):
+    from perceval_amd.marginals import marginal_tvds_exact
+    return [(N, marginal_tvds_exact(occ[:N], exact)) for N in grid if N <= len(occ)]
+
+
+def _samples_to_meet(curve, bar=0.01):
+    """Smallest grid count from which on both the 1- and the 2-mode marginal
+    TVD stay <= bar (None if the last point misses)."""
+    ok = [t["tvd1_max"] <= bar and t["tvd2_max"] <= bar for _, t in curve]
+    if not ok or not ok[-1]:
+        return None
+    i = len(ok) - 1
+    while i > 0 and ok[i - 1]:
+        i -= 1
+    return curve[i][0]
+
+
+def _batch_tau(x, batches=50):
+    """Integrated autocorrelation time of a chain statistic by batch means:
+    var(batch mean) * L / var(x)."""
+    L = len(x) // batches
+    if L < 2:
+        return float("nan")
+    bm = x[:L * batches].reshape(batches, L, -1).mean(axis=1)
+    v = x[:L * batches].var(axis=0)
+    keep = v > 0
+    return float(np.mean(bm.var(axis=0, ddof=1)[keep] * L / v[keep]))
+
+
 def bench_sampling(args, res):
     import torch
-    from perceval_amd.mis import LearnedMIS, ProposalNet, marginal_tvds, occ_to_seq, seq_to_occ
+    from perceval_amd.marginals import exact_marginals, marginal_tvds_exact
+    from perceval_amd.mis import LearnedMIS, ProposalNet, occ_to_seq, seq_to_occ
     from perceval_amd.sampling import cc_samples_device
     from perceval_amd.utils.workloads import haar_unitary
     m, n, N = 60, 20, args.samples
@@ -456,8 +556,8 @@ def bench_sampling(args, res):
     with Timer() as t:
         train = cc_samples_device(dU, inp, N, seed=1)
     cc_ms = t.ms
-    log(f"[bench] plain CC: {N} samples in {cc_ms:.1f} ms = {N / (cc_ms * 1e-3):.0f} samples/s")
-    ref = cc_samples_device(dU, inp, N, seed=2)                   # independent judge set
+    cc_rate = N / (cc_ms * 1e-3)
+    log(f"[bench] plain CC: {N} samples in {cc_ms:.1f} ms = {cc_rate:.0f} samples/s")
     net = ProposalNet(m, n, hidden=256, seed=0)
     tseq = occ_to_seq(train, n)
     t0 = time.perf_counter()
@@ -471,24 +571,51 @@ def bench_sampling(args, res):
     with Timer() as t:
         out = mis.run(N, seed=5, x0_seq=x0)
     mis_ms = t.ms
+    mis_rate = N / (mis_ms * 1e-3)
     acc = out["acceptance"]
+    # a longer chain for the equal-TVD comparison (same seed: its first N
+    # samples are the timed ones)
+    Nl = max(N, args.mis_judge_samples)
+    long = mis.run(Nl, seed=5, x0_seq=x0) if Nl > N else out
     mis.close()
-    occ_mis = seq_to_occ(out["seq"], m).cpu().numpy()
-    tv = marginal_tvds(occ_mis, ref.cpu().numpy())
-    tv_cc = marginal_tvds(train.cpu().numpy(), ref.cpu().numpy())  # noise floor of two independent CC sets
-    # A chain that (almost) never moves reports a high rate but samples nothing
-    # useful; the speed-up only counts when the marginals match to 0.01.
-    meets = bool(acc >= 0.01 and tv["tvd1_max"] <= 0.01)
+    occ_mis = seq_to_occ(long["seq"], m).cpu().numpy()
+    occ_cc = train.cpu().numpy()
+    # judge: the exact 1- and 2-mode marginals (200 seeded pairs), so the
+    # TVDs carry the sampler's error only
+    ex = exact_marginals(U, inp, pairs=200, seed=0)
+    tv = marginal_tvds_exact(occ_mis[:N], ex)
+    tv_cc = marginal_tvds_exact(occ_cc, ex)
+    grid = sorted({int(round(10000 * 2 ** (i / 2))) for i in range(0, 20)} | {N, Nl})
+    curve_mis = _tvd_curve(occ_mis, ex, grid)
+    curve_cc = _tvd_curve(occ_cc, ex, grid)
+    n_mis, n_cc = _samples_to_meet(curve_mis), _samples_to_meet(curve_cc)
+    # time for each sampler to reach both marginal TVDs <= 0.01
+    t_mis = n_mis / mis_rate if n_mis else None
+    t_cc = n_cc / cc_rate if n_cc else None
+    equal = (t_cc / t_mis) if (t_mis and t_cc) else None
+    tau = _batch_tau(occ_mis[:, :8].astype(np.float64))
+    # break-even: MIS wall time including its CC training samples and the
+    # training itself, against plain CC, at equal effective samples
+    over_s = cc_ms * 1e-3 + train_s
+    eff_mis = mis_rate / tau if tau == tau and tau > 0 else None
+    breakeven = (over_s / (1.0 / cc_rate - 1.0 / eff_mis)) if (eff_mis and eff_mis > cc_rate) else None
+    meets = bool(acc >= 0.01 and tv["tvd1_max"] <= 0.01 and tv["tvd2_max"] <= 0.01)
     res["sampling_m60_n20"] = {
-        "samples": N, "cc_samples_per_s": N / (cc_ms * 1e-3), "mis_samples_per_s": N / (mis_ms * 1e-3),
-        "mis_vs_cc": (N / (mis_ms * 1e-3)) / (N / (cc_ms * 1e-3)), "mis_acceptance": acc,
-        "mis_meets_tvd_0.01": meets, "tvd_mis_vs_cc": tv, "tvd_cc_vs_cc": tv_cc, "train_seconds": train_s,
-        "train_nll": [float(h) for h in hist]}
+        "samples": N, "cc_samples_per_s": cc_rate, "mis_samples_per_s": mis_rate,
+        "mis_vs_cc": mis_rate / cc_rate, "mis_acceptance": acc,
+        "mis_meets_tvd_0.01": meets, "tvd_mis_vs_exact": tv, "tvd_cc_vs_exact": tv_cc,
+        "samples_to_tvd_0.01": {"mis": n_mis, "cc": n_cc},
+        "mis_vs_cc_equal_tvd": equal, "mis_tau_int": tau,
+        "mis_effective_samples_per_s": eff_mis, "breakeven_samples": breakeven,
+        "train_seconds": train_s, "train_nll": [float(h) for h in hist],
+        "tvd2_max_curve": {"mis": {str(k): v["tvd2_max"] for k, v in curve_mis},
+                           "cc": {str(k): v["tvd2_max"] for k, v in curve_cc}}}
     if not meets:
-        log(f"[bench] MIS marginals do not meet TVD <= 0.01 (acceptance {acc:.3f}); the speed-up does not count")
-    log(f"[bench] MIS: {N} samples in {mis_ms:.1f} ms = {N / (mis_ms * 1e-3):.0f} samples/s "
-        f"({(N / (mis_ms * 1e-3)) / (N / (cc_ms * 1e-3)):.2f}x plain CC), acceptance {acc:.3f}, TVD {tv}, "
-        f"CC-vs-CC floor {tv_cc}")
+        log(f"[bench] MIS marginals at {N} samples miss TVD <= 0.01 (acceptance {acc:.3f}, {tv})")
+    log(f"[bench] MIS: {N} samples in {mis_ms:.1f} ms = {mis_rate:.0f} samples/s ({mis_rate / cc_rate:.2f}x plain CC "
+        f"per sample), acceptance {acc:.3f}, tau_int {tau:.2f}; TVD vs exact {tv}; CC {tv_cc}")
+    log(f"[bench] samples to both marginal TVDs <= 0.01: MIS {n_mis}, CC {n_cc}; MIS vs CC at equal TVD: "
+        f"{equal if equal is None else round(equal, 2)}x; break-even after {breakeven} samples")
     if not args.no_cpu:
         import oracle
         th = cpu_threads()
@@ -496,7 +623,7 @@ def bench_sampling(args, res):
         t0 = time.perf_counter()
         oracle.cc_samples(U, inp, k, seed=1, threads=th)
         dt = time.perf_counter() - t0
-        res["sampling_cpu"] = {"value": k / dt, "unit": "samples/s", "cores": th, "kind": "port",
+        res["sampling_cpu"] = {"value": k / dt, "unit": "samples/s", "cores": th, "kind": "port", "cpu_model": cpu_model(),
                                "sample": f"{k} samples m=60 n=20, oracle Clifford-Clifford, {th} POSIX threads"}
         log(f"[bench] CC CPU: {k / dt:.1f} samples/s on {th} threads")
 
@@ -511,11 +638,15 @@ def main():
     ap.add_argument("--chunk-log2", type=int, default=0, help="permanent lane chunk (0 = default)")
     ap.add_argument("--variant", type=int, default=0, help="permanent kernel variant (0 = by order)")
     ap.add_argument("--samples", type=int, default=100000)
+    ap.add_argument("--mis-judge-samples", type=int, default=400000,
+                    help="chain length for the equal-TVD comparison of the learned sampler")
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--cpu-budget", type=float, default=5.0, help="seconds of CPU baseline per section")
     ap.add_argument("--sections", default=",".join(SECTIONS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baselines")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-order batch")
+    ap.add_argument("--slos-widths", type=lambda v: [int(x) for x in v.split(",") if x], default=[6, 8, 10, 12],
+                    help="suffix widths of the block SLOS kernel to time")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

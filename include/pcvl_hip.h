@@ -143,6 +143,20 @@ int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, int mode, dou
 
 int pcvl_slos_plan_destroy(pcvl_slos_plan plan);
 
+/* A plan restricted to a herald mask: the masked FSArray layers of
+ * SLOSBackend._deploy (perceval/backends/_slos.py:161-164; the mask of
+ * set_mask, perceval/backends/_abstract_backends.py:103-137).  h_cond holds
+ * one condition per mode: -1 for any count, c >= 0 for exactly c photons.
+ * Intermediate layers hold only the states that can still match (herald
+ * counts at most c, and the free modes at most n - sum c photons), so the
+ * states outside the mask are never computed.  execute writes the matching
+ * output states only, in FSArray order (out_count of pcvl_slos_plan_info:
+ * C(m_free + n - sum c - 1, n - sum c), or 0 when the heralds need more than
+ * n photons).  Runs the block kernel (8); needs m >= 2, at least two free
+ * modes and 1..30 photons.  Summation runs in herald-modes-first order, so
+ * results match the unmasked plan to rounding, not to the bit. */
+int pcvl_slos_plan_create_masked(int m, const int32_t *h_in_occ, const int32_t *h_cond, pcvl_slos_plan *out);
+
 /* Several input states in one plan: the _Path tree of
  * SLOSBackend.preprocess([inputs]) (perceval/backends/_slos.py:37-86,170-185),
  * so layers on a shared photon-addition prefix are computed once.  h_inputs
@@ -159,15 +173,28 @@ int pcvl_slos_tree_execute(pcvl_slos_tree plan, const double *d_U, int mode, voi
                            size_t work_bytes, hipStream_t stream);
 int pcvl_slos_tree_destroy(pcvl_slos_tree plan);
 
-/* Kernel used by execute: 0 picks (4 when every parent layer is below 2^26
- * states, else 1), 1 one thread per output state, 2 incremental runs of
- * consecutive output states (layers below 2^32 states and 64 KiB of LDS
- * only), 3 one thread per output state with 32-bit ranks (layers below 2^32
- * states), 4 one thread per output state with the walk and the parent gather
- * in two phases (parent layers below 2^26 states), 5 as 4 with eight parent
- * loads in flight instead of four.  Kernels not applicable to the plan give
- * -1.  For comparisons. */
+/* Layer kernel used by execute (the same parents in the same summation
+ * order, so every kernel gives bit-identical results):
+ *   0  auto: 8 where its layout was built (m >= 2, 1..30 photons, every layer
+ *      below 2^32 states), else 6 where it fits, else 4 where it fits, else 1;
+ *   1  one thread per output state, 64-bit ranks (any plan);
+ *   4  one thread per output state, the unranking walk and the parent gather
+ *      in two phases (parent layers below 2^26 states, LDS below 64 KiB);
+ *   6  runs of three consecutive states per thread, successor steps instead
+ *      of walks, gather transposed through LDS (as 4, plus LDS below 64 KiB);
+ *   8  prefix/suffix blocks: one scalar walk per block of states sharing
+ *      their first m - w modes, the last w modes from a table per suffix
+ *      photon count (pcvl_slos_plan_set_block_width).
+ * Kernels not applicable to the plan give -1.  For comparisons. */
 int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel);
+
+/* Suffix width w of kernel 8 (0: the default, the widest w <= 10 below m
+ * whose tables stay under 16 MiB; else 1 <= w <= min(m - 1, 16)).  Rebuilds
+ * the plan's block tables (synchronises the device).  A tuning knob: the
+ * results do not depend on w.  No counterpart in the reference. */
+int pcvl_slos_plan_set_block_width(pcvl_slos_plan plan, int w);
+/* Current suffix width of kernel 8, 0 when the plan has no block layout. */
+int pcvl_slos_plan_block_width(pcvl_slos_plan plan);
 
 /* ---------------------------------------------------------------------------
  * Exact boson sampling -- replaces xq.Clifford2017.samples
@@ -344,6 +371,14 @@ int pcvl_gurvits_batch(const double *d_mats, const int32_t *h_n, const int64_t *
  * chains per lane); *flops receives the flops issued (FMA = 2).  d_out holds
  * blocks*256 doubles.  Timing it gives the attainable vector FP64 rate. */
 int pcvl_probe_fp64_fma(int blocks, int64_t iters, double *d_out, int64_t *flops, hipStream_t stream);
+
+/* Measurement helper, no counterpart in the reference: whether the FP64
+ * matrix cores run beside the vector ALU.  blocks x 256 threads, iters loop
+ * trips; mode 0: four v_mfma_f64_16x16x4f64 accumulators per wave, 1: 16
+ * v_fma_f64 chains per lane, 2: both in the same loop.  d_out: blocks*256
+ * doubles; *flops_mfma / *flops_valu receive each unit's flop count. */
+int pcvl_probe_fp64_mix(int blocks, int64_t iters, int mode, double *d_out, int64_t *flops_mfma,
+                        int64_t *flops_valu, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# PCVL_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of `make -C oracle asan-test`)
+_LIB_PATH = os.environ.get("PCVL_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)

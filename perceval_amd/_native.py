@@ -51,10 +51,13 @@ SIGNATURES = {
     "pcvl_fock_mask_select": (c_int, [c_int, c_int, P(c_i32), c_int, P(c_i32), c_void_p, c_void_p, c_void_p]),
     "pcvl_gather_f64": (c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_void_p]),
     "pcvl_slos_plan_create": (c_int, [c_int, P(c_i32), P(c_void_p)]),
+    "pcvl_slos_plan_create_masked": (c_int, [c_int, P(c_i32), P(c_i32), P(c_void_p)]),
     "pcvl_slos_plan_info": (c_int, [c_void_p, P(c_int), P(c_i64), P(c_size_t), P(c_i32)]),
     "pcvl_slos_plan_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pcvl_slos_plan_destroy": (c_int, [c_void_p]),
     "pcvl_slos_plan_set_kernel": (c_int, [c_void_p, c_int]),
+    "pcvl_slos_plan_set_block_width": (c_int, [c_void_p, c_int]),
+    "pcvl_slos_plan_block_width": (c_int, [c_void_p]),
     "pcvl_slos_tree_create": (c_int, [c_int, P(c_i32), c_int, ctypes.POINTER(c_void_p)]),
     "pcvl_slos_tree_info": (c_int, [c_void_p, P(c_int), P(c_i64), P(ctypes.c_size_t), P(c_i64)]),
     "pcvl_slos_tree_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
@@ -86,6 +89,8 @@ SIGNATURES = {
     "pcvl_mis_occ_to_seq": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
     "pcvl_mis_gather": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p]),
     "pcvl_probe_fp64_fma": (c_int, [c_int, c_i64, c_void_p, ctypes.POINTER(c_i64), c_void_p]),
+    "pcvl_probe_fp64_mix": (c_int, [c_int, c_i64, c_int, c_void_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
+                                    c_void_p]),
     "pcvl_gurvits_batch": (c_int, [c_void_p, P(c_i32), P(c_i64), c_i64, c_i64, c_u64, c_u64, c_void_p, c_void_p,
                                    c_void_p]),
     "pcvl_mesh_adjoint": (c_int, [P(c_i32), P(c_i32), P(c_i32), P(c_double), c_int, c_int, c_void_p, c_int, c_int,

@@ -141,44 +141,56 @@ def tune_permanent_mixed(orders, reps: int = 3, use_cache: bool = True) -> dict:
     return res
 
 
-SLOS_KERNELS = (1, 2, 3, 4, 5, 6, 7)
+SLOS_KERNELS = (1, 4, 6, 8)
+SLOS_BLOCK_WIDTHS = (6, 8, 10, 12)
 
 
 def tune_slos(in_occ, reps: int = 3, use_cache: bool = True) -> dict:
-    """Fastest SLOS layer kernel for an input state (SlosPlan.set_kernel ids),
-    timed on a Haar unitary.  Returns {"kernel", "ms", "trials": [...]}."""
+    """Fastest SLOS layer kernel for an input state (SlosPlan.set_kernel ids;
+    kernel 8 at each suffix width of SLOS_BLOCK_WIDTHS below m), timed on a
+    Haar unitary.  Returns {"kernel", "width", "ms", "trials": [...]}."""
     import torch
     from .slos import PROBS, SlosPlan
     from .utils.workloads import haar_unitary
     in_occ = [int(v) for v in in_occ]
     cache = _load() if use_cache else {}
     key = f"{torch.cuda.get_device_name(0)}|slos|{in_occ}"
-    if key in cache:
+    if key in cache and "width" in cache[key]:
         return cache[key]
     plan = SlosPlan(in_occ)
     dU = torch.from_numpy(haar_unitary(plan.m, seed=1)).cuda()
     out = torch.empty(max(1, plan.count), dtype=torch.float64, device="cuda")
     trials = []
+
+    def timed():
+        plan.run(dU, PROBS, out)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            plan.run(dU, PROBS, out)
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
     try:
         for k in SLOS_KERNELS:
             try:
                 plan.set_kernel(k)
             except ValueError:
                 continue   # not applicable to this plan's layer sizes
-            plan.run(dU, PROBS, out)
-            torch.cuda.synchronize()
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record()
-            for _ in range(reps):
-                plan.run(dU, PROBS, out)
-            b.record()
-            torch.cuda.synchronize()
-            trials.append({"kernel": k, "ms": a.elapsed_time(b) / reps})
+            widths = [0]
+            if k == 8:
+                widths = [w for w in SLOS_BLOCK_WIDTHS if w < plan.m] or [0]
+            for w in widths:
+                if k == 8:
+                    plan.set_block_width(w)
+                trials.append({"kernel": k, "width": plan.block_width if k == 8 else 0, "ms": timed()})
     finally:
         plan.close()
     best = min(trials, key=lambda t: t["ms"])
-    res = {"kernel": best["kernel"], "ms": best["ms"], "trials": trials}
+    res = {"kernel": best["kernel"], "width": best["width"], "ms": best["ms"], "trials": trials}
     cache[key] = res
     if use_cache:
         _save(cache)
@@ -189,5 +201,8 @@ def tuned_slos_plan(in_occ):
     """A SlosPlan running the layer kernel the tuner found fastest."""
     from .slos import SlosPlan
     plan = SlosPlan(in_occ)
-    plan.set_kernel(tune_slos(in_occ)["kernel"])
+    best = tune_slos(in_occ)
+    plan.set_kernel(best["kernel"])
+    if best["kernel"] == 8:
+        plan.set_block_width(best["width"])
     return plan

@@ -42,8 +42,11 @@ class NaiveBackend(AStrongSimulationBackend):
         return np.ascontiguousarray(self._umat[np.ix_(rows, cols)])
 
     def _compute_permanent(self, M) -> complex:
-        from ..permanent import permanent
-        return permanent(M)
+        # a plan and device buffers kept per order: no allocation per call
+        if getattr(self, "_single", None) is None:
+            from ..permanent import SinglePermanents
+            self._single = SinglePermanents()
+        return self._single(M)
 
     def prob_amplitude(self, output_state) -> complex:
         output_state = FockState(output_state)

@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from ..slos import COEFS, PROBS, SlosPlan, SlosTree, fock_mask_select, gather_f64
+from ..slos import COEFS, PROBS, SlosPlan, SlosTree, fock_mask_select, gather_f64, mask_conditions
 from ..utils.states import BSDistribution, FockState, StateVector, fock_space
 from ._abstract_backends import AStrongSimulationBackend, _as_circuit
 
@@ -60,6 +60,7 @@ class SLOSBackend(AStrongSimulationBackend):
         self._fsms: list = [[]]
         self._state_mapping: dict = {}
         self._results: dict = {}
+        self._plans_masked: dict = {}
         self.clear_iterator_cache()
 
     def set_circuit(self, circuit):
@@ -145,13 +146,41 @@ class SLOSBackend(AStrongSimulationBackend):
         assert 0 <= idx < len(coefs)
         return complex(coefs[idx] * math.sqrt(output_state.prodnfact() / self._input_state.prodnfact()))
 
+    def _herald_conditions(self, istate):
+        """Per-mode conditions of a single exact-count mask (the heralds), or
+        None when the mask needs the general path (several masks, at-least
+        modes, fewer than two free modes, no photon)."""
+        if self._mask is None or len(self._masks_str) != 1 or self._no_limit_modes:
+            return None
+        cond = mask_conditions(self._masks_str, istate.m)[0]
+        if (cond < 0).sum() < 2 or istate.n < 1 or istate.n > 30 or (cond < 0).all():
+            return None
+        if self._mask_n is not None and self._mask_n != istate.n:
+            return None
+        return cond
+
+    def _masked_plan(self, istate, cond):
+        key = ("masked_plan", istate, tuple(int(v) for v in cond))
+        if key not in self._plans_masked:
+            self._plans_masked[key] = SlosPlan(list(istate), mask_cond=cond)
+        return self._plans_masked[key]
+
     def all_prob(self, input_state=None):
         """SLOS specific signature, to enhance optimization in some computations"""
         if input_state is not None:
             self.set_input_state(input_state)
         if self._mask is None:
             return self._compute(self._input_state, PROBS).tolist()
-        # masked: select on the GPU and copy back only the kept probabilities
+        cond = self._herald_conditions(self._input_state)
+        if cond is not None:
+            # heralds: layers restricted to the states that can still match,
+            # the kept outputs in FSArray order
+            key = ("masked_probs", self._input_state)
+            if key not in self._results:
+                plan = self._masked_plan(self._input_state, cond)
+                self._results[key] = plan.run(self._umat, PROBS).cpu().numpy()
+            return self._results[key].tolist()
+        # other masks: select on the GPU and copy back only the kept probabilities
         idx = self._masked_indices_device(self._input_state)
         return gather_f64(self.all_prob_tensor(), idx).cpu().numpy().tolist()
 

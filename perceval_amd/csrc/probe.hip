@@ -24,9 +24,54 @@ __global__ void __launch_bounds__(256) probe_fma_kernel(int64_t iters, double a,
     out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// FP64 matrix cores beside the vector ALU: each wave runs NM independent
+// v_mfma_f64_16x16x4f64 accumulators (2,048 flops per instruction) and NF
+// independent v_fma_f64 chains per lane in the same loop.  Timing NM-only,
+// NF-only and both shows whether the two units overlap.
+typedef double probe_d4 __attribute__((ext_vector_type(4)));
+
+template <int NM, int NF>
+__global__ void __launch_bounds__(256) probe_mix_kernel(int64_t iters, double a, double b, double *__restrict__ out)
+{
+    probe_d4 acc[NM > 0 ? NM : 1];
+    double c[NF > 0 ? NF : 1];
+#pragma unroll
+    for (int k = 0; k < NM; k++) acc[k] = probe_d4{(double)k, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NF; k++) c[k] = (double)(threadIdx.x + k);
+    // operands near 1 and 1e-9 keep every accumulator finite
+    const double ma = 1e-9 * (double)(threadIdx.x & 7), mb = 1e-9;
+    for (int64_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int k = 0; k < NM; k++) acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(ma, mb, acc[k], 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < NF; k++) c[k] = fma(c[k], a, b);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; k++) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+#pragma unroll
+    for (int k = 0; k < NF; k++) s += c[k];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 } // namespace pcvl
 
 using namespace pcvl;
+
+extern "C" int pcvl_probe_fp64_mix(int blocks, int64_t iters, int mode, double *d_out, int64_t *flops_mfma,
+                                   int64_t *flops_valu, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(blocks > 0 && iters > 0 && d_out && flops_mfma && flops_valu, "bad probe arguments");
+    PCVL_ARG_CHECK(mode >= 0 && mode <= 2, "mode must be 0 (matrix cores), 1 (vector FMA) or 2 (both)");
+    const int nm = mode == 1 ? 0 : 4, nf = mode == 0 ? 0 : 16;
+    auto kp = mode == 0 ? probe_mix_kernel<4, 0> : mode == 1 ? probe_mix_kernel<0, 16> : probe_mix_kernel<4, 16>;
+    hipLaunchKernelGGL(kp, dim3(blocks), dim3(256), 0, stream, iters, 0.999999999, 1e-9, d_out);
+    PCVL_HIP_CHECK(hipGetLastError());
+    *flops_mfma = (int64_t)blocks * 4 * nm * iters * 2048;   // 4 waves per block
+    *flops_valu = (int64_t)blocks * 256 * nf * iters * 2;
+    return OK;
+}
 
 extern "C" int pcvl_probe_fp64_fma(int blocks, int64_t iters, double *d_out, int64_t *flops, hipStream_t stream)
 {

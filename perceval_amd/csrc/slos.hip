@@ -20,6 +20,7 @@
 // and gathers all its parents.  The last layer can emit probabilities
 // |c|^2 * prod s! / prod t! directly (fused epilogue).
 #include "common.h"
+#include "slos_block.h"
 #include "../../include/pcvl_hip.h"
 
 #include <algorithm>
@@ -161,75 +162,6 @@ __global__ void __launch_bounds__(SLOS_THREADS) slos_layer_kernel(
     }
 }
 
-
-// Per-state kernel for layers below 2^32 states: the same walk with 32-bit
-// ranks and tables, the unitary column and the factorials in LDS, and the
-// walk ended once no photons remain (later modes are empty).
-template <int MODE>
-__global__ void __launch_bounds__(SLOS_THREADS) slos_layer32_kernel(
-    const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
-    int k, uint32_t nchild, const uint64_t *__restrict__ gT, const uint64_t *__restrict__ gE, int w,
-    double inv_prod_in)
-{
-    extern __shared__ uint64_t s_dyn32[];
-    // layout: u (m cd) | fact (w double) | T (m*w u32) | E (m*w u32)
-    cd *su = reinterpret_cast<cd *>(s_dyn32);
-    double *sf = reinterpret_cast<double *>(su + m);
-    uint32_t *sT = reinterpret_cast<uint32_t *>(sf + w);
-    uint32_t *sE = sT + (size_t)m * w;
-    for (int e = threadIdx.x; e < m * w; e += blockDim.x) {
-        sT[e] = (uint32_t)gT[e];
-        sE[e] = (uint32_t)gE[e];
-    }
-    for (int e = threadIdx.x; e < m; e += blockDim.x) su[e] = U[(size_t)e * m + mk];
-    if (threadIdx.x == 0) {
-        double f = 1.0;
-        for (int v = 0; v < w; v++) {
-            if (v > 1) f *= (double)v;
-            sf[v] = f;
-        }
-    }
-    __syncthreads();
-
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i64 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i64 < nchild; i64 += stride) {
-        const uint32_t idx = (uint32_t)i64;
-        uint32_t rem = idx, pe = 0;
-        int R = k;
-        cd acc = {0.0, 0.0};
-        double pfact = 1.0;
-        for (int q = 0; q < m && R > 0; q++) {
-            int Rn = 0;
-            if (q + 1 < m) {
-                const uint32_t *tq = sT + q * w;
-                Rn = R;
-                uint32_t tv = tq[Rn];
-                while (Rn > 0 && tv > rem) { --Rn; tv = tq[Rn]; }   // T_q(0) = 0 ends the walk
-                rem -= tv;
-            }
-            const int sq = R - Rn;
-            if (sq > 0) {
-                const uint32_t pidx = idx - pe;
-                if (pidx < nparent) {   // always true for a consistent rank; guards the read
-                    const cd p = parent[pidx];
-                    const cd u = su[q];
-                    acc.r = fma(p.r, u.r, acc.r);
-                    acc.r = fma(-p.i, u.i, acc.r);
-                    acc.i = fma(p.r, u.i, acc.i);
-                    acc.i = fma(p.i, u.r, acc.i);
-                }
-                if (MODE == 1) pfact *= sf[sq];
-            }
-            if (q + 1 < m) pe += sE[q * w + Rn];
-            R = Rn;
-        }
-        if (MODE == 0) {
-            reinterpret_cast<cd *>(out)[idx] = acc;
-        } else {
-            reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * pfact * inv_prod_in;
-        }
-    }
-}
 
 // Per-state kernel in two phases (parent layers below 2^26 states): the walk first
 // records, per occupied mode, the parent rank and the mode in per-thread LDS
@@ -412,137 +344,6 @@ __device__ __forceinline__ void skip_walk(uint32_t first, int m, int k, const ui
     }
 }
 
-template <int MODE, int RUN, bool PFI, int G>
-__global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ_kernel(
-    const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
-    int k, uint32_t nchild, const uint64_t *__restrict__ gP, int wp, double inv_prod_in, int sentinel)
-{
-    constexpr uint32_t TILE = (uint32_t)RUN * SLOS7_THREADS;
-    constexpr uint32_t TPAD = TILE + TILE / 32;
-    using pf_t = typename std::conditional<PFI, uint32_t, double>::type;
-    extern __shared__ uint64_t s_dyn7[];
-    // layout: P (m*wp u64) | u (m cd) | fact (wp double) | pf (TILE pf_t, probabilities only) |
-    //         slots (k*TPAD u32) | cnt ((k+1)*THREADS u8) | dep (TILE u8)
-    uint64_t *sP = s_dyn7;
-    uint32_t *sCT = reinterpret_cast<uint32_t *>(sP + (size_t)m * wp);
-    uint32_t *sCE = sCT + (size_t)(wp - 1) * (m + 4);
-    cd *su = reinterpret_cast<cd *>(sCE + (size_t)(wp - 1) * (m + 4));
-    double *sf = reinterpret_cast<double *>(su + m);
-    pf_t *spf = reinterpret_cast<pf_t *>(sf + wp);
-    uint32_t *slot = reinterpret_cast<uint32_t *>(spf + (MODE == 1 ? TILE : 0));
-    uint8_t *cnt = reinterpret_cast<uint8_t *>(slot + (size_t)k * TPAD);
-    uint8_t *dep = cnt + (size_t)(k + 1) * SLOS7_THREADS;
-    for (int e = threadIdx.x; e < m * wp; e += blockDim.x) sP[e] = gP[e];
-    {
-        const uint32_t *gC = reinterpret_cast<const uint32_t *>(gP + (size_t)m * wp);
-        for (int e = threadIdx.x; e < 2 * (wp - 1) * (m + 4); e += blockDim.x) sCT[e] = gC[e];
-    }
-    for (int e = threadIdx.x; e < m; e += blockDim.x) su[e] = U[(size_t)e * m + mk];
-    if (threadIdx.x == 0) {
-        double f = 1.0;
-        for (int v = 0; v < wp; v++) {
-            if (v > 1) f *= (double)v;
-            sf[v] = f;
-        }
-        if (sentinel && blockIdx.x == 0) reinterpret_cast<cd *>(out)[nchild] = cd{0.0, 0.0};
-    }
-    __syncthreads();
-    const int tid = threadIdx.x;
-    const uint32_t sent_word = nparent << 6;
-#define CNT(d) cnt[(size_t)(d) * SLOS7_THREADS + tid]
-    const XcdTiles xt = xcd_tiles((uint32_t)(((uint64_t)nchild + TILE - 1) / TILE));
-    for (uint32_t tile = xt.first; tile < xt.end; tile += xt.step) {
-        const uint32_t blk0 = tile * TILE;
-        // phase 1: this thread's RUN consecutive children
-        const uint32_t c0 = (uint32_t)tid * RUN;
-        const uint32_t first = blk0 + c0;
-        const uint32_t p0 = succ_pad(c0);
-        int depth = 0;
-        pf_t pfact = (pf_t)1;
-        if (first < nchild) {
-            skip_walk(first, m, k, sP, wp, sCT, sCE, [&](int q, int sq, uint32_t pe) {
-                slot[depth * TPAD + p0] = ((first - pe) << 6) | (uint32_t)q;
-                CNT(depth) = (uint8_t)sq;
-                depth++;
-                if (MODE == 1) pfact *= (pf_t)sf[sq];
-            });
-        }
-        dep[c0] = first < nchild ? (uint8_t)depth : (uint8_t)0;
-        if (MODE == 1) spf[c0] = pfact;
-        for (int r = 1; r < RUN; r++) {
-            const uint32_t c = c0 + (uint32_t)r, idx = first + (uint32_t)r;
-            if (idx >= nchild) { dep[c] = 0; continue; }
-            // idx - 1 is not the last state, so after dropping a mode-(m-1)
-            // entry the list still has an entry below m-1.
-            const uint32_t pp = succ_pad(c - 1), pc = succ_pad(c);
-            for (int d = 0; d < depth; d++) slot[d * TPAD + pc] = slot[d * TPAD + pp] + 64u;
-            int top = depth - 1;
-            uint32_t ts = slot[top * TPAD + pc];
-            int tail = 0;
-            if ((int)(ts & 63u) == m - 1) {
-                tail = CNT(top);
-                top--;
-                ts = slot[top * TPAD + pc];
-            }
-            const int i = (int)(ts & 63u);
-            const int tc = CNT(top);
-            const uint32_t pe_top = idx - (ts >> 6);
-            if (tc == 1) {
-                depth = top;
-            } else {
-                CNT(top) = (uint8_t)(tc - 1);
-                depth = top + 1;
-            }
-            const uint32_t pe_new = pe_top + (uint32_t)(sP[i * wp + tail + 1] >> 32);
-            slot[depth * TPAD + pc] = ((idx - pe_new) << 6) | (uint32_t)(i + 1);
-            CNT(depth) = (uint8_t)(tail + 1);
-            depth++;
-            dep[c] = (uint8_t)depth;
-            if (MODE == 1) {
-                pf_t pf = (pf_t)1;
-                for (int d = 0; d < depth; d++) pf *= (pf_t)sf[CNT(d)];
-                spf[c] = pf;
-            }
-        }
-        __syncthreads();
-        // phase 2: lane-consecutive children
-        for (int r = 0; r < RUN; r++) {
-            const uint32_t c = (uint32_t)r * SLOS7_THREADS + (uint32_t)tid;
-            const uint32_t idx = blk0 + c;
-            if (idx >= nchild) break;
-            const int dd = dep[c];
-            const uint32_t pc = succ_pad(c);
-            cd acc = {0.0, 0.0};
-            for (int d0 = 0; d0 < dd; d0 += G) {
-                cd pv[G];
-                int qv[G];
-#pragma unroll
-                for (int t = 0; t < G; t++) {
-                    const int d = d0 + t;
-                    const uint32_t w32 = d < dd ? slot[d * TPAD + pc] : sent_word;
-                    pv[t] = parent[min(w32 >> 6, nparent)];
-                    qv[t] = (int)(w32 & 63u);
-                }
-#pragma unroll
-                for (int t = 0; t < G; t++) {
-                    const cd u = su[qv[t]];
-                    acc.r = fma(pv[t].r, u.r, acc.r);
-                    acc.r = fma(-pv[t].i, u.i, acc.r);
-                    acc.i = fma(pv[t].r, u.i, acc.i);
-                    acc.i = fma(pv[t].i, u.r, acc.i);
-                }
-            }
-            if (MODE == 0) {
-                reinterpret_cast<cd *>(out)[idx] = acc;
-            } else {
-                reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * (double)spf[c] * inv_prod_in;
-            }
-        }
-        __syncthreads();
-    }
-#undef CNT
-}
-
 // Successor kernel, compact form.  As slos_layer_succ_kernel, but a thread
 // stores the full slot list only for its first child (slot0); child s of
 // the run is the first child's first keep_s entries (parent ranks + s) plus
@@ -694,138 +495,6 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ2_kernel(
 #undef OWN
 #undef SLOT0
 #undef CNT
-}
-
-// ------------------------------------------------------------------------
-// Incremental layer kernel: each thread walks a run of RUN consecutive
-// children.  Only the first child of a run is unranked; the next child in
-// descending lexicographic order differs only at the end of its occupied-mode
-// list: with i the last occupied mode below m-1, one photon moves from mode
-// i to mode i+1 and the photons above i gather there.  The occupied modes are
-// kept as a per-thread stack (mode, count, PE = sum_{q < mode} E_q) in LDS,
-// so the successor is O(1) and the gather loops over occupied modes only:
-// parent(c - e_j) = idx - PE_j, and PE_{i+1} = PE_i + E_i(photons above i).
-// Results are staged in LDS and written with coalesced stores.
-// Requires every layer below 2^32 states (32-bit ranks).
-constexpr int SLOS2_THREADS = 128;
-constexpr int SLOS2_RUN = 8;
-
-struct StackEntry {
-    uint32_t pe;
-    uint16_t mode;
-    uint16_t count;
-};
-
-template <int MODE>
-__global__ void __launch_bounds__(SLOS2_THREADS) slos_layer_run_kernel(
-    const cd *__restrict__ parent, uint32_t nparent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk,
-    int k, uint32_t nchild, const uint64_t *__restrict__ gT, const uint64_t *__restrict__ gE, int w,
-    double inv_prod_in)
-{
-    extern __shared__ uint64_t s_dyn[];
-    // layout: T (m*w u64) | E (m*w u32) | u (m cd) | stack (k * THREADS entries) | staging (RUN*THREADS)
-    uint64_t *sT = s_dyn;
-    uint32_t *sE = reinterpret_cast<uint32_t *>(sT + (size_t)m * w);
-    cd *su = reinterpret_cast<cd *>(s_dyn + (size_t)m * w + ((size_t)m * w + 1) / 2);
-    StackEntry *stk = reinterpret_cast<StackEntry *>(su + m);
-    double *stage = reinterpret_cast<double *>(stk + (size_t)(k + 1) * SLOS2_THREADS);
-    const int tid = threadIdx.x;
-    for (int e = tid; e < m * w; e += SLOS2_THREADS) {
-        sT[e] = gT[e];
-        sE[e] = (uint32_t)gE[e];
-    }
-    for (int e = tid; e < m; e += SLOS2_THREADS) su[e] = U[(size_t)e * m + mk];
-    __syncthreads();
-#define STK(d) stk[(size_t)(d) * SLOS2_THREADS + tid]
-    const uint32_t per_block = (uint32_t)SLOS2_RUN * SLOS2_THREADS;
-    for (uint64_t blk0 = (uint64_t)blockIdx.x * per_block; blk0 < nchild; blk0 += (uint64_t)gridDim.x * per_block) {
-        const uint32_t first = (uint32_t)blk0 + (uint32_t)tid * SLOS2_RUN;
-        int depth = 0;
-        double pfact = 1.0;
-        if (first < nchild) {
-            // unrank the first child, pushing occupied modes
-            uint64_t rem = first;
-            uint32_t pe = 0;
-            int R = k;
-            for (int q = 0; q < m && R > 0; q++) {
-                int Rn = 0;
-                if (q + 1 < m) {
-                    const uint64_t *tq = sT + q * w;
-                    Rn = R;
-                    uint64_t tv = tq[Rn];
-                    while (Rn > 0 && tv > rem) { --Rn; tv = tq[Rn]; }
-                    rem -= tv;
-                }
-                const int sq = R - Rn;
-                if (sq > 0) {
-                    STK(depth) = StackEntry{pe, (uint16_t)q, (uint16_t)sq};
-                    depth++;
-                    if (MODE == 1) pfact *= fact_d(sq);
-                }
-                if (q + 1 < m) pe += sE[q * w + Rn];
-                R = Rn;
-            }
-        }
-        for (int r = 0; r < SLOS2_RUN; r++) {
-            const uint32_t idx = first + (uint32_t)r;
-            if (idx < nchild) {
-                cd acc = {0.0, 0.0};
-                for (int d = depth - 1; d >= 0; d--) {   // increasing parent rank, as the oracle
-                    const StackEntry en = STK(d);
-                    const uint32_t pidx = idx - en.pe;
-                    if (pidx < nparent) {
-                        const cd p = parent[pidx];
-                        const cd u = su[en.mode];
-                        acc.r = fma(p.r, u.r, acc.r);
-                        acc.r = fma(-p.i, u.i, acc.r);
-                        acc.i = fma(p.r, u.i, acc.i);
-                        acc.i = fma(p.i, u.r, acc.i);
-                    }
-                }
-                if (MODE == 0) {
-                    stage[(size_t)(tid * SLOS2_RUN + r) * 2] = acc.r;
-                    stage[(size_t)(tid * SLOS2_RUN + r) * 2 + 1] = acc.i;
-                } else {
-                    stage[tid * SLOS2_RUN + r] = (acc.r * acc.r + acc.i * acc.i) * pfact * inv_prod_in;
-                }
-                // successor in descending lexicographic order
-                if (r + 1 < SLOS2_RUN && idx + 1 < nchild && depth > 0) {
-                    StackEntry top = STK(depth - 1);
-                    int tail = 0;
-                    if (top.mode == m - 1) {
-                        tail = top.count;
-                        if (MODE == 1) pfact /= fact_d(tail);
-                        depth--;
-                        top = STK(depth - 1);   // depth >= 1: idx + 1 < nchild means a successor exists
-                    }
-                    const int i = top.mode;
-                    if (MODE == 1) pfact /= (double)top.count;   // count! -> (count-1)!
-                    if (top.count == 1) {
-                        depth--;
-                    } else {
-                        top.count -= 1;
-                        STK(depth - 1) = top;
-                    }
-                    const uint32_t pe_next = top.pe + sE[i * w + tail + 1];
-                    STK(depth) = StackEntry{pe_next, (uint16_t)(i + 1), (uint16_t)(tail + 1)};
-                    depth++;
-                    if (MODE == 1) pfact *= fact_d(tail + 1);
-                }
-            }
-        }
-        __syncthreads();
-        // coalesced store of the block's results
-        const uint32_t nblk = (uint32_t)min<uint64_t>(per_block, nchild - blk0);
-        if (MODE == 0) {
-            cd *o = reinterpret_cast<cd *>(out) + blk0;
-            for (uint32_t e = tid; e < nblk; e += SLOS2_THREADS) o[e] = {stage[2 * e], stage[2 * e + 1]};
-        } else {
-            double *o = reinterpret_cast<double *>(out) + blk0;
-            for (uint32_t e = tid; e < nblk; e += SLOS2_THREADS) o[e] = stage[e];
-        }
-        __syncthreads();
-    }
-#undef STK
 }
 
 // Writes the vacuum layer: coefficient (1, 0), or probability 1.
@@ -1008,18 +677,14 @@ struct SlosPlanImpl {
     double inv_prod_in = 1.0;         // 1 / prod t!
     uint64_t max_inter = 1;           // largest intermediate layer (k < n)
     uint64_t *d_T = nullptr, *d_E = nullptr;
-    uint64_t *d_P = nullptr;          // packed table of the wave-prefix kernel (inside the d_T allocation)
+    uint64_t *d_P = nullptr;          // packed table of the successor kernel (inside the d_T allocation)
     int grid_cap = 0;
-    int kernel = 0;                   // 0 auto, 1 per state, 2 incremental runs, 3 per state 32-bit,
-                                      // 4 per state in two phases, 5 the same with eight loads in flight,
-                                      // 6 / 7 successor runs with a transposed gather
+    int kernel = 0;                   // 0 auto, 1 per state, 4 per state in two phases,
+                                      // 6 successor runs with a transposed gather, 8 prefix/suffix blocks
+    BlkLayout blk;                    // block layout (kernel 8), built when the layers allow it
+    bool masked = false;              // layers restricted to a herald mask (kernel 8 only)
+    BlkMasked mask;                   // their sub-layers
 };
-
-static size_t run_kernel_lds(int m, int w, int k)
-{
-    return sizeof(uint64_t) * ((size_t)m * w + ((size_t)m * w + 1) / 2) + sizeof(cd) * m +
-           sizeof(StackEntry) * (size_t)(k + 1) * SLOS2_THREADS + sizeof(cd) * SLOS2_RUN * SLOS2_THREADS;
-}
 
 static size_t twophase_lds(int m, int w, int k)
 {
@@ -1035,6 +700,9 @@ static bool twophase_ok(const SlosPlanImpl &p)
     return p.counts[p.n] < (1ull << 32) && twophase_lds(p.m, p.w, p.n) <= 64 * 1024;
 }
 
+// Successor kernel: compact slot lists, runs of 3 children per thread.  At
+// m=24, n=10 runs of 2 / 3 / 4 / 5 take 1.80 / 1.71 / 1.86 / 1.98 ms: longer
+// runs save unranking work but need more LDS per block, so fewer waves.
 static size_t succ2_lds(int m, int n, int k, int run, bool probs)
 {
     const size_t rows = (size_t)k + (size_t)run * (run - 1) / 2;
@@ -1044,40 +712,27 @@ static size_t succ2_lds(int m, int n, int k, int run, bool probs)
            (size_t)(k + 1) * SLOS7_THREADS + 2 * (size_t)run * SLOS7_THREADS;
 }
 
-static size_t succ_lds(int m, int n, int k, int run, bool probs)
+static bool succ_ok(const SlosPlanImpl &p)
 {
-    const size_t tile = (size_t)run * SLOS7_THREADS, tpad = tile + tile / 32;
-    const size_t pf = probs ? (k <= 12 ? sizeof(uint32_t) : sizeof(double)) * tile : 0;
-    return sizeof(uint64_t) * (size_t)m * (n + 2) + 2 * sizeof(uint32_t) * (size_t)(n + 1) * (m + 4) +
-           sizeof(cd) * m + sizeof(double) * (n + 2) + pf + sizeof(uint32_t) * (size_t)k * tpad +
-           (size_t)(k + 1) * SLOS7_THREADS + tile;
+    return twophase_ok(p) && succ2_lds(p.m, p.n, p.n, 3, true) <= 64 * 1024;
 }
 
-// Successor kernels: 6 compact slot lists, runs of 3 children per thread;
-// 7 full slot lists, runs of 2 with eight loads in flight.  At m=24, n=10 runs
-// of 2 / 3 / 4 / 5 (compact) take 1.80 / 1.71 / 1.86 / 1.98 ms: longer runs
-// save unranking work but need more LDS per block, so fewer waves.
-static size_t succ_kernel_lds(const SlosPlanImpl &p, int kernel, int k, bool probs)
-{
-    return kernel == 6 ? succ2_lds(p.m, p.n, k, 3, probs) : succ_lds(p.m, p.n, k, 2, probs);
-}
+static bool blk_ok(const SlosPlanImpl &p) { return p.blk.d_mem != nullptr; }
 
-static bool succ_ok(const SlosPlanImpl &p, int kernel)
+// (Re)builds a masked plan's block layout with suffix width w (0: default).
+static int masked_layout(SlosPlanImpl &p, int w)
 {
-    return twophase_ok(p) && succ_kernel_lds(p, kernel, p.n, true) <= 64 * 1024;
-}
-
-static bool all_layers_32bit(const SlosPlanImpl &p)
-{
-    for (int k = 0; k <= p.n; k++)
-        if (p.counts[k] >= (1ull << 32)) return false;
-    return true;
-}
-
-// The incremental kernel needs 32-bit ranks and its LDS below 64 KiB.
-static bool run_kernel_ok(const SlosPlanImpl &p)
-{
-    return all_layers_32bit(p) && run_kernel_lds(p.m, p.w, p.n) <= 64 * 1024;
+    const BlkMasked &mk = p.mask;
+    if (w == 0) w = blk_default_width(mk.m_free, mk.nmax_free);
+    if (w < 1 || w >= mk.m_free || w > 16) { set_error("suffix width outside [1, min(free modes - 1, 16)]"); return ERR_ARG; }
+    BlkLayout lay;
+    std::vector<char> host;
+    int rc = blk_layout_build_host_ex(mk.m_free, mk.nmax_free, w, mk.subs, mk.counts, mk.rowmap, &lay, &host);
+    if (rc == OK) rc = blk_layout_upload(&lay, host);
+    if (rc != OK) return rc;
+    blk_layout_destroy(&p.blk);
+    p.blk = std::move(lay);
+    return OK;
 }
 
 static int device_grid_cap(int *cap)
@@ -1153,7 +808,59 @@ extern "C" int pcvl_slos_plan_create(int m, const int32_t *h_in_occ, pcvl_slos_p
         return ERR_HIP;
     }
     int rc = device_grid_cap(&p.grid_cap);
+    if (rc == OK && blk_layout_ok(m, n)) rc = blk_layout_create(m, n, blk_default_width(m, n), &p.blk);
     if (rc != OK) { (void)hipFree(p.d_T); delete plan; return rc; }
+    *out = plan;
+    return OK;
+}
+
+extern "C" int pcvl_slos_plan_create_masked(int m, const int32_t *h_in_occ, const int32_t *h_cond,
+                                            pcvl_slos_plan *out)
+{
+    PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
+    PCVL_ARG_CHECK(h_in_occ != nullptr && h_cond != nullptr, "input state or mask is null");
+    PCVL_ARG_CHECK(m >= 2 && m <= SLOS_MAX_MODES, "mode count outside [2, 64]");
+    *out = nullptr;
+    int n = 0, nfree = 0;
+    for (int i = 0; i < m; i++) {
+        PCVL_ARG_CHECK(h_in_occ[i] >= 0, "negative occupation");
+        PCVL_ARG_CHECK(h_cond[i] >= -1, "mask condition below -1");
+        n += h_in_occ[i];
+        nfree += h_cond[i] < 0 ? 1 : 0;
+    }
+    PCVL_ARG_CHECK(n >= 1 && n <= 30, "masked plans take 1 to 30 photons");
+    PCVL_ARG_CHECK(nfree >= 2, "a masked plan needs at least two modes without a condition");
+    auto *plan = new (std::nothrow) pcvl_slos_plan_s();
+    if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
+    SlosPlanImpl &p = plan->impl;
+    try {
+        p.m = m; p.n = n; p.w = n + 1;
+        p.masked = true;
+        p.kernel = 8;
+        std::vector<int32_t> rem(h_in_occ, h_in_occ + m);
+        for (int k = 0; k < n; k++) {   // _Path._decompose, as in pcvl_slos_plan_create
+            int best = 0;
+            for (int i = 1; i < m; i++) if (rem[i] > rem[best]) best = i;
+            p.mk.push_back(best);
+            rem[best]--;
+        }
+        double pin = 1.0;
+        for (int i = 0; i < m; i++) for (int v = 2; v <= h_in_occ[i]; v++) pin *= v;
+        int rc = blk_masked_layers(m, n, h_cond, &p.mask);
+        if (rc != OK) { delete plan; return rc; }
+        // the herald modes' factorials are the same for every kept state
+        p.inv_prod_in = p.mask.herald_fact / pin;
+        p.counts = p.mask.counts;
+        for (int k = 0; k < n; k++) p.max_inter = std::max<uint64_t>(p.max_inter, p.counts[(size_t)k]);
+        if (!p.mask.empty) {
+            rc = masked_layout(p, 0);
+            if (rc != OK) { delete plan; return rc; }
+        }
+    } catch (const std::bad_alloc &) {
+        delete plan;
+        set_error("host allocation failed while building the masked plan");
+        return ERR_ALLOC;
+    }
     *out = plan;
     return OK;
 }
@@ -1179,6 +886,7 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     PCVL_ARG_CHECK(mode == PCVL_SLOS_COEFS || mode == PCVL_SLOS_PROBS, "mode must be PCVL_SLOS_COEFS or PCVL_SLOS_PROBS");
     PCVL_ARG_CHECK(d_U && d_out, "null device buffer");
     const SlosPlanImpl &p = plan->impl;
+    if (p.masked && p.counts[(size_t)p.n] == 0) return OK;   // the heralds need more photons than there are
     size_t need = 0;
     pcvl_slos_plan_info(plan, nullptr, nullptr, &need, nullptr);
     PCVL_ARG_CHECK(d_work != nullptr && work_bytes >= need, "workspace missing or smaller than pcvl_slos_plan_info reports");
@@ -1186,7 +894,6 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     cd *bufA = reinterpret_cast<cd *>(d_work);
     cd *bufB = bufA + p.max_inter + 1;
     const size_t lds = 2 * sizeof(uint64_t) * (size_t)p.m * p.w;
-
     if (p.n == 0) {
         // Vacuum: one output state with coefficient 1 (probability 1).
         hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, d_out, mode == PCVL_SLOS_PROBS ? 1 : 0);
@@ -1197,25 +904,13 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(bufA), 2);
     PCVL_HIP_CHECK(hipGetLastError());
     const cd *prev = bufA;
-    // Auto takes the compact successor kernel where its packed parent ranks
-    // and LDS fit (parent layers below 2^26 states), else the two-phase
-    // kernel where its ranks fit, else the per-state kernel.  At m=24, n=10
-    // on MI355X: successor 1.7 ms, two-phase 3.1 ms, per state 3.4 ms,
-    // incremental 7.8 ms (its lanes gather parents at a stride of RUN
-    // elements, and the resulting L2 misses read 41.5 GB for the last layer).
-    const bool small = all_layers_32bit(p);
-    const int kernel = p.kernel != 0 ? p.kernel : succ_ok(p, 6) ? 6 : twophase_ok(p) ? 4 : 1;
-    const bool use_run = (kernel == 2);
-    const bool use32 = (kernel == 3);
-    const bool usesucc = (kernel == 6 || kernel == 7);
-    const bool use2p = (kernel == 4) || (kernel == 5);
-    const bool g8 = (kernel == 5);
-    if ((use32 && !small) || (use2p && !twophase_ok(p)) || (usesucc && !succ_ok(p, kernel))) {
+    // Auto takes the block kernel where its layout was built (every layer
+    // below 2^32 states), else the compact successor kernel where its packed
+    // parent ranks and LDS fit (parent layers below 2^26 states), else the
+    // two-phase kernel where its ranks fit, else the per-state kernel.
+    const int kernel = p.masked ? 8 : p.kernel != 0 ? p.kernel : blk_ok(p) ? 8 : succ_ok(p) ? 6 : twophase_ok(p) ? 4 : 1;
+    if ((kernel == 4 && !twophase_ok(p)) || (kernel == 6 && !succ_ok(p)) || (kernel == 8 && !blk_ok(p))) {
         set_error("SLOS kernel not applicable to this plan's layer sizes");
-        return ERR_ARG;
-    }
-    if (use_run && !run_kernel_ok(p)) {
-        set_error("incremental SLOS kernel needs layers below 2^32 states and 64 KiB of LDS");
         return ERR_ARG;
     }
     for (int k = 1; k <= p.n; k++) {
@@ -1223,50 +918,26 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         const bool last = (k == p.n);
         void *dst = last ? static_cast<void *>(d_out) : static_cast<void *>(prev == bufA ? bufB : bufA);
         const bool probs = last && mode == PCVL_SLOS_PROBS;
-        if (usesucc) {
-            const int run = kernel == 6 ? 3 : 2;
-            const uint64_t tile = (uint64_t)run * SLOS7_THREADS;
+        if (kernel == 8) {
+            const int rc = blk_layer_launch(p.blk, prev, dst, U, p.m, p.mk[k - 1], k, probs, p.inv_prod_in, !last,
+                                            stream);
+            if (rc != OK) return rc;
+        } else if (kernel == 6) {
+            const uint64_t tile = 3ull * SLOS7_THREADS;
             // a multiple of 8 workgroups, for the XCD-contiguous tile order
             const int grid = (int)((std::min<uint64_t>((nc + tile - 1) / tile, (uint64_t)p.grid_cap) + 7) / 8 * 8);
             const bool pfi = k <= 12;
-            auto kp = kernel == 6
-                          ? (probs ? (pfi ? slos_layer_succ2_kernel<1, 3, true, 4> : slos_layer_succ2_kernel<1, 3, false, 4>)
-                                   : slos_layer_succ2_kernel<0, 3, true, 4>)
-                          : (probs ? (pfi ? slos_layer_succ_kernel<1, 2, true, 8> : slos_layer_succ_kernel<1, 2, false, 8>)
-                                   : slos_layer_succ_kernel<0, 2, true, 8>);
-            hipLaunchKernelGGL(kp, dim3(grid), dim3(SLOS7_THREADS), succ_kernel_lds(p, kernel, k, probs), stream, prev,
+            auto kp = probs ? (pfi ? slos_layer_succ2_kernel<1, 3, true, 4> : slos_layer_succ2_kernel<1, 3, false, 4>)
+                            : slos_layer_succ2_kernel<0, 3, true, 4>;
+            hipLaunchKernelGGL(kp, dim3(grid), dim3(SLOS7_THREADS), succ2_lds(p.m, p.n, k, 3, probs), stream, prev,
                                (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_P, p.n + 2,
                                p.inv_prod_in, last ? 0 : 1);
-        } else if (use_run) {
-            const uint64_t per_block = (uint64_t)SLOS2_RUN * SLOS2_THREADS;
-            const int grid = (int)std::min<uint64_t>((nc + per_block - 1) / per_block, (uint64_t)p.grid_cap * 2);
-            const size_t lds2 = run_kernel_lds(p.m, p.w, k);
-            if (probs)
-                hipLaunchKernelGGL(slos_layer_run_kernel<1>, dim3(grid), dim3(SLOS2_THREADS), lds2, stream, prev,
-                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
-                                   p.d_E, p.w, p.inv_prod_in);
-            else
-                hipLaunchKernelGGL(slos_layer_run_kernel<0>, dim3(grid), dim3(SLOS2_THREADS), lds2, stream, prev,
-                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
-                                   p.d_E, p.w, p.inv_prod_in);
-        } else if (use2p) {
+        } else if (kernel == 4) {
             const int grid = (int)std::min<uint64_t>((nc + SLOS4_THREADS - 1) / SLOS4_THREADS, (uint64_t)p.grid_cap);
-            const size_t lds2p = twophase_lds(p.m, p.w, k);
-            auto kp = probs ? (g8 ? slos_layer2p_kernel<1, 8> : slos_layer2p_kernel<1, 4>)
-                            : (g8 ? slos_layer2p_kernel<0, 8> : slos_layer2p_kernel<0, 4>);
-            hipLaunchKernelGGL(kp, dim3(grid), dim3(SLOS4_THREADS), lds2p, stream, prev, (uint32_t)p.counts[k - 1], dst,
-                               U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T, p.d_E, p.w, p.inv_prod_in);
-        } else if (use32) {
-            const int grid = (int)std::min<uint64_t>((nc + SLOS_THREADS - 1) / SLOS_THREADS, (uint64_t)p.grid_cap);
-            const size_t lds32 = sizeof(cd) * p.m + sizeof(double) * p.w + 2 * sizeof(uint32_t) * p.m * p.w;
-            if (probs)
-                hipLaunchKernelGGL(slos_layer32_kernel<1>, dim3(grid), dim3(SLOS_THREADS), lds32, stream, prev,
-                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
-                                   p.d_E, p.w, p.inv_prod_in);
-            else
-                hipLaunchKernelGGL(slos_layer32_kernel<0>, dim3(grid), dim3(SLOS_THREADS), lds32, stream, prev,
-                                   (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T,
-                                   p.d_E, p.w, p.inv_prod_in);
+            auto kp = probs ? slos_layer2p_kernel<1, 4> : slos_layer2p_kernel<0, 4>;
+            hipLaunchKernelGGL(kp, dim3(grid), dim3(SLOS4_THREADS), twophase_lds(p.m, p.w, k), stream, prev,
+                               (uint32_t)p.counts[k - 1], dst, U, p.m, p.mk[k - 1], k, (uint32_t)nc, p.d_T, p.d_E, p.w,
+                               p.inv_prod_in);
         } else {
             const int grid = (int)std::min<uint64_t>((nc + SLOS_THREADS - 1) / SLOS_THREADS, (uint64_t)p.grid_cap);
             if (probs)
@@ -1285,16 +956,50 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
 extern "C" int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel)
 {
     PCVL_ARG_CHECK(plan != nullptr, "plan is null");
-    PCVL_ARG_CHECK(kernel >= 0 && kernel <= 7,
-                   "kernel must be 0 (auto), 1 (per state), 2 (incremental), 3 (per state, 32-bit), "
-                   "4 (per state, two phases), 5 (two phases, eight loads in flight), 6 (successor runs of 3, "
-                   "compact slots) or 7 (successor runs of 2, full slots)");
-    if (kernel >= 4) PCVL_ARG_CHECK(twophase_ok(plan->impl), "two-phase kernel needs parent layers below 2^26 states");
-    if (kernel >= 6) PCVL_ARG_CHECK(succ_ok(plan->impl, kernel), "successor kernel needs its LDS below 64 KiB");
-    if (kernel == 2) PCVL_ARG_CHECK(run_kernel_ok(plan->impl), "incremental kernel not applicable to this plan");
-    if (kernel == 3) PCVL_ARG_CHECK(all_layers_32bit(plan->impl), "32-bit kernel needs layers below 2^32 states");
+    PCVL_ARG_CHECK(kernel == 0 || kernel == 1 || kernel == 4 || kernel == 6 || kernel == 8,
+                   "kernel must be 0 (auto), 1 (per state), 4 (per state, two phases), 6 (successor runs) "
+                   "or 8 (prefix/suffix blocks)");
+    if (kernel == 4 || kernel == 6)
+        PCVL_ARG_CHECK(twophase_ok(plan->impl), "two-phase and successor kernels need parent layers below 2^26 states");
+    if (kernel == 6) PCVL_ARG_CHECK(succ_ok(plan->impl), "successor kernel needs its LDS below 64 KiB");
+    if (kernel == 8) PCVL_ARG_CHECK(blk_ok(plan->impl), "block kernel needs every layer below 2^32 states and m >= 2");
+    if (plan->impl.masked) {
+        PCVL_ARG_CHECK(kernel == 0 || kernel == 8, "masked plans run the block kernel only (0 or 8)");
+        return OK;
+    }
     plan->impl.kernel = kernel;
     return OK;
+}
+
+extern "C" int pcvl_slos_plan_set_block_width(pcvl_slos_plan plan, int w)
+{
+    PCVL_ARG_CHECK(plan != nullptr, "plan is null");
+    SlosPlanImpl &p = plan->impl;
+    if (p.masked) {
+        if (p.mask.empty) return OK;
+        if (w == 0) w = blk_default_width(p.mask.m_free, p.mask.nmax_free);
+        if (p.blk.d_mem && p.blk.w == w) return OK;
+        PCVL_HIP_CHECK(hipDeviceSynchronize());   // the previous layout may still be read by enqueued work
+        return masked_layout(p, w);
+    }
+    PCVL_ARG_CHECK(blk_layout_ok(p.m, p.n), "block kernel needs every layer below 2^32 states, m >= 2 and n in [1, 30]");
+    if (w == 0) w = blk_default_width(p.m, p.n);
+    PCVL_ARG_CHECK(w >= 1 && w < p.m && w <= 16, "suffix width outside [1, min(m - 1, 16)]");
+    if (p.blk.d_mem && p.blk.w == w) return OK;
+    BlkLayout lay;
+    const int rc = blk_layout_create(p.m, p.n, w, &lay);
+    if (rc != OK) return rc;
+    // the previous layout may still be read by enqueued work
+    PCVL_HIP_CHECK(hipDeviceSynchronize());
+    blk_layout_destroy(&p.blk);
+    p.blk = std::move(lay);
+    return OK;
+}
+
+extern "C" int pcvl_slos_plan_block_width(pcvl_slos_plan plan)
+{
+    if (plan == nullptr) return 0;
+    return plan->impl.blk.d_mem ? plan->impl.blk.w : 0;
 }
 
 extern "C" int pcvl_slos_plan_destroy(pcvl_slos_plan plan)
@@ -1302,6 +1007,7 @@ extern "C" int pcvl_slos_plan_destroy(pcvl_slos_plan plan)
     if (!plan) return OK;
     hipError_t e = hipSuccess;
     if (plan->impl.d_T) e = hipFree(plan->impl.d_T);
+    blk_layout_destroy(&plan->impl.blk);
     delete plan;
     if (e != hipSuccess) { set_error(std::string("hipFree(slos plan): ") + hipGetErrorString(e)); return ERR_HIP; }
     return OK;
@@ -1336,6 +1042,7 @@ struct SlosTreeImpl {
     std::vector<TreeOp> ops;            // depth-first launch list
     uint64_t *d_T = nullptr, *d_E = nullptr;
     int grid_cap = 0;
+    BlkLayout blk;                      // block layout of the layers, when they allow it
 };
 
 static void tree_decompose(SlosTreeImpl &t, int k, int mk, std::vector<std::vector<int32_t>> targets,
@@ -1454,6 +1161,7 @@ extern "C" int pcvl_slos_tree_create(int m, const int32_t *h_inputs, int count, 
         return ERR_HIP;
     }
     int rc = device_grid_cap(&t.grid_cap);
+    if (rc == OK && blk_layout_ok(m, t.nmax)) rc = blk_layout_create(m, t.nmax, blk_default_width(m, t.nmax), &t.blk);
     if (rc != OK) { (void)hipFree(t.d_T); delete plan; return rc; }
     *out = plan;
     return OK;
@@ -1511,7 +1219,13 @@ extern "C" int pcvl_slos_tree_execute(pcvl_slos_tree plan, const double *d_U, in
             dst = static_cast<char *>(d_out) + esz * (size_t)t.out_off[(size_t)op.input];
             probs = mode == PCVL_SLOS_PROBS;
         }
-        if (probs)
+        if (t.blk.d_mem) {
+            // no sentinel: depth buffers hold exactly their layer, and only
+            // the block and per-state kernels run here
+            const int rc = blk_layer_launch(t.blk, parent, dst, U, t.m, op.mk, op.k, probs, op.inv_prod_in, false,
+                                            stream);
+            if (rc != OK) return rc;
+        } else if (probs)
             hipLaunchKernelGGL(slos_layer_kernel<1>, dim3(grid), dim3(SLOS_THREADS), lds, stream, parent,
                                t.counts[(size_t)op.k - 1], dst, U, t.m, op.mk, op.k, nc, t.d_T, t.d_E, t.w,
                                op.inv_prod_in);
@@ -1529,6 +1243,7 @@ extern "C" int pcvl_slos_tree_destroy(pcvl_slos_tree plan)
     if (!plan) return OK;
     hipError_t e = hipSuccess;
     if (plan->impl.d_T) e = hipFree(plan->impl.d_T);
+    blk_layout_destroy(&plan->impl.blk);
     delete plan;
     if (e != hipSuccess) { set_error(std::string("hipFree(slos tree): ") + hipGetErrorString(e)); return ERR_HIP; }
     return OK;

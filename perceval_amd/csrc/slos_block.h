@@ -1,0 +1,128 @@
+// Prefix/suffix block layout of the SLOS layers (internal to the library).
+//
+// A layer F(m, k) in descending lexicographic order is cut at mode h = m - w:
+// a child c = (alpha, beta) has alpha on modes 0..h-1 and beta on the w
+// suffix modes, |beta| = r.  The children sharing alpha form one contiguous
+// block of S(r) = N(w, r) states, ordered by the rank of beta in F(w, r):
+//     rank_k(c) = B_k(alpha) + rank(beta).
+// Parents, in increasing mode order (the summation order of
+// FSMap.compute_slos_layer, perceval/backends/_slos.py:88-102):
+//   prefix mode j (alpha_j > 0): rank_k(c) - PE_j(alpha)   (the same beta)
+//   suffix mode j (beta_j > 0):  B_k(alpha) - PE_h(alpha) + rank(beta - e_j)
+// with PE_j(alpha) = sum_{q<j} E_q(R_{q+1}) as in the per-state kernels.  The
+// prefix part is the same for every child of a block and the suffix part the
+// same for every block with the same r, so one walk per block and one table
+// per r (shared by every layer) replace the per-child unranking.
+//
+// Masked layers (heralds).  With a mask fixing the photon count c_j of some
+// modes H (the masked FSArray of _slos.py:161-164), layer k only needs the
+// states whose herald part a = s_H can still reach c: a <= c and
+// k - |a| <= n - |c| (partial match).  Every parent of such a state is again
+// one.  With the modes reordered herald modes first, each needed layer is a
+// concatenation, over the admissible a in descending lexicographic order, of
+// full free-mode spaces F(m_F, k - |a|) ("sub-layers"); a child (a, f) takes
+// its parents (a - e_j, f) from sub-layer a - e_j of layer k-1 at the same
+// position, and (a, f - e_j) from sub-layer a of layer k-1 by the block
+// recursion over the free modes.  The unmasked layer is the one sub-layer
+// with no herald modes.
+#pragma once
+
+#include "common.h"
+
+#include <utility>
+#include <vector>
+
+namespace pcvl {
+
+// One group of a layer: the blocks of one sub-layer with r photons in the
+// suffix modes.  64 bytes.
+struct BlkGroup {
+    uint32_t wg0;       // first workgroup of the group
+    uint32_t nwg;       // workgroups (a multiple of 8, for the XCD-contiguous order)
+    uint32_t nitems;    // wave items: (block, 64-child chunk), or 64 consecutive children
+    uint32_t nb;        // blocks: N(h, K)
+    uint32_t S;         // children per block: N(w, r)
+    uint32_t nch;       // 64-child chunks per block (one wave each); 0: waves span several blocks
+    uint32_t tpf;       // offset of r's factorial products in the suffix pf table
+    uint32_t tslot;     // offset of r's slot words in the suffix slot table
+    uint32_t r;         // photons in the suffix modes
+    uint32_t K;         // photons in the prefix (free) modes
+    uint32_t out_base;  // offset of the group's sub-layer in layer k
+    uint32_t par_base;  // offset of the same herald configuration's sub-layer in layer k - 1
+    uint32_t her_off;   // herald parents: entries [her_off, her_off + nher) of the herald table
+    uint32_t nher;
+    uint32_t pad0, pad1;
+};
+
+// One sub-layer of layer k: herald configuration a.
+struct BlkSub {
+    uint32_t out_base = 0;   // offset in layer k
+    uint32_t par_base = 0;   // offset of sub-layer a in layer k - 1 (used when kf >= 1)
+    int kf = 0;              // photons in the free modes: k - |a|
+    // herald parents, in herald mode order: (herald index i, offset of
+    // sub-layer a - e_{H_i} in layer k - 1)
+    std::vector<std::pair<uint32_t, uint32_t>> her;
+};
+
+struct BlkLayout {
+    int m = 0;                // free modes (all modes when unmasked)
+    int nmax = 0;             // most photons in the free modes
+    int w = 0, h = 0, wt = 0;
+    int nlayers = 0;          // layers 1..nlayers
+    int mloc = 0;             // U rows used: free modes then herald modes
+    std::vector<uint64_t> counts;                // states of layer k (compact when masked), k = 0..nlayers
+    std::vector<std::vector<BlkGroup>> groups;   // per layer k (index k)
+    std::vector<uint32_t> grid;                  // workgroups per layer
+    std::vector<size_t> group_off;               // byte offset of layer k's groups in d_mem
+    size_t off_th = 0, off_te = 0, off_cte = 0, off_f32 = 0, off_fd = 0, off_pf = 0, off_slot = 0, off_row = 0,
+           off_her = 0;
+    char *d_mem = nullptr;
+    size_t bytes = 0;
+};
+
+// Suffix width used when the caller does not choose one (0): the widest
+// w <= 10 below m whose suffix tables stay under 16 MiB.
+int blk_default_width(int m, int nmax);
+
+// The layout of layers 1..n of the full space F(m, k), suffix width w
+// (1 <= w < m), with its device tables.  Every layer must hold fewer than
+// 2^32 states.
+int blk_layout_create(int m, int nmax, int w, BlkLayout *out);
+// The same without the device upload: fills `image` with the bytes that
+// blk_layout_create copies to the device (offsets in the layout).
+int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<char> *image);
+// General form: m_free free modes holding at most nmax_free photons, the
+// sub-layers of each layer (subs[k], k = 1..subs.size()-1, subs[0] unused),
+// their state counts (counts[k], k = 0..), and the U row of each local mode
+// (rowmap: free modes, then herald modes).
+int blk_layout_build_host_ex(int m_free, int nmax_free, int w, const std::vector<std::vector<BlkSub>> &subs,
+                             const std::vector<uint64_t> &counts, const std::vector<int32_t> &rowmap,
+                             BlkLayout *out, std::vector<char> *image);
+// The masked layers of an n-photon plan over m modes whose mask fixes
+// cond[j] >= 0 photons in mode j (cond[j] < 0: any count).
+struct BlkMasked {
+    int m_free = 0, nher = 0;
+    int nmax_free = 0;                          // photons in the free modes of the final layer
+    bool empty = false;                         // the heralds need more photons than the input has
+    std::vector<int32_t> rowmap;                // free modes, then herald modes (original indices)
+    std::vector<std::vector<BlkSub>> subs;      // per layer k = 1..n
+    std::vector<uint64_t> counts;               // states per layer k = 0..n
+    double herald_fact = 1.0;                   // prod over herald modes of c_j!
+};
+int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out);
+
+// Uploads a host-built layout (allocates d_mem).
+int blk_layout_upload(BlkLayout *lay, const std::vector<char> &image);
+void blk_layout_destroy(BlkLayout *lay);
+bool blk_layout_ok(int m, int nmax);
+
+// Enqueues layer k of the layout: children from the parent layer k - 1,
+// column mk of the m x m unitary U.  probs: write |c|^2 prod s! *
+// inv_prod_in (doubles) instead of complex coefficients (prod s! over the
+// free modes: herald factorials are the caller's constant, folded into
+// inv_prod_in).  sentinel: write one zero coefficient after the last child
+// (read by the successor kernels of the next layer).
+int blk_layer_launch(const BlkLayout &lay, const cd *parent, void *out, const cd *U, int m, int mk, int k, bool probs,
+                     double inv_prod_in, bool sentinel, hipStream_t stream);
+
+} // namespace pcvl

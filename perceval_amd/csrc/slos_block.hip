@@ -1,0 +1,629 @@
+// SLOS layer kernel over prefix/suffix blocks (layout in slos_block.h).
+//
+// Replaces FSMap.compute_slos_layer (perceval/backends/_slos.py:88-102) and,
+// on the last layer, xq.all_prob_normalize_output (_slos.py:195-214), as the
+// per-state and successor kernels of slos.hip do, with the same parents in
+// the same summation order: results are bit-identical to theirs.
+//
+// Work.  The children of layer k are split into groups by r = photons in
+// the w suffix modes; group r holds N(h, k-r) blocks of S(r) = N(w, r)
+// children.  A wave is one work item:
+//   * S(r) >= 32: one 64-child chunk of one block.  The block index is
+//     wave-uniform, so the walk that unranks alpha runs on the scalar unit
+//     and leaves the parent base of every occupied prefix mode in a VGPR
+//     (lane d holds entry d).  Each prefix parent load is then a uniform base
+//     plus the lane's beta: one contiguous 1 KiB run per load instruction.
+//   * S(r) < 32: 64 consecutive children of the group, across blocks; each
+//     lane walks its own alpha and keeps its list in LDS.
+// The suffix parents come from a table per r (every block with the same r
+// has the same suffix structure): slot d of beta holds
+// (rank(beta - e_j) << 5) | (j - h) for its d-th occupied suffix mode j, or
+// all ones past its last one.  Groups run in increasing r, so the parent
+// groups r and r - 1 that a group reads stay in the last-level cache, and
+// each group gives XCD x a contiguous share of its workgroups (each XCD has
+// its own L2).
+#include "common.h"
+#include "slos_block.h"
+#include "../../include/pcvl_hip.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+#include <type_traits>
+
+namespace pcvl {
+
+constexpr int BLK_THREADS = 256;
+constexpr int BLK_WMAX = 16;        // widest suffix
+constexpr uint32_t BLK_A_MIN = 32;  // smallest block given whole waves
+
+static uint64_t blk_binom(int64_t a, int64_t b)
+{
+    if (b < 0 || a < 0 || b > a) return 0;
+    if (b > a - b) b = a - b;
+    unsigned __int128 r = 1;
+    for (int64_t i = 1; i <= b; i++) {
+        r = r * (unsigned __int128)(a - b + i) / (unsigned __int128)i;
+        if (r > (unsigned __int128)UINT64_MAX) return UINT64_MAX;
+    }
+    return (uint64_t)r;
+}
+
+// N(modes, x): states of x photons in `modes` modes.
+static uint64_t nstates(int modes, int x)
+{
+    if (x < 0) return 0;
+    if (modes == 0) return x == 0 ? 1 : 0;
+    return blk_binom((int64_t)x + modes - 1, x);
+}
+
+// Herald parents of the child at position loc of its sub-layer: entry (i, base)
+// gives parent[base + loc] times U[H_i, mk] (su[mfree + i]).  They come first
+// in the summation order (herald modes lead the local mode order).
+__device__ __forceinline__ void blk_heralds(cd &acc, const cd *__restrict__ parent, const uint2 *__restrict__ her,
+                                            uint32_t nher, const cd *su, int mfree, uint32_t loc)
+{
+    for (uint32_t d0 = 0; d0 < nher; d0 += 4) {
+        cd pv[4], uv[4];
+#pragma unroll
+        for (uint32_t t = 0; t < 4; t++) {
+            pv[t] = cd{0.0, 0.0};
+            uv[t] = cd{0.0, 0.0};
+            if (d0 + t < nher) {
+                const uint2 e = her[d0 + t];
+                pv[t] = parent[(size_t)e.y + loc];
+                uv[t] = su[mfree + (int)e.x];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);   // zero past the list: adds an exact 0
+    }
+}
+
+template <int MODE, bool PFI>
+__global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
+    const cd *__restrict__ parent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk, int k,
+    uint32_t nchild, int h, int wsfx, int mfree, int mloc, const int32_t *__restrict__ rowmap,
+    const BlkGroup *__restrict__ groups, int ngroups, const uint2 *__restrict__ her,
+    const uint32_t *__restrict__ Th, const uint64_t *__restrict__ TE, const uint64_t *__restrict__ CTE, int wt,
+    const uint32_t *__restrict__ f32, const double *__restrict__ fd, const double *__restrict__ spf,
+    const uint32_t *__restrict__ sslot, double inv_prod_in, int sentinel)
+{
+    using pf_t = typename std::conditional<PFI, uint32_t, double>::type;
+    extern __shared__ uint64_t s_blk[];
+    cd *su = reinterpret_cast<cd *>(s_blk);                               // local mode -> U[row, mk]
+    uint32_t *lpe = reinterpret_cast<uint32_t *>(su + mloc);              // k x THREADS (lists of waves over several blocks)
+    uint8_t *lmd = reinterpret_cast<uint8_t *>(lpe + (size_t)k * BLK_THREADS);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < mloc; e += BLK_THREADS) su[e] = U[(size_t)rowmap[e] * m + mk];
+    if (sentinel && blockIdx.x == 0 && tid == 0) reinterpret_cast<cd *>(out)[nchild] = cd{0.0, 0.0};
+    __syncthreads();
+
+    int lo = 0, hi = ngroups - 1;   // last group whose first workgroup is <= blockIdx.x
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (groups[mid].wg0 <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const BlkGroup g = groups[lo];
+    const uint32_t gl = blockIdx.x - g.wg0, t8 = g.nwg >> 3;
+    const uint32_t gx = (gl & 7u) * t8 + (gl >> 3);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t item = gx * (BLK_THREADS / 64) + wave;
+    if (item >= g.nitems) return;
+    const uint32_t lane = tid & 63u;
+    const int r = (int)g.r, K = (int)g.K;
+    const int smax = min(r, wsfx);
+    const uint32_t S = g.S;
+
+    uint32_t beta, Bk, PEh;
+    pf_t pfp = (pf_t)1;
+    cd acc = {0.0, 0.0};
+    if (g.nch != 0) {
+        // ---- one chunk of one block: alpha is wave-uniform
+        const uint32_t b = item / g.nch;
+        const uint32_t c = item - b * g.nch;
+        beta = c * 64u + lane;
+        uint32_t rem = b, PE = 0;
+        int P = K, cnt = 0, vpe = 0, vmd = 0;
+        Bk = 0;
+        for (int q = 0; q < h; q++) {
+            int x = 0;
+            if (q < h - 1) {
+                x = P;
+                uint32_t t = Th[q * wt + x];
+                while (t > rem) { --x; t = Th[q * wt + x]; }
+                rem -= t;
+            }
+            const uint64_t te = TE[q * wt + x + r];
+            Bk += (uint32_t)te;
+            if (P > x) {
+                vpe = lane == (uint32_t)cnt ? (int)PE : vpe;   // lane cnt takes entry cnt
+                vmd = lane == (uint32_t)cnt ? q : vmd;
+                cnt++;
+                if (PFI) pfp *= (pf_t)f32[P - x];
+                else pfp *= (pf_t)fd[P - x];
+            }
+            PE += (uint32_t)(te >> 32);
+            P = x;
+            if (P == 0) {   // modes q+1..h-1 are empty
+                const uint64_t ce = CTE[r * (h + 1) + q + 1];
+                Bk += (uint32_t)ce;
+                PE += (uint32_t)(ce >> 32);
+                break;
+            }
+        }
+        PEh = PE;
+        if (beta >= S) return;
+        if (g.nher) blk_heralds(acc, parent, her + g.her_off, g.nher, su, mfree, Bk + beta);
+        const uint32_t boff = beta << 4;
+        const cd *pp = parent + g.par_base;
+        for (int d0 = 0; d0 < cnt; d0 += 4) {
+            cd pv[4], uv[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                pv[t] = cd{0.0, 0.0};
+                uv[t] = cd{0.0, 0.0};
+                if (d0 + t < cnt) {
+                    const uint32_t pe = (uint32_t)__builtin_amdgcn_readlane(vpe, d0 + t);
+                    const int j = __builtin_amdgcn_readlane(vmd, d0 + t);
+                    const char *pb = reinterpret_cast<const char *>(pp + (Bk - pe));
+                    pv[t] = *reinterpret_cast<const cd *>(pb + boff);
+                    uv[t] = su[j];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);   // zero past the list: adds an exact 0
+        }
+    } else {
+        // ---- 64 consecutive children of the group, each lane its own block
+        const uint32_t i = item * 64u + lane;
+        if (i >= g.nb * S) return;
+        const uint32_t b = i / S;
+        beta = i - b * S;
+        uint32_t rem = b, PE = 0;
+        int P = K, cnt = 0;
+        Bk = 0;
+        for (int q = 0; q < h; q++) {
+            int x = 0;
+            if (q < h - 1) {
+                x = P;
+                uint32_t t = Th[q * wt + x];
+                while (t > rem) { --x; t = Th[q * wt + x]; }
+                rem -= t;
+            }
+            const uint64_t te = TE[q * wt + x + r];
+            Bk += (uint32_t)te;
+            if (P > x) {
+                lpe[cnt * BLK_THREADS + tid] = PE;
+                lmd[cnt * BLK_THREADS + tid] = (uint8_t)q;
+                cnt++;
+                if (PFI) pfp *= (pf_t)f32[P - x];
+                else pfp *= (pf_t)fd[P - x];
+            }
+            PE += (uint32_t)(te >> 32);
+            P = x;
+            if (P == 0) {
+                const uint64_t ce = CTE[r * (h + 1) + q + 1];
+                Bk += (uint32_t)ce;
+                PE += (uint32_t)(ce >> 32);
+                break;
+            }
+        }
+        PEh = PE;
+        const uint32_t cidx = Bk + beta;
+        if (g.nher) blk_heralds(acc, parent, her + g.her_off, g.nher, su, mfree, cidx);
+        const cd *pp = parent + g.par_base;
+        for (int d0 = 0; d0 < cnt; d0 += 4) {
+            cd pv[4], uv[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                pv[t] = cd{0.0, 0.0};
+                uv[t] = cd{0.0, 0.0};
+                if (d0 + t < cnt) {
+                    pv[t] = pp[cidx - lpe[(d0 + t) * BLK_THREADS + tid]];
+                    uv[t] = su[lmd[(d0 + t) * BLK_THREADS + tid]];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);   // zero past the list: adds an exact 0
+        }
+    }
+
+    // ---- suffix parents, from the group's slot table
+    if (smax > 0) {
+        const uint32_t *ts = sslot + g.tslot + beta;
+        const char *ps = reinterpret_cast<const char *>(parent + g.par_base + (Bk - PEh));
+        uint32_t sw[BLK_WMAX];
+#pragma unroll
+        for (int d = 0; d < BLK_WMAX; d++)
+            if (d < smax) sw[d] = ts[(size_t)d * S];
+#pragma unroll
+        for (int d0 = 0; d0 < BLK_WMAX; d0 += 4) {
+            if (d0 < smax) {
+                cd pv[4], uv[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    pv[t] = cd{0.0, 0.0};
+                    uv[t] = cd{0.0, 0.0};
+                    if (d0 + t < smax && sw[d0 + t] != 0xffffffffu) {
+                        pv[t] = *reinterpret_cast<const cd *>(ps + ((sw[d0 + t] >> 5) << 4));
+                        uv[t] = su[h + (int)(sw[d0 + t] & 31u)];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);
+            }
+        }
+    }
+
+    const size_t o = (size_t)g.out_base + Bk + beta;
+    if (MODE == 0) {
+        reinterpret_cast<cd *>(out)[o] = acc;
+    } else {
+        const double pf = (double)pfp * spf[g.tpf + beta];
+        reinterpret_cast<double *>(out)[o] = (acc.r * acc.r + acc.i * acc.i) * pf * inv_prod_in;
+    }
+}
+
+// ------------------------------------------------------------------ host
+
+bool blk_layout_ok(int m, int nmax)
+{
+    if (m < 2 || nmax < 1 || nmax > 30) return false;
+    for (int k = 0; k <= nmax; k++)
+        if (nstates(m, k) >= (1ull << 32)) return false;
+    return true;
+}
+
+static uint64_t suffix_table_words(int w, int nmax)
+{
+    uint64_t words = 0;
+    for (int r = 0; r <= nmax; r++) words += nstates(w, r) * (2 + (uint64_t)std::min(r, w));
+    return words;
+}
+
+int blk_default_width(int m, int nmax)
+{
+    int w = std::min(10, m - 1);
+    while (w > 1 && suffix_table_words(w, nmax) * 4 > (16u << 20)) w--;
+    return std::max(w, 1);
+}
+
+// Rank of beta in F(w, |beta|), descending lexicographic order.
+static uint64_t local_rank(const std::vector<int> &beta)
+{
+    const int w = (int)beta.size();
+    int R = 0;
+    for (int v : beta) R += v;
+    uint64_t rk = 0;
+    for (int q = 0; q + 1 < w; q++) {
+        R -= beta[(size_t)q];
+        rk += blk_binom((int64_t)R + w - q - 2, w - q - 1);
+    }
+    return rk;
+}
+
+int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vector<BlkSub>> &subs,
+                             const std::vector<uint64_t> &counts, const std::vector<int32_t> &rowmap,
+                             BlkLayout *out, std::vector<char> *image)
+{
+    if (m < 2 || nmax < 0 || nmax > 30) { set_error("block SLOS layout: needs 2 or more free modes and at most 30 photons"); return ERR_ARG; }
+    if (w < 1 || w >= m || w > BLK_WMAX) { set_error("block SLOS layout: suffix width outside [1, min(m-1, 16)]"); return ERR_ARG; }
+    if (subs.empty() || counts.size() < subs.size()) { set_error("block SLOS layout: inconsistent layers"); return ERR_ARG; }
+    for (uint64_t c : counts)
+        if (c >= (1ull << 32)) { set_error("block SLOS layout: layers must hold fewer than 2^32 states"); return ERR_ARG; }
+    if ((int)rowmap.size() < m) { set_error("block SLOS layout: row map shorter than the free modes"); return ERR_ARG; }
+    BlkLayout lay;
+    lay.m = m; lay.nmax = nmax; lay.w = w; lay.h = m - w; lay.wt = nmax + 2;
+    lay.nlayers = (int)subs.size() - 1;
+    lay.mloc = (int)rowmap.size();
+    lay.counts = counts;
+    const int h = lay.h, wt = lay.wt;
+    // tables
+    std::vector<uint32_t> th((size_t)h * wt, 0);
+    std::vector<uint64_t> te((size_t)h * wt, 0);
+    std::vector<uint64_t> cte((size_t)(nmax + 1) * (h + 1), 0);
+    for (int q = 0; q < h; q++)
+        for (int x = 0; x < wt; x++) {
+            if (q + 1 < h) th[(size_t)q * wt + x] = (uint32_t)blk_binom((int64_t)x + h - q - 2, h - q - 1);
+            const uint64_t t = blk_binom((int64_t)x + m - q - 2, m - q - 1);
+            const uint64_t e = blk_binom((int64_t)x + m - q - 3, m - q - 2);
+            te[(size_t)q * wt + x] = (t & 0xffffffffull) | (e << 32);
+        }
+    for (int r = 0; r <= nmax; r++) {
+        uint64_t st = 0, se = 0;
+        for (int q = h - 1; q >= 0; q--) {
+            st += te[(size_t)q * wt + r] & 0xffffffffull;
+            se += te[(size_t)q * wt + r] >> 32;
+            cte[(size_t)r * (h + 1) + q] = (st & 0xffffffffull) | (se << 32);
+        }
+    }
+    std::vector<uint32_t> f32(13, 1);
+    std::vector<double> fd((size_t)nmax + 1, 1.0);
+    for (int a = 2; a <= 12; a++) f32[(size_t)a] = f32[(size_t)a - 1] * (uint32_t)a;
+    for (int a = 2; a <= nmax; a++) fd[(size_t)a] = fd[(size_t)a - 1] * (double)a;
+    // suffix tables: per r, pf[beta] and slot[d][beta]
+    std::vector<double> spf;
+    std::vector<uint32_t> sslot;
+    std::vector<uint32_t> tpf((size_t)nmax + 1), tslot((size_t)nmax + 1);
+    for (int r = 0; r <= nmax; r++) {
+        const uint64_t S = nstates(w, r);
+        const int ns = std::min(r, w);
+        tpf[(size_t)r] = (uint32_t)spf.size();
+        tslot[(size_t)r] = (uint32_t)sslot.size();
+        const size_t p0 = spf.size(), s0 = sslot.size();
+        spf.resize(p0 + S);
+        sslot.resize(s0 + S * (size_t)ns, 0xffffffffu);
+        std::vector<int> beta((size_t)w, 0);
+        beta[0] = r;
+        for (uint64_t i = 0; i < S; i++) {
+            double pf = 1.0;
+            int d = 0;
+            for (int j = 0; j < w; j++) {
+                for (int v = 2; v <= beta[(size_t)j]; v++) pf *= v;
+                if (beta[(size_t)j] > 0) {
+                    beta[(size_t)j]--;
+                    const uint64_t lp = local_rank(beta);
+                    beta[(size_t)j]++;
+                    sslot[s0 + (size_t)d * S + i] = (uint32_t)(lp << 5) | (uint32_t)j;
+                    d++;
+                }
+            }
+            spf[p0 + i] = pf;
+            // next state in descending lexicographic order
+            if (i + 1 < S) {
+                const int tail = beta[(size_t)w - 1];
+                beta[(size_t)w - 1] = 0;
+                int t = w - 2;
+                while (beta[(size_t)t] == 0) t--;
+                beta[(size_t)t]--;
+                beta[(size_t)t + 1] = tail + 1;
+            }
+        }
+    }
+    // groups per layer: each sub-layer, suffix photons r = 0..kf
+    std::vector<uint32_t> hert;   // (herald index, parent sub-layer offset) pairs
+    lay.groups.assign((size_t)lay.nlayers + 1, {});
+    lay.grid.assign((size_t)lay.nlayers + 1, 0);
+    for (int k = 1; k <= lay.nlayers; k++) {
+        uint64_t wg = 0;
+        for (const BlkSub &sub : subs[(size_t)k]) {
+            if (sub.kf < 0 || sub.kf > nmax) { set_error("block SLOS layout: sub-layer photons outside the tables"); return ERR_ARG; }
+            const uint32_t her_off = (uint32_t)(hert.size() / 2);
+            for (const auto &e : sub.her) {
+                if ((int)e.first >= lay.mloc - m) {
+                    set_error("block SLOS layout: herald index outside the row map");
+                    return ERR_ARG;
+                }
+                hert.push_back(e.first);
+                hert.push_back(e.second);
+            }
+            for (int r = 0; r <= sub.kf; r++) {
+                BlkGroup g{};
+                g.nb = (uint32_t)nstates(h, sub.kf - r);
+                g.S = (uint32_t)nstates(w, r);
+                if (g.S >= BLK_A_MIN) {
+                    g.nch = (g.S + 63) / 64;
+                    g.nitems = g.nb * g.nch;
+                } else {
+                    g.nch = 0;
+                    g.nitems = (uint32_t)(((uint64_t)g.nb * g.S + 63) / 64);
+                }
+                g.nwg = (((g.nitems + 3) / 4) + 7) / 8 * 8;
+                g.wg0 = (uint32_t)wg;
+                g.tpf = tpf[(size_t)r];
+                g.tslot = tslot[(size_t)r];
+                g.r = (uint32_t)r;
+                g.K = (uint32_t)(sub.kf - r);
+                g.out_base = sub.out_base;
+                g.par_base = sub.par_base;
+                g.her_off = her_off;
+                g.nher = (uint32_t)sub.her.size();
+                wg += g.nwg;
+                lay.groups[(size_t)k].push_back(g);
+            }
+        }
+        if (wg >= (1ull << 31)) { set_error("block SLOS layout: grid too large"); return ERR_RANGE; }
+        lay.grid[(size_t)k] = (uint32_t)wg;
+    }
+    // one device allocation
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t off = 0;
+    lay.off_th = off; off += al(sizeof(uint32_t) * th.size());
+    lay.off_te = off; off += al(sizeof(uint64_t) * te.size());
+    lay.off_cte = off; off += al(sizeof(uint64_t) * cte.size());
+    lay.off_f32 = off; off += al(sizeof(uint32_t) * f32.size());
+    lay.off_fd = off; off += al(sizeof(double) * fd.size());
+    lay.off_row = off; off += al(sizeof(int32_t) * rowmap.size());
+    lay.off_her = off; off += al(sizeof(uint32_t) * std::max<size_t>(hert.size(), 2));
+    lay.group_off.assign((size_t)lay.nlayers + 1, 0);
+    for (int k = 1; k <= lay.nlayers; k++) {
+        lay.group_off[(size_t)k] = off;
+        off += al(sizeof(BlkGroup) * lay.groups[(size_t)k].size());
+    }
+    lay.off_pf = off; off += al(sizeof(double) * spf.size());
+    lay.off_slot = off; off += al(sizeof(uint32_t) * std::max<size_t>(sslot.size(), 1));
+    lay.bytes = off;
+    std::vector<char> &host = *image;
+    host.assign(off, 0);
+    std::memcpy(host.data() + lay.off_th, th.data(), sizeof(uint32_t) * th.size());
+    std::memcpy(host.data() + lay.off_te, te.data(), sizeof(uint64_t) * te.size());
+    std::memcpy(host.data() + lay.off_cte, cte.data(), sizeof(uint64_t) * cte.size());
+    std::memcpy(host.data() + lay.off_f32, f32.data(), sizeof(uint32_t) * f32.size());
+    std::memcpy(host.data() + lay.off_fd, fd.data(), sizeof(double) * fd.size());
+    std::memcpy(host.data() + lay.off_row, rowmap.data(), sizeof(int32_t) * rowmap.size());
+    if (!hert.empty()) std::memcpy(host.data() + lay.off_her, hert.data(), sizeof(uint32_t) * hert.size());
+    for (int k = 1; k <= lay.nlayers; k++)
+        std::memcpy(host.data() + lay.group_off[(size_t)k], lay.groups[(size_t)k].data(),
+                    sizeof(BlkGroup) * lay.groups[(size_t)k].size());
+    std::memcpy(host.data() + lay.off_pf, spf.data(), sizeof(double) * spf.size());
+    if (!sslot.empty()) std::memcpy(host.data() + lay.off_slot, sslot.data(), sizeof(uint32_t) * sslot.size());
+    *out = std::move(lay);
+    return OK;
+}
+
+int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<char> *image)
+{
+    if (!blk_layout_ok(m, nmax)) { set_error("block SLOS layout: layers must hold fewer than 2^32 states"); return ERR_ARG; }
+    std::vector<std::vector<BlkSub>> subs((size_t)nmax + 1);
+    std::vector<uint64_t> counts;
+    for (int k = 0; k <= nmax; k++) {
+        counts.push_back(nstates(m, k));
+        if (k >= 1) {
+            BlkSub sub;
+            sub.kf = k;
+            subs[(size_t)k].push_back(sub);
+        }
+    }
+    std::vector<int32_t> rowmap((size_t)m);
+    for (int i = 0; i < m; i++) rowmap[(size_t)i] = i;
+    return blk_layout_build_host_ex(m, nmax, w, subs, counts, rowmap, out, image);
+}
+
+int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out)
+{
+    BlkMasked mk;
+    std::vector<int32_t> H, c;
+    for (int j = 0; j < m; j++) {
+        if (cond[j] >= 0) {
+            H.push_back(j);
+            c.push_back(cond[j]);
+        } else {
+            mk.rowmap.push_back(j);
+        }
+    }
+    mk.m_free = (int)mk.rowmap.size();
+    mk.nher = (int)H.size();
+    for (int32_t j : H) mk.rowmap.push_back(j);
+    int C = 0;
+    double hf = 1.0;
+    uint64_t configs = 1;
+    for (int32_t v : c) {
+        C += v;
+        for (int t = 2; t <= v; t++) hf *= t;
+        configs *= (uint64_t)std::min(v, n) + 1;
+        if (configs > 1000000) { set_error("masked SLOS: more than 10^6 herald configurations"); return ERR_RANGE; }
+    }
+    mk.herald_fact = hf;
+    mk.subs.assign((size_t)n + 1, {});
+    mk.counts.assign((size_t)n + 1, 0);
+    if (C > n) {
+        mk.empty = true;
+        *out = std::move(mk);
+        return OK;
+    }
+    mk.nmax_free = n - C;
+    // every herald configuration a <= c (componentwise, capped at n), in
+    // descending lexicographic order
+    std::vector<std::vector<int32_t>> all;
+    {
+        std::vector<int32_t> a(H.size(), 0);
+        for (size_t i = 0; i < H.size(); i++) a[i] = std::min(c[i], n);
+        while (true) {
+            all.push_back(a);
+            // next in descending lexicographic order of a box: decrement the
+            // last coordinate that is above 0, reset the ones after it
+            int i = (int)a.size() - 1;
+            while (i >= 0 && a[(size_t)i] == 0) i--;
+            if (i < 0) break;
+            a[(size_t)i]--;
+            for (size_t t = (size_t)i + 1; t < a.size(); t++) a[t] = std::min(c[t], n);
+        }
+    }
+    auto size_of = [](const std::vector<int32_t> &a) { int s = 0; for (int32_t v : a) s += v; return s; };
+    std::vector<std::map<std::vector<int32_t>, uint32_t>> base((size_t)n + 1);
+    for (int k = 0; k <= n; k++) {
+        uint64_t off = 0;
+        for (const auto &a : all) {
+            const int sa = size_of(a);
+            if (sa > k || k - sa > n - C) continue;
+            base[(size_t)k][a] = (uint32_t)off;
+            off += nstates(mk.m_free, k - sa);
+            if (off >= (1ull << 32)) { set_error("masked SLOS: a layer holds 2^32 states or more"); return ERR_RANGE; }
+        }
+        mk.counts[(size_t)k] = off;
+        if (k == 0) continue;
+        for (const auto &a : all) {
+            auto it = base[(size_t)k].find(a);
+            if (it == base[(size_t)k].end()) continue;
+            BlkSub sub;
+            sub.out_base = it->second;
+            sub.kf = k - size_of(a);
+            if (sub.kf >= 1) sub.par_base = base[(size_t)k - 1].at(a);
+            for (size_t i = 0; i < a.size(); i++) {
+                if (a[i] == 0) continue;
+                auto b = a;
+                b[i]--;
+                sub.her.push_back({(uint32_t)i, base[(size_t)k - 1].at(b)});
+            }
+            mk.subs[(size_t)k].push_back(std::move(sub));
+        }
+    }
+    *out = std::move(mk);
+    return OK;
+}
+
+int blk_layout_upload(BlkLayout *lay, const std::vector<char> &image)
+{
+    hipError_t e = hipMalloc(&lay->d_mem, lay->bytes);
+    if (e != hipSuccess) {
+        lay->d_mem = nullptr;
+        set_error(std::string("hipMalloc(block SLOS tables): ") + hipGetErrorString(e));
+        return ERR_ALLOC;
+    }
+    e = hipMemcpy(lay->d_mem, image.data(), lay->bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(lay->d_mem);
+        lay->d_mem = nullptr;
+        set_error(std::string("hipMemcpy(block SLOS tables): ") + hipGetErrorString(e));
+        return ERR_HIP;
+    }
+    return OK;
+}
+
+int blk_layout_create(int m, int nmax, int w, BlkLayout *out)
+{
+    BlkLayout lay;
+    std::vector<char> host;
+    int rc = blk_layout_build_host(m, nmax, w, &lay, &host);
+    if (rc == OK) rc = blk_layout_upload(&lay, host);
+    if (rc != OK) return rc;
+    *out = std::move(lay);
+    return OK;
+}
+
+void blk_layout_destroy(BlkLayout *lay)
+{
+    if (lay->d_mem) (void)hipFree(lay->d_mem);
+    lay->d_mem = nullptr;
+}
+
+int blk_layer_launch(const BlkLayout &lay, const cd *parent, void *out, const cd *U, int m, int mk, int k, bool probs,
+                     double inv_prod_in, bool sentinel, hipStream_t stream)
+{
+    if (k < 1 || k > lay.nlayers || !lay.d_mem) { set_error("block SLOS: layer outside the layout"); return ERR_ARG; }
+    if (lay.grid[(size_t)k] == 0) return OK;
+    const char *base = lay.d_mem;
+    const BlkGroup *groups = reinterpret_cast<const BlkGroup *>(base + lay.group_off[(size_t)k]);
+    const uint32_t nchild = (uint32_t)lay.counts[(size_t)k];
+    const size_t lds = sizeof(cd) * lay.mloc + (sizeof(uint32_t) + 1) * (size_t)k * BLK_THREADS;
+    const bool pfi = lay.nmax <= 12;
+    auto kp = probs ? (pfi ? slos_block_kernel<1, true> : slos_block_kernel<1, false>)
+                    : (pfi ? slos_block_kernel<0, true> : slos_block_kernel<0, false>);
+    hipLaunchKernelGGL(kp, dim3(lay.grid[(size_t)k]), dim3(BLK_THREADS), lds, stream, parent, out, U, m, mk, k, nchild,
+                       lay.h, lay.w, lay.m, lay.mloc, reinterpret_cast<const int32_t *>(base + lay.off_row), groups,
+                       (int)lay.groups[(size_t)k].size(), reinterpret_cast<const uint2 *>(base + lay.off_her),
+                       reinterpret_cast<const uint32_t *>(base + lay.off_th),
+                       reinterpret_cast<const uint64_t *>(base + lay.off_te),
+                       reinterpret_cast<const uint64_t *>(base + lay.off_cte), lay.wt,
+                       reinterpret_cast<const uint32_t *>(base + lay.off_f32),
+                       reinterpret_cast<const double *>(base + lay.off_fd),
+                       reinterpret_cast<const double *>(base + lay.off_pf),
+                       reinterpret_cast<const uint32_t *>(base + lay.off_slot), inv_prod_in, sentinel ? 1 : 0);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+} // namespace pcvl

@@ -55,29 +55,6 @@ def _rank(state) -> int:
     return r
 
 
-class _SinglePermanents:
-    """One permanent per call without a plan or device allocation per call:
-    a plan and device buffers kept per order."""
-
-    def __init__(self):
-        self._by_order = {}
-
-    def __call__(self, M) -> complex:
-        import torch
-        M = np.ascontiguousarray(np.asarray(M, dtype=np.complex128))
-        n = M.shape[0]
-        if n not in self._by_order:
-            plan = _permanent.PermanentPlan([n])
-            d_in = torch.empty(max(1, n * n), dtype=torch.complex128, device="cuda")
-            d_out = torch.empty(1, dtype=torch.complex128, device="cuda")
-            self._by_order[n] = (plan, d_in, d_out)
-        plan, d_in, d_out = self._by_order[n]
-        if n:
-            d_in[:n * n].copy_(torch.from_numpy(M.ravel()))
-        plan(d_in, d_out)
-        return complex(d_out.cpu().numpy()[0])
-
-
 def _default_types():
     import perceval.utils as pu
     return _types.SimpleNamespace(FockState=pu.FockState, BSDistribution=pu.BSDistribution,
@@ -226,7 +203,7 @@ def make_backends(backends_module, types=None) -> dict:
     class MI355XNaiveBackend(_GpuStates, NaiveBase):
         def __init__(self, *args, **kwargs):
             self._gpu_cache = {}
-            self._single = _SinglePermanents()
+            self._single = _permanent.SinglePermanents()
             NaiveBase.__init__(self, *args, **kwargs)
 
         def _compute_permanent(self, M):

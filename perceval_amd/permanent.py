@@ -96,9 +96,48 @@ def permanents(mats, device="cuda") -> np.ndarray:
         plan.close()
 
 
+class SinglePermanents:
+    """One permanent per call with no plan or device allocation per call: a
+    plan and device buffers are kept per order (the per-output path of the
+    Naive backend, _naive.py:46-49,70-71)."""
+
+    def __init__(self, device: str = "cuda"):
+        self.device = device
+        self._by_order = {}
+
+    def __call__(self, M) -> complex:
+        import torch
+        M = np.ascontiguousarray(np.asarray(M, dtype=np.complex128))
+        n = M.shape[0]
+        if M.shape != (n, n):
+            raise ValueError("permanent of a non-square matrix")
+        if n not in self._by_order:
+            plan = PermanentPlan([n])
+            d_in = torch.empty(max(1, n * n), dtype=torch.complex128, device=self.device)
+            d_out = torch.empty(1, dtype=torch.complex128, device=self.device)
+            self._by_order[n] = (plan, d_in, d_out)
+        plan, d_in, d_out = self._by_order[n]
+        if n:
+            d_in[:n * n].copy_(torch.from_numpy(M.ravel()))
+        plan(d_in, d_out)
+        return complex(d_out.cpu().numpy()[0])
+
+    def close(self):
+        for plan, _, _ in self._by_order.values():
+            plan.close()
+        self._by_order = {}
+
+
+_single = None
+
+
 def permanent(M) -> complex:
-    """Permanent of one square complex matrix, computed on the GPU."""
-    return complex(permanents([np.asarray(M, dtype=np.complex128)])[0])
+    """Permanent of one square complex matrix, computed on the GPU (cached
+    per-order plans, see SinglePermanents)."""
+    global _single
+    if _single is None:
+        _single = SinglePermanents()
+    return _single(M)
 
 
 def gurvits(mats, iterations: int, seed: int = 0, stream0: int = 0):

@@ -20,13 +20,29 @@ PROBS = 1
 
 
 class SlosPlan:
-    def __init__(self, in_occ):
+    """mask_cond: optional per-mode conditions (-1 any count, c >= 0 exactly
+    c photons; see mask_conditions).  A masked plan computes only the layers'
+    states that can still match (the masked FSArray of _slos.py:161-164) and
+    returns the matching output states only, in FSArray order."""
+
+    def __init__(self, in_occ, mask_cond=None):
         lib = _native.load()
         self.in_occ = np.ascontiguousarray(np.asarray(list(in_occ), dtype=np.int32))
         self.m = int(self.in_occ.size)
         self._plan = ctypes.c_void_p()
-        _native.check(lib.pcvl_slos_plan_create(self.m, self.in_occ.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                                ctypes.byref(self._plan)), "pcvl_slos_plan_create")
+        self.mask_cond = None
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        if mask_cond is None:
+            _native.check(lib.pcvl_slos_plan_create(self.m, self.in_occ.ctypes.data_as(i32p), ctypes.byref(self._plan)),
+                          "pcvl_slos_plan_create")
+        else:
+            cond = np.ascontiguousarray(np.asarray(list(mask_cond), dtype=np.int32))
+            if cond.size != self.m:
+                raise ValueError("mask conditions must have one entry per mode")
+            self.mask_cond = cond
+            _native.check(lib.pcvl_slos_plan_create_masked(self.m, self.in_occ.ctypes.data_as(i32p),
+                                                           cond.ctypes.data_as(i32p), ctypes.byref(self._plan)),
+                          "pcvl_slos_plan_create_masked")
         self._lib = lib
         n, count, work = ctypes.c_int(), ctypes.c_int64(), ctypes.c_size_t()
         path = np.zeros(max(1, int(self.in_occ.sum())), dtype=np.int32)
@@ -39,15 +55,25 @@ class SlosPlan:
         self.path = path[:self.n].tolist()
         self._work = None
 
+    # layer kernels accepted by set_kernel
+    KERNELS = (0, 1, 4, 6, 8)
+
     def set_kernel(self, kernel: int):
-        """0 = pick, 1 = one thread per output state, 2 = incremental runs,
-        3 = one thread per output state with 32-bit ranks, 4 = one thread per
-        output state, walk and gather in two phases, 5 = the same with eight
-        loads in flight, 6 = runs of 3 consecutive states per thread stepped
-        by successor, gathered lane-consecutively (compact slot lists; the
-        default where it fits), 7 = the same with runs of 2 and full slot
-        lists."""
+        """0 = pick (8 where its layout fits, else 6, else 4, else 1),
+        1 = one thread per output state, 4 = one thread per output state, walk
+        and gather in two phases, 6 = runs of 3 consecutive states per thread
+        stepped by successor and gathered lane-consecutively, 8 = blocks of
+        states sharing their first m - w modes: one scalar walk per block,
+        the last w modes from a table.  All give bit-identical results."""
         _native.check(self._lib.pcvl_slos_plan_set_kernel(self._plan, int(kernel)), "pcvl_slos_plan_set_kernel")
+
+    def set_block_width(self, w: int):
+        """Suffix width of kernel 8 (0 = default).  Results do not depend on it."""
+        _native.check(self._lib.pcvl_slos_plan_set_block_width(self._plan, int(w)), "pcvl_slos_plan_set_block_width")
+
+    @property
+    def block_width(self) -> int:
+        return int(self._lib.pcvl_slos_plan_block_width(self._plan))
 
     def _workspace(self, device):
         import torch
@@ -68,9 +94,11 @@ class SlosPlan:
             raise ValueError(f"unitary must be {self.m} x {self.m}")
         dtype = torch.complex128 if mode == COEFS else torch.float64
         if out is None:
-            out = torch.empty(self.count, dtype=dtype, device=dU.device)
+            out = torch.empty(max(1, self.count), dtype=dtype, device=dU.device)[:self.count]
         elif out.dtype != dtype or out.numel() < self.count or not out.is_contiguous():
             raise ValueError("out has the wrong dtype, size or layout")
+        if self.count == 0:
+            return out
         work = self._workspace(dU.device)
         _native.check(self._lib.pcvl_slos_plan_execute(self._plan, _native.ptr(dU), mode, _native.ptr(out),
                                                        _native.ptr(work), work.numel(),

@@ -1,0 +1,305 @@
+// Host-side check of the block SLOS layout (perceval_amd/csrc/slos_block.hip),
+// built with AddressSanitizer and UndefinedBehaviorSanitizer on the host code.
+//
+// For many (m, n, w) it builds the layout's tables with blk_layout_build_host
+// and replays slos_block_kernel's indexing on the CPU for every workgroup,
+// wave and lane (the group search, the XCD-contiguous tile order, both work
+// regimes, the prefix walk, the suffix slot tables), then checks:
+//   * every table, parent and output index is in bounds;
+//   * every child of every layer is written exactly once;
+//   * each child's parent list, in the order it is summed, equals brute force
+//     (rank of c - e_j for the occupied modes j, increasing);
+//   * the factorial product of each child equals prod s!.
+// The replay restates the kernel's index arithmetic line for line; it must be
+// kept in step with the kernel.  Exit status 0 when every check holds.
+#include "../../perceval_amd/csrc/slos_block.hip"
+#include "../../perceval_amd/csrc/capi.cpp"
+#include <cstdio>
+#include <cstdlib>
+#include <algorithm>
+#include <map>
+
+using namespace pcvl;
+
+static std::vector<std::vector<int>> space(int m, int n)
+{
+    std::vector<std::vector<int>> out;
+    std::vector<int> s((size_t)m, 0);
+    s[0] = n;
+    while (true) {
+        out.push_back(s);
+        if (m == 1) break;
+        int tail = s[(size_t)m - 1];
+        if (tail == n) break;
+        s[(size_t)m - 1] = 0;
+        int t = m - 2;
+        while (s[(size_t)t] == 0) t--;
+        s[(size_t)t]--;
+        s[(size_t)t + 1] = tail + 1;
+    }
+    return out;
+}
+
+static uint64_t rank_of(const std::vector<int> &s)
+{
+    const int m = (int)s.size();
+    int R = 0;
+    for (int v : s) R += v;
+    uint64_t rk = 0;
+    for (int q = 0; q + 1 < m; q++) {
+        R -= s[(size_t)q];
+        rk += blk_binom((int64_t)R + m - q - 2, m - q - 1);
+    }
+    return rk;
+}
+
+static int fails = 0;
+#define CHECK(c, ...) do { if (!(c)) { if (fails++ < 20) { printf("FAIL %s:%d ", __FILE__, __LINE__); printf(__VA_ARGS__); printf("\n"); } } } while (0)
+
+// Replays every layer of a layout.  `layers[k]` lists the states of layer k in
+// the layout's order as occupation vectors over the ORIGINAL modes; `perm`
+// gives the local mode order (free modes, then herald modes) as original
+// mode indices.  Expected parents of a child: its occupied modes in local
+// order, each parent's position in layers[k-1].
+static void replay(const BlkLayout &lay, const std::vector<char> &img, const std::vector<std::vector<std::vector<int>>> &layers,
+                   const std::vector<int32_t> &perm, const char *what)
+{
+    const int h = lay.h, wt = lay.wt, w = lay.w, mfree = lay.m;
+    const uint32_t *Th = (const uint32_t *)(img.data() + lay.off_th);
+    const uint64_t *TE = (const uint64_t *)(img.data() + lay.off_te);
+    const uint64_t *CTE = (const uint64_t *)(img.data() + lay.off_cte);
+    const uint32_t *f32 = (const uint32_t *)(img.data() + lay.off_f32);
+    const double *fd = (const double *)(img.data() + lay.off_fd);
+    const double *spf = (const double *)(img.data() + lay.off_pf);
+    const uint32_t *sslot = (const uint32_t *)(img.data() + lay.off_slot);
+    const int32_t *rowmap = (const int32_t *)(img.data() + lay.off_row);
+    const uint32_t *her = (const uint32_t *)(img.data() + lay.off_her);
+    const size_t n_th = (lay.off_te - lay.off_th) / 4, n_te = (lay.off_cte - lay.off_te) / 8;
+    const size_t n_pf = (lay.off_slot - lay.off_pf) / 8, n_slot = (lay.bytes - lay.off_slot) / 4;
+    const size_t n_cte = (lay.off_f32 - lay.off_cte) / 8, n_her = (lay.group_off[1] - lay.off_her) / 8;
+    for (int e = 0; e < lay.mloc; e++) CHECK(rowmap[e] == perm[(size_t)e], "%s: rowmap[%d]", what, e);
+    for (int k = 1; k <= lay.nlayers; k++) {
+        const auto &children = layers[(size_t)k];
+        const auto &parents = layers[(size_t)k - 1];
+        std::map<std::vector<int>, uint64_t> pidx;
+        for (size_t i = 0; i < parents.size(); i++) pidx[parents[i]] = i;
+        const uint64_t nchild = children.size(), nparent = parents.size();
+        CHECK(lay.counts[(size_t)k] == nchild, "%s: count of layer %d", what, k);
+        std::vector<int> written(nchild, 0);
+        const BlkGroup *groups = (const BlkGroup *)(img.data() + lay.group_off[(size_t)k]);
+        const int ngroups = (int)lay.groups[(size_t)k].size();
+        for (uint32_t wg = 0; wg < lay.grid[(size_t)k]; wg++) {
+            int lo = 0, hi = ngroups - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (groups[mid].wg0 <= wg) lo = mid;
+                else hi = mid - 1;
+            }
+            const BlkGroup g = groups[lo];
+            CHECK(wg >= g.wg0 && wg < g.wg0 + g.nwg, "%s: wg %u outside group %d", what, wg, lo);
+            const uint32_t gl = wg - g.wg0, t8 = g.nwg >> 3;
+            const uint32_t gx = (gl & 7u) * t8 + (gl >> 3);
+            for (uint32_t wave = 0; wave < 4; wave++) {
+                const uint32_t item = gx * 4 + wave;
+                if (item >= g.nitems) continue;
+                const int r = (int)g.r, K = (int)g.K, smax = std::min(r, w);
+                const uint32_t S = g.S;
+                for (uint32_t lane = 0; lane < 64; lane++) {
+                    uint32_t b, beta;
+                    if (g.nch != 0) {
+                        b = item / g.nch;
+                        beta = (item - b * g.nch) * 64u + lane;
+                    } else {
+                        const uint32_t i = item * 64u + lane;
+                        if (i >= g.nb * S) continue;
+                        b = i / S;
+                        beta = i - b * S;
+                    }
+                    CHECK(b < g.nb, "%s: block %u >= nb %u", what, b, g.nb);
+                    uint32_t rem = b, PE = 0, Bk = 0;
+                    int P = K;
+                    double pfp = 1;
+                    std::vector<std::pair<uint32_t, int>> lst;
+                    for (int q = 0; q < h; q++) {
+                        int x = 0;
+                        if (q < h - 1) {
+                            x = P;
+                            CHECK((size_t)(q * wt + x) < n_th, "%s: Th index", what);
+                            uint32_t t = Th[q * wt + x];
+                            while (t > rem) { --x; CHECK(x >= 0, "%s: walk below 0", what); t = Th[q * wt + x]; }
+                            rem -= t;
+                        }
+                        CHECK((size_t)(q * wt + x + r) < n_te, "%s: TE index", what);
+                        const uint64_t te = TE[q * wt + x + r];
+                        Bk += (uint32_t)te;
+                        if (P > x) {
+                            lst.push_back({PE, q});
+                            pfp *= (lay.nmax <= 12) ? (double)f32[P - x] : fd[P - x];
+                        }
+                        PE += (uint32_t)(te >> 32);
+                        P = x;
+                        if (P == 0) {
+                            CHECK((size_t)(r * (h + 1) + q + 1) < n_cte, "%s: CTE index", what);
+                            const uint64_t ce = CTE[r * (h + 1) + q + 1];
+                            Bk += (uint32_t)ce;
+                            PE += (uint32_t)(ce >> 32);
+                            break;
+                        }
+                    }
+                    if (beta >= S) continue;
+                    const uint64_t loc = (uint64_t)Bk + beta;
+                    const uint64_t o = (uint64_t)g.out_base + loc;
+                    CHECK(o < nchild, "%s: out %lu >= %lu", what, (unsigned long)o, (unsigned long)nchild);
+                    if (o >= nchild) continue;
+                    written[o]++;
+                    std::vector<std::pair<uint64_t, int>> got;   // (parent index, original mode)
+                    for (uint32_t d = 0; d < g.nher; d++) {
+                        CHECK(g.her_off + d < n_her, "%s: herald index", what);
+                        const uint32_t hi2 = her[2 * (g.her_off + d)], hb = her[2 * (g.her_off + d) + 1];
+                        CHECK(mfree + (int)hi2 < lay.mloc, "%s: herald row", what);
+                        got.push_back({(uint64_t)hb + loc, rowmap[mfree + (int)hi2]});
+                    }
+                    for (auto &e : lst) got.push_back({(uint64_t)g.par_base + loc - e.first, rowmap[e.second]});
+                    for (int d = 0; d < smax; d++) {
+                        const size_t ti = (size_t)g.tslot + (size_t)d * S + beta;
+                        CHECK(ti < n_slot, "%s: slot index", what);
+                        const uint32_t sw = sslot[ti];
+                        if (sw == 0xffffffffu) continue;
+                        got.push_back({(uint64_t)g.par_base + (Bk - PE) + (sw >> 5), rowmap[h + (int)(sw & 31u)]});
+                    }
+                    CHECK((size_t)g.tpf + beta < n_pf, "%s: pf index", what);
+                    const double pf = pfp * spf[g.tpf + beta];
+                    // brute force, local mode order
+                    const auto &c = children[o];
+                    std::vector<std::pair<uint64_t, int>> exp;
+                    double pfe = 1;
+                    for (int e = mfree; e < lay.mloc; e++) {
+                        const int j = perm[(size_t)e];
+                        if (c[(size_t)j] > 0) {
+                            auto cc = c;
+                            cc[(size_t)j]--;
+                            auto it = pidx.find(cc);
+                            CHECK(it != pidx.end(), "%s: parent missing from layer %d", what, k - 1);
+                            exp.push_back({it == pidx.end() ? ~0ull : it->second, j});
+                        }
+                    }
+                    for (int e = 0; e < mfree; e++) {
+                        const int j = perm[(size_t)e];
+                        for (int v = 2; v <= c[(size_t)j]; v++) pfe *= v;
+                        if (c[(size_t)j] > 0) {
+                            auto cc = c;
+                            cc[(size_t)j]--;
+                            auto it = pidx.find(cc);
+                            CHECK(it != pidx.end(), "%s: parent missing from layer %d", what, k - 1);
+                            exp.push_back({it == pidx.end() ? ~0ull : it->second, j});
+                        }
+                    }
+                    CHECK(got == exp, "%s: k=%d r=%d b=%u beta=%u: parent list differs", what, k, r, b, beta);
+                    for (auto &e : got) CHECK(e.first < nparent, "%s: parent %lu >= %lu", what, (unsigned long)e.first, (unsigned long)nparent);
+                    CHECK(pf == pfe, "%s: pf %g vs %g", what, pf, pfe);
+                }
+            }
+        }
+        for (uint64_t i = 0; i < nchild; i++)
+            CHECK(written[i] == 1, "%s: layer %d child %lu written %d times", what, k, (unsigned long)i, written[i]);
+    }
+}
+
+static void emulate(int m, int nmax, int w)
+{
+    BlkLayout lay;
+    std::vector<char> img;
+    if (blk_layout_build_host(m, nmax, w, &lay, &img) != OK) { printf("build failed %s\n", pcvl_last_error()); exit(1); }
+    std::vector<std::vector<std::vector<int>>> layers;
+    for (int k = 0; k <= nmax; k++) layers.push_back(space(m, k));
+    std::vector<int32_t> perm((size_t)m);
+    for (int j = 0; j < m; j++) perm[(size_t)j] = j;
+    char what[64];
+    snprintf(what, sizeof what, "m=%d n=%d w=%d", m, nmax, w);
+    replay(lay, img, layers, perm, what);
+}
+
+// Masked: cond[j] >= 0 fixes mode j.  The expected layer k holds the states
+// with s_H <= c and k - |s_H| <= n - |c|, in descending lexicographic order
+// of (herald modes, free modes).
+static void emulate_masked(int m, int n, const std::vector<int32_t> &cond, int w)
+{
+    BlkMasked mk;
+    if (blk_masked_layers(m, n, cond.data(), &mk) != OK) { printf("masked failed %s\n", pcvl_last_error()); exit(1); }
+    char what[96];
+    snprintf(what, sizeof what, "masked m=%d n=%d w=%d", m, n, w);
+    int C = 0;
+    for (int32_t v : cond) if (v >= 0) C += v;
+    CHECK(mk.empty == (C > n), "%s: empty flag", what);
+    if (mk.empty) return;
+    std::vector<int32_t> perm = mk.rowmap;
+    std::vector<std::vector<std::vector<int>>> layers;
+    for (int k = 0; k <= n; k++) {
+        std::vector<std::pair<std::vector<int>, std::vector<int>>> kept;   // (local order key, state)
+        for (const auto &s : space(m, k)) {
+            int sa = 0;
+            bool ok = true;
+            for (int j = 0; j < m; j++)
+                if (cond[(size_t)j] >= 0) {
+                    sa += s[(size_t)j];
+                    if (s[(size_t)j] > cond[(size_t)j]) ok = false;
+                }
+            if (!ok || k - sa > n - C) continue;
+            std::vector<int> key;
+            for (int e = mk.m_free; e < (int)perm.size(); e++) key.push_back(s[(size_t)perm[(size_t)e]]);
+            for (int e = 0; e < mk.m_free; e++) key.push_back(s[(size_t)perm[(size_t)e]]);
+            kept.push_back({key, s});
+        }
+        std::sort(kept.begin(), kept.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::vector<std::vector<int>> L;
+        for (auto &e : kept) L.push_back(e.second);
+        CHECK(mk.counts[(size_t)k] == L.size(), "%s: count of layer %d: %lu vs %zu", what, k,
+              (unsigned long)mk.counts[(size_t)k], L.size());
+        layers.push_back(L);
+    }
+    BlkLayout lay;
+    std::vector<char> img;
+    if (blk_layout_build_host_ex(mk.m_free, mk.nmax_free, w, mk.subs, mk.counts, mk.rowmap, &lay, &img) != OK) {
+        printf("masked build failed %s\n", pcvl_last_error());
+        exit(1);
+    }
+    replay(lay, img, layers, perm, what);
+}
+
+int main()
+{
+    int cases = 0;
+    for (int m = 2; m <= 9; m++)
+        for (int n = 1; n <= 7; n++)
+            for (int w = 1; w < m && w <= 16; w++) {
+                if (nstates(m, n) > 200000) continue;
+                emulate(m, n, w);
+                cases++;
+            }
+    emulate(12, 6, 10); cases++;
+    emulate(14, 9, 10); cases++;
+    emulate(24, 5, 10); cases++;
+    emulate(24, 6, 8); cases++;
+    emulate(20, 6, 12); cases++;
+    emulate(3, 14, 2); cases++;
+    emulate(8, 13, 7); cases++;
+    // masks: heralds anywhere among the modes, several counts, empty masks
+    const std::vector<std::pair<int, std::vector<int32_t>>> masks = {
+        {6, {-1, -1, -1, -1, 1, 1}}, {6, {1, -1, 0, -1, -1, -1}}, {7, {-1, 2, -1, -1, 0, -1, 1}},
+        {8, {0, -1, -1, 1, -1, -1, -1, -1}}, {5, {-1, -1, -1, 3, -1}}, {9, {-1, 1, -1, 1, -1, 1, -1, -1, -1}},
+        {6, {2, 2, -1, -1, -1, -1}}, {7, {-1, -1, 4, -1, -1, -1, -1}}};
+    for (const auto &mm : masks) {
+        const int m = mm.first;
+        int nfree = 0;
+        for (int32_t v : mm.second) nfree += v < 0;
+        for (int n = 0; n <= 6; n++)
+            for (int w = 1; w < nfree && w <= 16; w++) {
+                if (n == 0) continue;
+                emulate_masked(m, n, mm.second, w);
+                cases++;
+            }
+    }
+    emulate_masked(16, 7, {1, -1, -1, -1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0}, 8); cases++;
+    printf("%d cases, %d failures\n", cases, fails);
+    return fails ? 1 : 0;
+}

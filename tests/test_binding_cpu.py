@@ -107,7 +107,7 @@ def bound(monkeypatch):
     monkeypatch.setattr(slos, "fock_mask_select", _fake_mask_select)
     monkeypatch.setattr(slos, "gather_f64", lambda src, idx: src[idx])
     monkeypatch.setattr(permanent, "PermanentPlan", _FakePermPlan)
-    monkeypatch.setattr(perceval_binding, "_SinglePermanents", _FakeSingle)
+    monkeypatch.setattr(permanent, "SinglePermanents", _FakeSingle)
     monkeypatch.setattr(sampling, "cc_samples", lambda U, inp, count, seed, first_sample=0:
                         oracle.cc_samples(np.asarray(U), inp, count, seed=seed % (2 ** 63)))
     monkeypatch.setattr(torch.Tensor, "cuda", lambda self, *a, **k: self)

@@ -44,9 +44,8 @@ def test_fock_rank_unrank_m24_n10_sampled(gpu):
     assert (bad == -1).all()
 
 
-KERNELS = (1, 2, 3, 4, 5, 6, 7)   # per state; incremental runs; 32-bit ranks; two-phase gather (4 / 8 loads
-#                                   in flight); successor runs of 3 (compact slots) / 2 (full slots) with a
-#                                   transposed gather
+KERNELS = (1, 4, 6, 8)   # per state; two-phase gather; successor runs of 3 with a transposed gather;
+#                          prefix/suffix blocks
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -104,37 +103,75 @@ def test_slos_kernels_agree(gpu, inp):
     ref = oracle.slos_coefs(U, inp)
     for kernel in KERNELS:
         assert np.max(np.abs(got[kernel] - ref)) < ABS_TOL, kernel
-        assert np.max(np.abs(got[kernel] - got[1])) < 1e-14, kernel
-    assert np.array_equal(got[1], got[3])   # same walk, same summation order
-    assert np.max(np.abs(got[1] - got[4])) == 0.0 and np.max(np.abs(got[1] - got[5])) == 0.0
-    for kernel in (6, 7):
-        assert np.array_equal(got[4], got[kernel]), kernel   # same slots and summation order
+    for kernel in KERNELS:   # same parents, same summation order
+        assert np.array_equal(got[4], got[kernel]), kernel
     plan.close()
 
 
 @pytest.mark.parametrize("inp", [[1] * 8 + [0] * 8, [2, 0, 3, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0]])
-def test_slos_successor_kernels_bit_identical(gpu, inp):
-    """The successor kernels (6, 7) against the two-phase kernel, on every
-    coefficient and probability of a 490,314-state layer (a last tile of 10
-    states, successor runs cut by the end of the layer)."""
+def test_slos_successor_and_block_kernels_bit_identical(gpu, inp):
+    """The successor (6) and block (8) kernels against the two-phase kernel,
+    on every coefficient and probability of a 490,314-state layer (a last
+    tile of 10 states, successor runs cut by the end of the layer)."""
     from perceval_amd.slos import COEFS, PROBS, SlosPlan
     U = haar_unitary(16, seed=7)
     plan = SlosPlan(inp)
     assert plan.count == 490314 and plan.count % 64 == 10
     out = {}
-    for kernel in (4, 6, 7):
+    for kernel in (4, 6, 8):
         plan.set_kernel(kernel)
         out[kernel] = (plan.run(U, COEFS).cpu().numpy(), plan.run(U, PROBS).cpu().numpy())
-    for kernel in (6, 7):
+    for kernel in (6, 8):
         assert np.array_equal(out[4][0], out[kernel][0]) and np.array_equal(out[4][1], out[kernel][1]), kernel
     plan.close()
+
+
+@pytest.mark.parametrize("inp", [[1] * 6 + [0] * 6, [1, 1, 0, 2, 0, 0, 1, 0, 3, 0, 0, 1], [0] * 11 + [7],
+                                 [3, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1], [9, 5, 0], [4, 3]])
+def test_slos_block_kernel_every_width(gpu, inp):
+    """Kernel 8 at every suffix width (1 .. min(m-1, 16)) is bit-identical to
+    the two-phase kernel, coefficients and probabilities: regimes with one
+    block per wave (uniform walk) and several blocks per wave (per-lane walk),
+    chunks cut by the end of a block, groups of every suffix photon count."""
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan
+    m = len(inp)
+    U = haar_unitary(m, seed=3 * m + sum(inp))
+    plan = SlosPlan(inp)
+    plan.set_kernel(4)
+    ref = (plan.run(U, COEFS).cpu().numpy(), plan.run(U, PROBS).cpu().numpy())
+    assert np.max(np.abs(ref[1] - oracle.slos_probs(U, inp, "gather"))) < ABS_TOL
+    plan.set_kernel(8)
+    for w in range(1, min(m - 1, 16) + 1):
+        plan.set_block_width(w)
+        assert plan.block_width == w
+        got = (plan.run(U, COEFS).cpu().numpy(), plan.run(U, PROBS).cpu().numpy())
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), w
+    plan.close()
+
+
+def test_slos_block_width_rejects_bad_values(gpu):
+    from perceval_amd.slos import SlosPlan
+    plan = SlosPlan([1, 1, 0, 0])
+    for bad in (-1, 4, 17):
+        with pytest.raises(ValueError):
+            plan.set_block_width(bad)
+    plan.set_block_width(0)
+    assert 1 <= plan.block_width <= 3
+    plan.close()
+    one = SlosPlan([3])            # one mode: no block layout, auto falls back
+    assert one.block_width == 0
+    with pytest.raises(ValueError):
+        one.set_kernel(8)
+    assert one.run(np.eye(1, dtype=complex)).cpu().numpy().tolist() == [1.0]
+    one.close()
 
 
 def test_slos_set_kernel_rejects_bad_values(gpu):
     from perceval_amd.slos import SlosPlan
     plan = SlosPlan([1, 1, 0])
-    with pytest.raises(ValueError):
-        plan.set_kernel(8)
+    for bad in (-1, 2, 3, 5, 7, 9):
+        with pytest.raises(ValueError):
+            plan.set_kernel(bad)
     plan.close()
 
 
@@ -297,3 +334,79 @@ def test_fock_mask_select_m24_n10_heralds(gpu):
         s = oracle.fock_unrank(int(idx[int(k)].item()), 24, 10)
         ref = abs(oracle.perm(oracle.submatrix(U, inp, s))) ** 2 / oracle.prodnfact(s)
         assert abs(kept[k] - ref) < ABS_TOL
+
+
+@pytest.mark.parametrize("m,inp,cond", [
+    (6, [1, 1, 1, 1, 0, 0], [-1, -1, -1, -1, 1, 1]),
+    (6, [1, 0, 1, 0, 1, 1], [1, -1, 0, -1, -1, -1]),
+    (7, [2, 0, 1, 0, 1, 0, 1], [-1, 2, -1, -1, 0, -1, 1]),
+    (8, [1, 1, 1, 1, 1, 0, 0, 0], [0, -1, -1, 1, -1, -1, -1, -1]),
+    (9, [1, 1, 1, 0, 1, 1, 0, 0, 1], [-1, 1, -1, 1, -1, 1, -1, -1, -1]),
+    (6, [1, 1, 1, 0, 0, 0], [2, 2, -1, -1, -1, -1]),      # heralds need 4 of 3 photons: empty
+    (12, [1] * 6 + [0] * 6, [-1] * 10 + [3, 3]),         # every photon heralded: one output
+    (16, [1] * 8 + [0] * 8, [1, -1, -1, -1, -1, 1] + [-1] * 9 + [0]),
+])
+def test_slos_masked_plan_vs_full(gpu, m, inp, cond):
+    """A plan restricted to a herald mask against the full plan's output
+    filtered by the same mask, every kept state, 1e-12 absolute: layers that
+    drop the states which cannot reach the mask, a different summation order
+    (herald modes first), kept outputs in FSArray order."""
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan
+    U = haar_unitary(m, seed=5 * m + sum(cond))
+    full = SlosPlan(inp)
+    states = oracle.fock_enumerate(m, sum(inp))
+    keep = np.array([all(c < 0 or s[j] == c for j, c in enumerate(cond)) for s in states], dtype=bool)
+    pf = full.run(U, PROBS).cpu().numpy()[keep]
+    cf = full.run(U, COEFS).cpu().numpy()[keep]
+    plan = SlosPlan(inp, mask_cond=cond)
+    assert plan.count == int(keep.sum())
+    if plan.count == 0:
+        assert plan.run(U, PROBS).numel() == 0
+        return
+    free = [j for j in range(m) if cond[j] < 0]
+    for w in range(1, min(len(free) - 1, 16) + 1):
+        plan.set_block_width(w)
+        p = plan.run(U, PROBS).cpu().numpy()
+        c = plan.run(U, COEFS).cpu().numpy()
+        assert np.max(np.abs(p - pf)) < ABS_TOL, w
+        assert np.max(np.abs(c - cf)) < ABS_TOL, w
+    assert np.max(np.abs(pf - oracle.slos_probs(U, inp)[keep])) < ABS_TOL
+    plan.close()
+    full.close()
+
+
+def test_slos_masked_plan_m24_n10_heralds(gpu):
+    """Heralds 0, 1, 0, 1 on modes 20..23 of the m=24, n=10 configuration:
+    the masked plan computes C(27, 8) = 2,220,075 outputs, each within 1e-12
+    of the full plan's probability of the same state."""
+    import torch
+    from perceval_amd.slos import PROBS, SlosPlan, fock_mask_select, gather_f64
+    U = haar_unitary(24, seed=24)
+    inp = [1] * 10 + [0] * 14
+    cond = [-1] * 20 + [0, 1, 0, 1]
+    plan = SlosPlan(inp, mask_cond=cond)
+    assert plan.count == math.comb(27, 8)
+    _say(f"masked plan: {plan.count} outputs, workspace {plan.work_bytes / 2**20:.0f} MiB")
+    got = plan.run(U, PROBS)
+    idx = fock_mask_select(24, 10, [" " * 20 + "0101"])
+    ref = gather_f64(SlosPlan(inp).run(U, PROBS), idx)
+    torch.cuda.synchronize()
+    err = float((got - ref).abs().max().item())
+    _say(f"masked vs full: max abs err {err:.3e}")
+    assert err < ABS_TOL
+    plan.close()
+
+
+def test_slos_masked_plan_rejects_bad_masks(gpu):
+    from perceval_amd.slos import SlosPlan
+    with pytest.raises(ValueError):
+        SlosPlan([1, 1, 0], mask_cond=[-1, 1])            # wrong length
+    with pytest.raises(ValueError):
+        SlosPlan([1, 1, 0], mask_cond=[1, 1, -1])         # one free mode
+    with pytest.raises(ValueError):
+        SlosPlan([1, 1, 0], mask_cond=[-2, -1, -1])       # bad condition
+    plan = SlosPlan([1, 1, 0, 0], mask_cond=[1, -1, -1, -1])
+    with pytest.raises(ValueError):
+        plan.set_kernel(6)
+    plan.set_kernel(0)
+    plan.close()
