@@ -75,6 +75,18 @@ int pcvl_perm_plan_info(pcvl_perm_plan plan, int64_t *nitems, int *grid, int *si
 int pcvl_perm_batch(const double *d_mats, const int32_t *h_n, const int64_t *h_offsets,
                     int64_t count, double *d_out, hipStream_t stream);
 
+/* pcvl_perm_batch without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The plan
+ * (work items, queue cursor, partial sums) goes into a 16 MiB scratch array
+ * that each GPU holds from module load until the process ends
+ * (pcvl_perm_noalloc_max_bytes); a batch whose plan needs more returns -4
+ * before anything runs.  Calls that overlap on one GPU share that array and
+ * race.  Synchronises `stream`; results equal pcvl_perm_batch's bit for bit. */
+int pcvl_perm_batch_noalloc(const double *d_mats, const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                            double *d_out, hipStream_t stream);
+size_t pcvl_perm_noalloc_max_bytes(void);
+
 
 /* ---------------------------------------------------------------------------
  * Fock-space indexing -- replaces xq.FSArray(m, n) iteration order and

@@ -45,6 +45,8 @@ SIGNATURES = {
     "pcvl_perm_plan_destroy": (c_int, [c_void_p]),
     "pcvl_perm_plan_info": (c_int, [c_void_p, P(c_i64), P(c_int), P(c_int)]),
     "pcvl_perm_batch": (c_int, [c_void_p, P(c_i32), P(c_i64), c_i64, c_void_p, c_void_p]),
+    "pcvl_perm_batch_noalloc": (c_int, [c_void_p, P(c_i32), P(c_i64), c_i64, c_void_p, c_void_p]),
+    "pcvl_perm_noalloc_max_bytes": (c_size_t, []),
     "pcvl_fock_count": (c_i64, [c_int, c_int]),
     "pcvl_fock_rank": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
     "pcvl_fock_unrank": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),

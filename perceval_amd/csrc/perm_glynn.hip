@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <mutex>
 #include <vector>
 
 namespace pcvl {
@@ -480,16 +481,24 @@ struct pcvl_perm_plan_s {
 // device); partial-sum slots are int32 on the device.
 static constexpr int64_t PERM_MAX_ITEMS = int64_t(1) << 27;
 
-static int perm_plan_create_impl(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
-                                 int max_chunk_log2, int variant, pcvl_perm_plan *out)
+// Device image of a plan: one block holding items | cursor | partials | wave
+// arrays | small arrays, at these byte offsets.
+struct PermLayout {
+    size_t o_items = 0, o_cursor = 0, o_part = 0, o_widx = 0, o_ws0 = 0, o_wns = 0, o_wn = 0, o_sidx = 0,
+           o_soff = 0, o_sn = 0, total = 0;
+};
+
+// Builds a plan on the host: the scheduling fields of p, the layout of its
+// device block and the block's contents (cursor zero).  No device call but
+// occupancy queries.
+static int perm_plan_build(const int32_t *h_n, const int64_t *h_offsets, int64_t count, int max_chunk_log2,
+                           int variant, PermPlanImpl &p, PermLayout &L, std::vector<char> &host)
 {
-    PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
     PCVL_ARG_CHECK(variant >= 0 && variant <= 4, "variant outside [0, 4]");
     PCVL_ARG_CHECK(count >= 0, "count < 0");
     PCVL_ARG_CHECK(count == 0 || (h_n && h_offsets), "null order/offset array");
     if (max_chunk_log2 <= 0) max_chunk_log2 = 11;
     PCVL_ARG_CHECK(max_chunk_log2 >= 2 && max_chunk_log2 <= 24, "max_chunk_log2 outside [2, 24]");
-    *out = nullptr;
 
     std::vector<PermItem> items;
     std::vector<int64_t> wave_idx, small_idx, small_off;
@@ -562,66 +571,82 @@ static int perm_plan_create_impl(const int32_t *h_n, const int64_t *h_offsets, i
         return ((int64_t)a.n << a.c) > ((int64_t)b.n << b.c);
     });
 
-    auto *plan = new (std::nothrow) pcvl_perm_plan_s();
-    if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
-    PermPlanImpl &p = plan->impl;
     p.count = count;
     p.nitems = (int64_t)items.size();
     p.nwave_mats = (int64_t)wave_idx.size();
     p.nsmall = (int64_t)small_idx.size();
     p.single_n = single_n;
     p.variant = kvariant;
+    p.grid = p.nitems > 0 ? (int)std::min<int64_t>(p.nitems, (int64_t)rw) : 0;
 
-    // One device block: items | cursor | partials | wave arrays | small arrays.
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-    size_t o_items = 0;
-    size_t o_cursor = align(o_items + sizeof(PermItem) * items.size());
-    size_t o_part = align(o_cursor + sizeof(unsigned long long));
-    size_t o_widx = align(o_part + sizeof(cd) * (size_t)slots);
-    size_t o_ws0 = align(o_widx + sizeof(int64_t) * wave_idx.size());
-    size_t o_wns = align(o_ws0 + sizeof(int32_t) * wave_idx.size());
-    size_t o_wn = align(o_wns + sizeof(int32_t) * wave_idx.size());
-    size_t o_sidx = align(o_wn + sizeof(int32_t) * wave_idx.size());
-    size_t o_soff = align(o_sidx + sizeof(int64_t) * small_idx.size());
-    size_t o_sn = align(o_soff + sizeof(int64_t) * small_idx.size());
-    size_t total = align(o_sn + sizeof(int32_t) * small_idx.size());
-    hipError_t e = hipMalloc(&p.d_block, total);
+    L.o_items = 0;
+    L.o_cursor = align(L.o_items + sizeof(PermItem) * items.size());
+    L.o_part = align(L.o_cursor + sizeof(unsigned long long));
+    L.o_widx = align(L.o_part + sizeof(cd) * (size_t)slots);
+    L.o_ws0 = align(L.o_widx + sizeof(int64_t) * wave_idx.size());
+    L.o_wns = align(L.o_ws0 + sizeof(int32_t) * wave_idx.size());
+    L.o_wn = align(L.o_wns + sizeof(int32_t) * wave_idx.size());
+    L.o_sidx = align(L.o_wn + sizeof(int32_t) * wave_idx.size());
+    L.o_soff = align(L.o_sidx + sizeof(int64_t) * small_idx.size());
+    L.o_sn = align(L.o_soff + sizeof(int64_t) * small_idx.size());
+    L.total = align(L.o_sn + sizeof(int32_t) * small_idx.size());
+
+    host.assign(L.total, 0);
+    auto put = [&](size_t off, const void *src, size_t bytes) { if (bytes) std::memcpy(host.data() + off, src, bytes); };
+    put(L.o_items, items.data(), sizeof(PermItem) * items.size());
+    put(L.o_widx, wave_idx.data(), sizeof(int64_t) * wave_idx.size());
+    put(L.o_ws0, wave_slot0.data(), sizeof(int32_t) * wave_idx.size());
+    put(L.o_wns, wave_nslot.data(), sizeof(int32_t) * wave_idx.size());
+    put(L.o_wn, wave_n.data(), sizeof(int32_t) * wave_idx.size());
+    put(L.o_sidx, small_idx.data(), sizeof(int64_t) * small_idx.size());
+    put(L.o_soff, small_off.data(), sizeof(int64_t) * small_idx.size());
+    put(L.o_sn, small_n.data(), sizeof(int32_t) * small_idx.size());
+    return OK;
+}
+
+// Points the plan's device arrays into a block laid out as L.
+static void perm_plan_bind(PermPlanImpl &p, const PermLayout &L, void *block)
+{
+    char *base = static_cast<char *>(block);
+    p.d_items = reinterpret_cast<PermItem *>(base + L.o_items);
+    p.d_cursor = reinterpret_cast<unsigned long long *>(base + L.o_cursor);
+    p.d_partials = reinterpret_cast<cd *>(base + L.o_part);
+    p.d_wave_idx = reinterpret_cast<int64_t *>(base + L.o_widx);
+    p.d_wave_slot0 = reinterpret_cast<int32_t *>(base + L.o_ws0);
+    p.d_wave_nslot = reinterpret_cast<int32_t *>(base + L.o_wns);
+    p.d_wave_n = reinterpret_cast<int32_t *>(base + L.o_wn);
+    p.d_small_idx = reinterpret_cast<int64_t *>(base + L.o_sidx);
+    p.d_small_off = reinterpret_cast<int64_t *>(base + L.o_soff);
+    p.d_small_n = reinterpret_cast<int32_t *>(base + L.o_sn);
+}
+
+static int perm_plan_create_impl(const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                                 int max_chunk_log2, int variant, pcvl_perm_plan *out)
+{
+    PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
+    *out = nullptr;
+    auto *plan = new (std::nothrow) pcvl_perm_plan_s();
+    if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
+    PermPlanImpl &p = plan->impl;
+    PermLayout L;
+    std::vector<char> host;
+    int rc = perm_plan_build(h_n, h_offsets, count, max_chunk_log2, variant, p, L, host);
+    if (rc != OK) { delete plan; return rc; }
+    hipError_t e = hipMalloc(&p.d_block, L.total);
     if (e != hipSuccess) {
         set_error(std::string("hipMalloc(perm plan): ") + hipGetErrorString(e));
         delete plan;
         return ERR_ALLOC;
     }
-    char *base = static_cast<char *>(p.d_block);
-    p.d_items = reinterpret_cast<PermItem *>(base + o_items);
-    p.d_cursor = reinterpret_cast<unsigned long long *>(base + o_cursor);
-    p.d_partials = reinterpret_cast<cd *>(base + o_part);
-    p.d_wave_idx = reinterpret_cast<int64_t *>(base + o_widx);
-    p.d_wave_slot0 = reinterpret_cast<int32_t *>(base + o_ws0);
-    p.d_wave_nslot = reinterpret_cast<int32_t *>(base + o_wns);
-    p.d_wave_n = reinterpret_cast<int32_t *>(base + o_wn);
-    p.d_small_idx = reinterpret_cast<int64_t *>(base + o_sidx);
-    p.d_small_off = reinterpret_cast<int64_t *>(base + o_soff);
-    p.d_small_n = reinterpret_cast<int32_t *>(base + o_sn);
-
-    std::vector<char> host(total, 0);
-    auto put = [&](size_t off, const void *src, size_t bytes) { if (bytes) std::memcpy(host.data() + off, src, bytes); };
-    put(o_items, items.data(), sizeof(PermItem) * items.size());
-    put(o_widx, wave_idx.data(), sizeof(int64_t) * wave_idx.size());
-    put(o_ws0, wave_slot0.data(), sizeof(int32_t) * wave_idx.size());
-    put(o_wns, wave_nslot.data(), sizeof(int32_t) * wave_idx.size());
-    put(o_wn, wave_n.data(), sizeof(int32_t) * wave_idx.size());
-    put(o_sidx, small_idx.data(), sizeof(int64_t) * small_idx.size());
-    put(o_soff, small_off.data(), sizeof(int64_t) * small_idx.size());
-    put(o_sn, small_n.data(), sizeof(int32_t) * small_idx.size());
-    e = hipMemcpy(p.d_block, host.data(), total, hipMemcpyHostToDevice);
+    perm_plan_bind(p, L, p.d_block);
+    e = hipMemcpy(p.d_block, host.data(), L.total, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_error(std::string("hipMemcpy(perm plan): ") + hipGetErrorString(e));
         (void)hipFree(p.d_block);
         delete plan;
         return ERR_HIP;
     }
-
-    if (p.nitems > 0) p.grid = (int)std::min<int64_t>(p.nitems, (int64_t)rw);
     *out = plan;
     return OK;
 }
@@ -707,3 +732,86 @@ extern "C" int pcvl_perm_batch(const double *d_mats, const int32_t *h_n, const i
     if (e != hipSuccess) { set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e)); return ERR_HIP; }
     return rc2;
 }
+
+// ------------------------------------------------------- no allocation
+//
+// pcvl_perm_batch_noalloc: the same batch with no device allocation.  The
+// plan's block (work items, queue cursor, partial sums, per-matrix arrays)
+// goes into `scratch`, a file-scope device array that each GPU holds from
+// module load until the process ends.  Host code never passes the array's
+// name to a kernel: its device address is looked up once per device with
+// hipGetSymbolAddress, on the entry point's first call on that device, and
+// that pointer is used for the copy and the kernel arguments.  Every call on
+// a GPU shares its array: calls that overlap on one GPU (several host threads
+// or streams) race.  Each call rewrites the whole block it uses, the queue
+// cursor included (zero), before its kernels read it.
+constexpr size_t PERM_SCRATCH_BYTES = size_t(16) << 20;
+static __device__ __align__(256) char scratch[PERM_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64] = {};
+
+static int scratch_ptr(void **out)
+{
+    int dev = 0;
+    PCVL_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) { set_error("device index outside [0, 64)"); return ERR_ARG; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return OK;
+}
+
+static int perm_batch_noalloc_impl(const double *d_mats, const int32_t *h_n, const int64_t *h_offsets, int64_t count,
+                                   double *d_out, hipStream_t stream)
+{
+    pcvl_perm_plan_s plan;
+    PermPlanImpl &p = plan.impl;
+    PermLayout L;
+    std::vector<char> host;
+    int rc = perm_plan_build(h_n, h_offsets, count, 0, 0, p, L, host);
+    if (rc != OK) return rc;
+    if (L.total > PERM_SCRATCH_BYTES) {
+        set_error("permanent batch needs " + std::to_string(L.total) + " bytes of plan scratch, above the " +
+                  std::to_string(PERM_SCRATCH_BYTES) + " of pcvl_perm_batch_noalloc (split the batch or use "
+                  "pcvl_perm_batch)");
+        return ERR_RANGE;
+    }
+    if (count == 0) return OK;
+    PCVL_ARG_CHECK(d_mats && d_out, "null device buffer");
+    void *base = nullptr;
+    rc = scratch_ptr(&base);
+    if (rc != OK) return rc;
+    perm_plan_bind(p, L, base);
+    PCVL_HIP_CHECK(hipMemcpyAsync(base, host.data(), L.total, hipMemcpyHostToDevice, stream));
+    rc = pcvl_perm_plan_execute(&plan, d_mats, d_out, stream);
+    // the host image must outlive its copy, and results are ready on return
+    hipError_t e = hipStreamSynchronize(stream);
+    if (rc != OK) return rc;
+    if (e != hipSuccess) { set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e)); return ERR_HIP; }
+    return OK;
+}
+
+extern "C" int pcvl_perm_batch_noalloc(const double *d_mats, const int32_t *h_n, const int64_t *h_offsets,
+                                       int64_t count, double *d_out, hipStream_t stream)
+{
+    try {
+        return perm_batch_noalloc_impl(d_mats, h_n, h_offsets, count, d_out, stream);
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed while building the permanent plan");
+        return ERR_ALLOC;
+    } catch (...) {
+        set_error("unexpected exception while building the permanent plan");
+        return ERR_ALLOC;
+    }
+}
+
+extern "C" size_t pcvl_perm_noalloc_max_bytes(void) { return PERM_SCRATCH_BYTES; }
