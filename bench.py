@@ -342,7 +342,7 @@ def bench_slos(args, res):
     # 1 per state, 4 per state in two phases, 6 successor runs of 3 children
     # per thread with a transposed gather, 8 prefix/suffix blocks, 0 the
     # library's own pick (the headline number)
-    for kernel in (1, 4, 6, 8, 0):
+    for kernel in args.slos_kernels:
         plan.set_kernel(kernel)
         plan.run(dU, PROBS, out)
         with Timer() as t:
@@ -363,6 +363,12 @@ def bench_slos(args, res):
     plan.set_block_width(default_w)
     plan.set_kernel(0)
     plan.run(dU, PROBS, out)
+    if 0 not in per_kernel:
+        plan.run(dU, PROBS, out)
+        with Timer() as t:
+            for _ in range(reps):
+                plan.run(dU, PROBS, out)
+        per_kernel[0] = t.ms / reps
     ms = per_kernel[0]
     total = float(out.sum().item())
     plan.close()
@@ -370,8 +376,9 @@ def bench_slos(args, res):
     res["slos_m24_n10"] = {"ms": ms, "outputs": plan.count, "algorithmic_GB": gbytes,
                            "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,
                            "sum_probs": total, "block_width": default_w,
-                           "ms_by_kernel": {"per_state": per_kernel[1], "two_phase": per_kernel[4],
-                                            "successor_run3": per_kernel[6], "prefix_suffix_blocks": per_kernel[8]},
+                           "ms_by_kernel": {name: per_kernel[kid] for kid, name in
+                                            ((1, "per_state"), (4, "two_phase"), (6, "successor_run3"),
+                                             (8, "prefix_suffix_blocks")) if kid in per_kernel},
                            "ms_by_block_width": {str(w): v for w, v in per_width.items()}}
     log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}; "
         f"by kernel " + ", ".join(f"{k}: {v:.3f}" for k, v in per_kernel.items()) +
@@ -561,7 +568,7 @@ def bench_sampling(args, res):
     net = ProposalNet(m, n, hidden=256, seed=0)
     tseq = occ_to_seq(train, n)
     t0 = time.perf_counter()
-    hist = net.train(tseq, epochs=args.epochs, batch=2048, lr=2e-3)
+    hist = net.train(tseq, epochs=args.epochs, batch=2048, lr=args.lr, lr_end=args.lr_end)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     log(f"[bench] proposal training: {train_s:.2f} s, mean -log q per epoch {['%.3f' % h for h in hist]}")
@@ -641,10 +648,14 @@ def main():
     ap.add_argument("--mis-judge-samples", type=int, default=400000,
                     help="chain length for the equal-TVD comparison of the learned sampler")
     ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--lr", type=float, default=2e-3, help="proposal training step size")
+    ap.add_argument("--lr-end", type=float, default=None, help="cosine decay of the step size down to this value")
     ap.add_argument("--cpu-budget", type=float, default=5.0, help="seconds of CPU baseline per section")
     ap.add_argument("--sections", default=",".join(SECTIONS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baselines")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-order batch")
+    ap.add_argument("--slos-kernels", type=lambda v: [int(x) for x in v.split(",") if x], default=[1, 4, 6, 8, 0],
+                    help="SLOS layer kernels to time (0 = the plan's own pick)")
     ap.add_argument("--slos-widths", type=lambda v: [int(x) for x in v.split(",") if x], default=[6, 8, 10, 12],
                     help="suffix widths of the block SLOS kernel to time")
     args = ap.parse_args()

@@ -37,8 +37,6 @@ namespace pcvl {
 // One group of a layer: the blocks of one sub-layer with r photons in the
 // suffix modes.  64 bytes.
 struct BlkGroup {
-    uint32_t wg0;       // first workgroup of the group
-    uint32_t nwg;       // workgroups (a multiple of 8, for the XCD-contiguous order)
     uint32_t nitems;    // wave items: (block, 64-child chunk), or 64 consecutive children
     uint32_t nb;        // blocks: N(h, K)
     uint32_t S;         // children per block: N(w, r)
@@ -51,7 +49,15 @@ struct BlkGroup {
     uint32_t par_base;  // offset of the same herald configuration's sub-layer in layer k - 1
     uint32_t her_off;   // herald parents: entries [her_off, her_off + nher) of the herald table
     uint32_t nher;
-    uint32_t pad0, pad1;
+    uint32_t rec0;      // record of the group's block 0 (groups with nch != 0)
+    uint32_t pad0, pad1, pad2;
+};
+
+// Precomputed walk of one block (groups with whole waves per block): the
+// block's base B, PE_h, prod alpha! and its occupied prefix modes.
+struct BlkRec {
+    uint32_t Bk, PEh, cnt, pad;
+    double pf;
 };
 
 // One sub-layer of layer k: herald configuration a.
@@ -72,10 +78,13 @@ struct BlkLayout {
     int mloc = 0;             // U rows used: free modes then herald modes
     std::vector<uint64_t> counts;                // states of layer k (compact when masked), k = 0..nlayers
     std::vector<std::vector<BlkGroup>> groups;   // per layer k (index k)
-    std::vector<uint32_t> grid;                  // workgroups per layer
+    std::vector<uint64_t> items;                 // wave items per layer
+    int grid_cap = 0;                            // workgroups of the persistent launch (a multiple of 8)
     std::vector<size_t> group_off;               // byte offset of layer k's groups in d_mem
     size_t off_th = 0, off_te = 0, off_cte = 0, off_f32 = 0, off_fd = 0, off_pf = 0, off_slot = 0, off_row = 0,
-           off_her = 0;
+           off_her = 0, off_rec = 0, off_rlist = 0;
+    int cmax = 1;             // entries per block in the record list: min(h, nmax)
+    uint64_t nrec = 0;        // block records
     char *d_mem = nullptr;
     size_t bytes = 0;
 };

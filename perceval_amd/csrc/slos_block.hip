@@ -37,6 +37,7 @@ namespace pcvl {
 constexpr int BLK_THREADS = 256;
 constexpr int BLK_WMAX = 16;        // widest suffix
 constexpr uint32_t BLK_A_MIN = 32;  // smallest block given whole waves
+constexpr int BLK_WG_PER_CU = 8;    // workgroups per CU of the persistent launch
 
 static uint64_t blk_binom(int64_t a, int64_t b)
 {
@@ -81,38 +82,19 @@ __device__ __forceinline__ void blk_heralds(cd &acc, const cd *__restrict__ pare
     }
 }
 
+// One wave item of group g: one 64-child chunk of one block (g.nch != 0), or
+// 64 consecutive children of the group.  Lanes past the item's children
+// return at once.
 template <int MODE, bool PFI>
-__global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
-    const cd *__restrict__ parent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk, int k,
-    uint32_t nchild, int h, int wsfx, int mfree, int mloc, const int32_t *__restrict__ rowmap,
-    const BlkGroup *__restrict__ groups, int ngroups, const uint2 *__restrict__ her,
+__device__ __forceinline__ void blk_item(
+    const BlkGroup &g, uint32_t item, uint32_t lane, int tid, const cd *__restrict__ parent, void *__restrict__ out,
+    const cd *su, uint32_t *lpe, uint8_t *lmd, int h, int wsfx, int mfree, const uint2 *__restrict__ her,
     const uint32_t *__restrict__ Th, const uint64_t *__restrict__ TE, const uint64_t *__restrict__ CTE, int wt,
     const uint32_t *__restrict__ f32, const double *__restrict__ fd, const double *__restrict__ spf,
-    const uint32_t *__restrict__ sslot, double inv_prod_in, int sentinel)
+    const uint32_t *__restrict__ sslot, const BlkRec *__restrict__ recs, const uint2 *__restrict__ rlist, int cmax,
+    double inv_prod_in)
 {
     using pf_t = typename std::conditional<PFI, uint32_t, double>::type;
-    extern __shared__ uint64_t s_blk[];
-    cd *su = reinterpret_cast<cd *>(s_blk);                               // local mode -> U[row, mk]
-    uint32_t *lpe = reinterpret_cast<uint32_t *>(su + mloc);              // k x THREADS (lists of waves over several blocks)
-    uint8_t *lmd = reinterpret_cast<uint8_t *>(lpe + (size_t)k * BLK_THREADS);
-    const int tid = threadIdx.x;
-    for (int e = tid; e < mloc; e += BLK_THREADS) su[e] = U[(size_t)rowmap[e] * m + mk];
-    if (sentinel && blockIdx.x == 0 && tid == 0) reinterpret_cast<cd *>(out)[nchild] = cd{0.0, 0.0};
-    __syncthreads();
-
-    int lo = 0, hi = ngroups - 1;   // last group whose first workgroup is <= blockIdx.x
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (groups[mid].wg0 <= blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const BlkGroup g = groups[lo];
-    const uint32_t gl = blockIdx.x - g.wg0, t8 = g.nwg >> 3;
-    const uint32_t gx = (gl & 7u) * t8 + (gl >> 3);
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint32_t item = gx * (BLK_THREADS / 64) + wave;
-    if (item >= g.nitems) return;
-    const uint32_t lane = tid & 63u;
     const int r = (int)g.r, K = (int)g.K;
     const int smax = min(r, wsfx);
     const uint32_t S = g.S;
@@ -120,45 +102,24 @@ __global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
     uint32_t beta, Bk, PEh;
     pf_t pfp = (pf_t)1;
     cd acc = {0.0, 0.0};
+    double pfd = 1.0;   // prod alpha! of a whole-wave block (from its record)
     if (g.nch != 0) {
-        // ---- one chunk of one block: alpha is wave-uniform
+        // ---- one chunk of one block: alpha is wave-uniform, its walk
+        // precomputed in the block's record
         const uint32_t b = item / g.nch;
         const uint32_t c = item - b * g.nch;
         beta = c * 64u + lane;
-        uint32_t rem = b, PE = 0;
-        int P = K, cnt = 0, vpe = 0, vmd = 0;
-        Bk = 0;
-        for (int q = 0; q < h; q++) {
-            int x = 0;
-            if (q < h - 1) {
-                x = P;
-                uint32_t t = Th[q * wt + x];
-                while (t > rem) { --x; t = Th[q * wt + x]; }
-                rem -= t;
-            }
-            const uint64_t te = TE[q * wt + x + r];
-            Bk += (uint32_t)te;
-            if (P > x) {
-                vpe = lane == (uint32_t)cnt ? (int)PE : vpe;   // lane cnt takes entry cnt
-                vmd = lane == (uint32_t)cnt ? q : vmd;
-                cnt++;
-                if (PFI) pfp *= (pf_t)f32[P - x];
-                else pfp *= (pf_t)fd[P - x];
-            }
-            PE += (uint32_t)(te >> 32);
-            P = x;
-            if (P == 0) {   // modes q+1..h-1 are empty
-                const uint64_t ce = CTE[r * (h + 1) + q + 1];
-                Bk += (uint32_t)ce;
-                PE += (uint32_t)(ce >> 32);
-                break;
-            }
-        }
-        PEh = PE;
+        const uint32_t rb = g.rec0 + b;
+        const BlkRec rec = recs[rb];
+        Bk = rec.Bk;
+        PEh = rec.PEh;
+        pfd = rec.pf;
+        const int cnt = (int)rec.cnt;
         if (beta >= S) return;
         if (g.nher) blk_heralds(acc, parent, her + g.her_off, g.nher, su, mfree, Bk + beta);
         const uint32_t boff = beta << 4;
         const cd *pp = parent + g.par_base;
+        const uint2 *rl = rlist + (size_t)rb * cmax;
         for (int d0 = 0; d0 < cnt; d0 += 4) {
             cd pv[4], uv[4];
 #pragma unroll
@@ -166,11 +127,10 @@ __global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
                 pv[t] = cd{0.0, 0.0};
                 uv[t] = cd{0.0, 0.0};
                 if (d0 + t < cnt) {
-                    const uint32_t pe = (uint32_t)__builtin_amdgcn_readlane(vpe, d0 + t);
-                    const int j = __builtin_amdgcn_readlane(vmd, d0 + t);
-                    const char *pb = reinterpret_cast<const char *>(pp + (Bk - pe));
+                    const uint2 e = rl[d0 + t];
+                    const char *pb = reinterpret_cast<const char *>(pp + (Bk - e.x));
                     pv[t] = *reinterpret_cast<const cd *>(pb + boff);
-                    uv[t] = su[j];
+                    uv[t] = su[e.y];
                 }
             }
 #pragma unroll
@@ -231,29 +191,34 @@ __global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
         }
     }
 
-    // ---- suffix parents, from the group's slot table
+    // ---- suffix parents, from the group's slot table (slot words loaded
+    // eight at a time, then their parents four at a time)
     if (smax > 0) {
         const uint32_t *ts = sslot + g.tslot + beta;
         const char *ps = reinterpret_cast<const char *>(parent + g.par_base + (Bk - PEh));
-        uint32_t sw[BLK_WMAX];
 #pragma unroll
-        for (int d = 0; d < BLK_WMAX; d++)
-            if (d < smax) sw[d] = ts[(size_t)d * S];
+        for (int h0 = 0; h0 < BLK_WMAX; h0 += 8) {
+            if (h0 < smax) {
+                uint32_t sw[8];
 #pragma unroll
-        for (int d0 = 0; d0 < BLK_WMAX; d0 += 4) {
-            if (d0 < smax) {
-                cd pv[4], uv[4];
+                for (int d = 0; d < 8; d++) sw[d] = h0 + d < smax ? ts[(size_t)(h0 + d) * S] : 0xffffffffu;
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    pv[t] = cd{0.0, 0.0};
-                    uv[t] = cd{0.0, 0.0};
-                    if (d0 + t < smax && sw[d0 + t] != 0xffffffffu) {
-                        pv[t] = *reinterpret_cast<const cd *>(ps + ((sw[d0 + t] >> 5) << 4));
-                        uv[t] = su[h + (int)(sw[d0 + t] & 31u)];
+                for (int d0 = 0; d0 < 8; d0 += 4) {
+                    if (h0 + d0 < smax) {
+                        cd pv[4], uv[4];
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            pv[t] = cd{0.0, 0.0};
+                            uv[t] = cd{0.0, 0.0};
+                            if (sw[d0 + t] != 0xffffffffu) {
+                                pv[t] = *reinterpret_cast<const cd *>(ps + ((sw[d0 + t] >> 5) << 4));
+                                uv[t] = su[h + (int)(sw[d0 + t] & 31u)];
+                            }
+                        }
+#pragma unroll
+                        for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);
                     }
                 }
-#pragma unroll
-                for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);
             }
         }
     }
@@ -262,8 +227,46 @@ __global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
     if (MODE == 0) {
         reinterpret_cast<cd *>(out)[o] = acc;
     } else {
-        const double pf = (double)pfp * spf[g.tpf + beta];
+        const double pf = (g.nch != 0 ? pfd : (double)pfp) * spf[g.tpf + beta];
         reinterpret_cast<double *>(out)[o] = (acc.r * acc.r + acc.i * acc.i) * pf * inv_prod_in;
+    }
+}
+
+// Persistent layer kernel: the workgroups of XCD x (blockIdx % 8, dispatch
+// being round robin over the 8 XCDs) take the x-th eighth of every group's
+// items, groups in increasing suffix photon count, so all XCDs work on the
+// same group at a time and each on a contiguous part of it (each XCD has its
+// own L2).  Waves take items at a stride of the XCD's waves.
+template <int MODE, bool PFI>
+__global__ void __launch_bounds__(BLK_THREADS) __attribute__((amdgpu_waves_per_eu(8))) slos_block_kernel(
+    const cd *__restrict__ parent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk, int k,
+    uint32_t nchild, int h, int wsfx, int mfree, int mloc, const int32_t *__restrict__ rowmap,
+    const BlkGroup *__restrict__ groups, int ngroups, const uint2 *__restrict__ her,
+    const uint32_t *__restrict__ Th, const uint64_t *__restrict__ TE, const uint64_t *__restrict__ CTE, int wt,
+    const uint32_t *__restrict__ f32, const double *__restrict__ fd, const double *__restrict__ spf,
+    const uint32_t *__restrict__ sslot, const BlkRec *__restrict__ recs, const uint2 *__restrict__ rlist, int cmax,
+    double inv_prod_in, int sentinel)
+{
+    extern __shared__ uint64_t s_blk[];
+    cd *su = reinterpret_cast<cd *>(s_blk);                               // local mode -> U[row, mk]
+    uint32_t *lpe = reinterpret_cast<uint32_t *>(su + mloc);              // k x THREADS (lists of waves over several blocks)
+    uint8_t *lmd = reinterpret_cast<uint8_t *>(lpe + (size_t)k * BLK_THREADS);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < mloc; e += BLK_THREADS) su[e] = U[(size_t)rowmap[e] * m + mk];
+    if (sentinel && blockIdx.x == 0 && tid == 0) reinterpret_cast<cd *>(out)[nchild] = cd{0.0, 0.0};
+    __syncthreads();
+
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t x = blockIdx.x & 7u, j = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;
+    const uint32_t stride = per_xcd * (BLK_THREADS / 64);
+    for (int gi = 0; gi < ngroups; gi++) {
+        const BlkGroup g = groups[gi];
+        const uint32_t share = (g.nitems + 7u) >> 3;
+        const uint32_t lo = min(x * share, g.nitems), hi = min(lo + share, g.nitems);
+        for (uint32_t item = lo + j * (BLK_THREADS / 64) + wave; item < hi; item += stride)
+            blk_item<MODE, PFI>(g, item, lane, tid, parent, out, su, lpe, lmd, h, wsfx, mfree, her, Th, TE, CTE, wt,
+                                f32, fd, spf, sslot, recs, rlist, cmax, inv_prod_in);
     }
 }
 
@@ -383,12 +386,58 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
             }
         }
     }
-    // groups per layer: each sub-layer, suffix photons r = 0..kf
+    // groups per layer: each sub-layer, suffix photons r = 0..kf; the blocks
+    // of whole-wave groups get their walk precomputed (records)
+    lay.cmax = std::max(1, std::min(h, nmax));
+    const int cmax = lay.cmax;
+    std::vector<BlkRec> recs;
+    std::vector<uint32_t> rlist;  // (PE, mode) pairs, cmax per record
+    auto add_records = [&](int K, int r) {
+        std::vector<int> alpha((size_t)h, 0);
+        alpha[0] = K;
+        const uint64_t nb = nstates(h, K);
+        for (uint64_t i = 0; i < nb; i++) {
+            BlkRec rec{};
+            uint64_t Bk = 0, PE = 0;
+            double pf = 1.0;
+            int R = r;   // photons in modes q+1.. (prefix part plus the suffix)
+            std::vector<int> after((size_t)h + 1, 0);
+            for (int q = h - 1; q >= 0; q--) after[(size_t)q] = after[(size_t)q + 1] + alpha[(size_t)q];
+            const size_t l0 = rlist.size();
+            rlist.resize(l0 + 2 * (size_t)cmax, 0);
+            int cnt = 0;
+            for (int q = 0; q < h; q++) {
+                R = r + after[(size_t)q + 1];
+                const uint64_t tev = te[(size_t)q * wt + R];
+                Bk += tev & 0xffffffffull;
+                if (alpha[(size_t)q] > 0) {
+                    rlist[l0 + 2 * (size_t)cnt] = (uint32_t)PE;
+                    rlist[l0 + 2 * (size_t)cnt + 1] = (uint32_t)q;
+                    cnt++;
+                    for (int v = 2; v <= alpha[(size_t)q]; v++) pf *= v;
+                }
+                PE += tev >> 32;
+            }
+            rec.Bk = (uint32_t)Bk;
+            rec.PEh = (uint32_t)PE;
+            rec.cnt = (uint32_t)cnt;
+            rec.pf = pf;
+            recs.push_back(rec);
+            if (i + 1 < nb) {   // next alpha in descending lexicographic order
+                const int tail = alpha[(size_t)h - 1];
+                alpha[(size_t)h - 1] = 0;
+                int t = h - 2;
+                while (alpha[(size_t)t] == 0) t--;
+                alpha[(size_t)t]--;
+                alpha[(size_t)t + 1] = tail + 1;
+            }
+        }
+    };
     std::vector<uint32_t> hert;   // (herald index, parent sub-layer offset) pairs
     lay.groups.assign((size_t)lay.nlayers + 1, {});
-    lay.grid.assign((size_t)lay.nlayers + 1, 0);
+    lay.items.assign((size_t)lay.nlayers + 1, 0);
     for (int k = 1; k <= lay.nlayers; k++) {
-        uint64_t wg = 0;
+        uint64_t items = 0;
         for (const BlkSub &sub : subs[(size_t)k]) {
             if (sub.kf < 0 || sub.kf > nmax) { set_error("block SLOS layout: sub-layer photons outside the tables"); return ERR_ARG; }
             const uint32_t her_off = (uint32_t)(hert.size() / 2);
@@ -407,12 +456,16 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
                 if (g.S >= BLK_A_MIN) {
                     g.nch = (g.S + 63) / 64;
                     g.nitems = g.nb * g.nch;
+                    g.rec0 = (uint32_t)recs.size();
+                    add_records(sub.kf - r, r);
+                    if (recs.size() > (64ull << 20) / sizeof(BlkRec)) {
+                        set_error("block SLOS layout: more than 2^21 block records (use a wider suffix)");
+                        return ERR_RANGE;
+                    }
                 } else {
                     g.nch = 0;
                     g.nitems = (uint32_t)(((uint64_t)g.nb * g.S + 63) / 64);
                 }
-                g.nwg = (((g.nitems + 3) / 4) + 7) / 8 * 8;
-                g.wg0 = (uint32_t)wg;
                 g.tpf = tpf[(size_t)r];
                 g.tslot = tslot[(size_t)r];
                 g.r = (uint32_t)r;
@@ -421,12 +474,11 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
                 g.par_base = sub.par_base;
                 g.her_off = her_off;
                 g.nher = (uint32_t)sub.her.size();
-                wg += g.nwg;
+                items += g.nitems;
                 lay.groups[(size_t)k].push_back(g);
             }
         }
-        if (wg >= (1ull << 31)) { set_error("block SLOS layout: grid too large"); return ERR_RANGE; }
-        lay.grid[(size_t)k] = (uint32_t)wg;
+        lay.items[(size_t)k] = items;
     }
     // one device allocation
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -445,6 +497,9 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
     }
     lay.off_pf = off; off += al(sizeof(double) * spf.size());
     lay.off_slot = off; off += al(sizeof(uint32_t) * std::max<size_t>(sslot.size(), 1));
+    lay.nrec = recs.size();
+    lay.off_rec = off; off += al(sizeof(BlkRec) * std::max<size_t>(recs.size(), 1));
+    lay.off_rlist = off; off += al(sizeof(uint32_t) * std::max<size_t>(rlist.size(), 2));
     lay.bytes = off;
     std::vector<char> &host = *image;
     host.assign(off, 0);
@@ -460,6 +515,8 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
                     sizeof(BlkGroup) * lay.groups[(size_t)k].size());
     std::memcpy(host.data() + lay.off_pf, spf.data(), sizeof(double) * spf.size());
     if (!sslot.empty()) std::memcpy(host.data() + lay.off_slot, sslot.data(), sizeof(uint32_t) * sslot.size());
+    if (!recs.empty()) std::memcpy(host.data() + lay.off_rec, recs.data(), sizeof(BlkRec) * recs.size());
+    if (!rlist.empty()) std::memcpy(host.data() + lay.off_rlist, rlist.data(), sizeof(uint32_t) * rlist.size());
     *out = std::move(lay);
     return OK;
 }
@@ -480,6 +537,13 @@ int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<ch
     std::vector<int32_t> rowmap((size_t)m);
     for (int i = 0; i < m; i++) rowmap[(size_t)i] = i;
     return blk_layout_build_host_ex(m, nmax, w, subs, counts, rowmap, out, image);
+}
+
+// LDS of layer k's launch: the U column, then the per-thread lists of waves
+// whose lanes walk their own blocks.
+static size_t blk_lds_bytes(const BlkLayout &lay, int k)
+{
+    return sizeof(cd) * lay.mloc + (sizeof(uint32_t) + 1) * (size_t)k * BLK_THREADS;
 }
 
 int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out)
@@ -567,6 +631,23 @@ int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out)
 
 int blk_layout_upload(BlkLayout *lay, const std::vector<char> &image)
 {
+    // The persistent launch needs every workgroup resident at once (each
+    // takes an equal share of the items): size it by the kernel's occupancy
+    // at the largest LDS any layer of the layout asks for.
+    int dev = 0, cus = 0, per_cu = 0;
+    PCVL_HIP_CHECK(hipGetDevice(&dev));
+    PCVL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t lds = blk_lds_bytes(*lay, lay->nlayers);
+    int worst = 1 << 30;
+    for (const void *kp : {reinterpret_cast<const void *>(&slos_block_kernel<0, true>),
+                           reinterpret_cast<const void *>(&slos_block_kernel<0, false>),
+                           reinterpret_cast<const void *>(&slos_block_kernel<1, true>),
+                           reinterpret_cast<const void *>(&slos_block_kernel<1, false>)}) {
+        PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, BLK_THREADS, lds));
+        worst = std::min(worst, per_cu);
+    }
+    worst = std::max(1, std::min(worst, BLK_WG_PER_CU));
+    lay->grid_cap = std::max(8, cus * worst / 8 * 8);
     hipError_t e = hipMalloc(&lay->d_mem, lay->bytes);
     if (e != hipSuccess) {
         lay->d_mem = nullptr;
@@ -604,15 +685,18 @@ int blk_layer_launch(const BlkLayout &lay, const cd *parent, void *out, const cd
                      double inv_prod_in, bool sentinel, hipStream_t stream)
 {
     if (k < 1 || k > lay.nlayers || !lay.d_mem) { set_error("block SLOS: layer outside the layout"); return ERR_ARG; }
-    if (lay.grid[(size_t)k] == 0) return OK;
+    if (lay.items[(size_t)k] == 0) return OK;
+    // persistent grid: a multiple of 8 workgroups (the XCD split), at most grid_cap
+    const uint64_t want = (lay.items[(size_t)k] + (BLK_THREADS / 64) - 1) / (BLK_THREADS / 64);
+    const uint32_t grid = (uint32_t)((std::min<uint64_t>(want, (uint64_t)lay.grid_cap) + 7) / 8 * 8);
     const char *base = lay.d_mem;
     const BlkGroup *groups = reinterpret_cast<const BlkGroup *>(base + lay.group_off[(size_t)k]);
     const uint32_t nchild = (uint32_t)lay.counts[(size_t)k];
-    const size_t lds = sizeof(cd) * lay.mloc + (sizeof(uint32_t) + 1) * (size_t)k * BLK_THREADS;
+    const size_t lds = blk_lds_bytes(lay, k);
     const bool pfi = lay.nmax <= 12;
     auto kp = probs ? (pfi ? slos_block_kernel<1, true> : slos_block_kernel<1, false>)
                     : (pfi ? slos_block_kernel<0, true> : slos_block_kernel<0, false>);
-    hipLaunchKernelGGL(kp, dim3(lay.grid[(size_t)k]), dim3(BLK_THREADS), lds, stream, parent, out, U, m, mk, k, nchild,
+    hipLaunchKernelGGL(kp, dim3(grid), dim3(BLK_THREADS), lds, stream, parent, out, U, m, mk, k, nchild,
                        lay.h, lay.w, lay.m, lay.mloc, reinterpret_cast<const int32_t *>(base + lay.off_row), groups,
                        (int)lay.groups[(size_t)k].size(), reinterpret_cast<const uint2 *>(base + lay.off_her),
                        reinterpret_cast<const uint32_t *>(base + lay.off_th),
@@ -621,7 +705,9 @@ int blk_layer_launch(const BlkLayout &lay, const cd *parent, void *out, const cd
                        reinterpret_cast<const uint32_t *>(base + lay.off_f32),
                        reinterpret_cast<const double *>(base + lay.off_fd),
                        reinterpret_cast<const double *>(base + lay.off_pf),
-                       reinterpret_cast<const uint32_t *>(base + lay.off_slot), inv_prod_in, sentinel ? 1 : 0);
+                       reinterpret_cast<const uint32_t *>(base + lay.off_slot),
+                       reinterpret_cast<const BlkRec *>(base + lay.off_rec),
+                       reinterpret_cast<const uint2 *>(base + lay.off_rlist), lay.cmax, inv_prod_in, sentinel ? 1 : 0);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
 }

@@ -133,23 +133,31 @@ class ProposalNet:
                       "features_all")
         return x, lo, tgt
 
-    def train(self, seqs, epochs: int = 3, batch: int = 2048, lr: float = 2e-3, seed: int = 0, log=None) -> list:
-        """Maximum likelihood on sorted mode sequences (N x n int32 CUDA)."""
+    def train(self, seqs, epochs: int = 3, batch: int = 2048, lr: float = 2e-3, seed: int = 0, log=None,
+              lr_end: float | None = None) -> list:
+        """Maximum likelihood on sorted mode sequences (N x n int32 CUDA).
+        With lr_end, the step size follows a cosine from lr down to lr_end
+        over the run (per batch)."""
         import torch
         g = torch.Generator(device="cpu").manual_seed(seed)
         N = seqs.shape[0]
         history = []
+        steps = epochs * ((N + batch - 1) // batch)
+        t = 0
         for ep in range(epochs):
             order = torch.randperm(N, generator=g).to(seqs.device)
-            tot, cnt = 0.0, 0
+            tot = torch.zeros((), dtype=torch.float64, device=seqs.device)
+            cnt = 0
             for i0 in range(0, N, batch):
                 sb = seqs[order[i0:i0 + batch]].contiguous()
                 x, lo, tgt = self.teacher_forcing(sb)
                 loss, grads = self.loss_and_grads(x, lo, tgt, 1.0 / sb.shape[0])
-                self.adam_step(grads, lr=lr)
-                tot += float(loss.sum().item())
+                step_lr = lr if lr_end is None else lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * t / steps))
+                self.adam_step(grads, lr=step_lr)
+                tot += loss.sum(dtype=torch.float64)
                 cnt += sb.shape[0]
-            history.append(tot / cnt)   # mean -log q per sequence
+                t += 1
+            history.append(float(tot.item()) / cnt)   # mean -log q per sequence
             if log:
                 log(f"epoch {ep}: mean -log q = {history[-1]:.4f}")
         return history

@@ -9,7 +9,8 @@
 //   * every child of every layer is written exactly once;
 //   * each child's parent list, in the order it is summed, equals brute force
 //     (rank of c - e_j for the occupied modes j, increasing);
-//   * the factorial product of each child equals prod s!.
+//   * the factorial product of each child equals prod s!;
+//   * each precomputed block record equals the block's walk.
 // The replay restates the kernel's index arithmetic line for line; it must be
 // kept in step with the kernel.  Exit status 0 when every check holds.
 #include "../../perceval_amd/csrc/slos_block.hip"
@@ -62,7 +63,7 @@ static int fails = 0;
 // mode indices.  Expected parents of a child: its occupied modes in local
 // order, each parent's position in layers[k-1].
 static void replay(const BlkLayout &lay, const std::vector<char> &img, const std::vector<std::vector<std::vector<int>>> &layers,
-                   const std::vector<int32_t> &perm, const char *what)
+                   const std::vector<int32_t> &perm, const char *what, int grid_cap)
 {
     const int h = lay.h, wt = lay.wt, w = lay.w, mfree = lay.m;
     const uint32_t *Th = (const uint32_t *)(img.data() + lay.off_th);
@@ -74,6 +75,8 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
     const uint32_t *sslot = (const uint32_t *)(img.data() + lay.off_slot);
     const int32_t *rowmap = (const int32_t *)(img.data() + lay.off_row);
     const uint32_t *her = (const uint32_t *)(img.data() + lay.off_her);
+    const BlkRec *recs = (const BlkRec *)(img.data() + lay.off_rec);
+    const uint32_t *rlist = (const uint32_t *)(img.data() + lay.off_rlist);
     const size_t n_th = (lay.off_te - lay.off_th) / 4, n_te = (lay.off_cte - lay.off_te) / 8;
     const size_t n_pf = (lay.off_slot - lay.off_pf) / 8, n_slot = (lay.bytes - lay.off_slot) / 4;
     const size_t n_cte = (lay.off_f32 - lay.off_cte) / 8, n_her = (lay.group_off[1] - lay.off_her) / 8;
@@ -88,20 +91,22 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
         std::vector<int> written(nchild, 0);
         const BlkGroup *groups = (const BlkGroup *)(img.data() + lay.group_off[(size_t)k]);
         const int ngroups = (int)lay.groups[(size_t)k].size();
-        for (uint32_t wg = 0; wg < lay.grid[(size_t)k]; wg++) {
-            int lo = 0, hi = ngroups - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (groups[mid].wg0 <= wg) lo = mid;
-                else hi = mid - 1;
-            }
-            const BlkGroup g = groups[lo];
-            CHECK(wg >= g.wg0 && wg < g.wg0 + g.nwg, "%s: wg %u outside group %d", what, wg, lo);
-            const uint32_t gl = wg - g.wg0, t8 = g.nwg >> 3;
-            const uint32_t gx = (gl & 7u) * t8 + (gl >> 3);
-            for (uint32_t wave = 0; wave < 4; wave++) {
-                const uint32_t item = gx * 4 + wave;
-                if (item >= g.nitems) continue;
+        // the persistent launch of blk_layer_launch with this grid cap
+        const uint64_t want = (lay.items[(size_t)k] + 3) / 4;
+        const uint32_t grid = (uint32_t)((std::min<uint64_t>(want, (uint64_t)grid_cap) + 7) / 8 * 8);
+        const uint32_t per_xcd = grid >> 3, stride = per_xcd * 4;
+        std::vector<std::vector<uint8_t>> seen((size_t)ngroups);
+        for (int gi = 0; gi < ngroups; gi++) seen[(size_t)gi].assign(groups[gi].nitems, 0);
+        for (uint32_t wg = 0; wg < grid; wg++)
+          for (int gi = 0; gi < ngroups; gi++) {
+            const BlkGroup g = groups[gi];
+            const uint32_t x = wg & 7u, j = wg >> 3;
+            const uint32_t share = (g.nitems + 7u) >> 3;
+            const uint32_t lo = std::min(x * share, g.nitems), hi = std::min(lo + share, g.nitems);
+            for (uint32_t wave = 0; wave < 4; wave++)
+              for (uint32_t item = lo + j * 4 + wave; item < hi; item += stride) {
+                CHECK(seen[(size_t)gi][item] == 0, "%s: item %u of group %d taken twice", what, item, gi);
+                seen[(size_t)gi][item] = 1;
                 const int r = (int)g.r, K = (int)g.K, smax = std::min(r, w);
                 const uint32_t S = g.S;
                 for (uint32_t lane = 0; lane < 64; lane++) {
@@ -145,6 +150,20 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
                             PE += (uint32_t)(ce >> 32);
                             break;
                         }
+                    }
+                    if (g.nch != 0) {
+                        // whole-wave blocks read their walk from a record
+                        const uint64_t rb = (uint64_t)g.rec0 + b;
+                        CHECK(rb < lay.nrec, "%s: record index", what);
+                        if (rb >= lay.nrec) continue;
+                        const BlkRec &rec = recs[rb];
+                        CHECK(rec.Bk == Bk && rec.PEh == PE && rec.cnt == lst.size() && rec.pf == pfp,
+                              "%s: record of block %u differs from its walk", what, b);
+                        for (size_t d = 0; d < lst.size() && d < (size_t)lay.cmax; d++)
+                            CHECK(rlist[2 * (rb * lay.cmax + d)] == lst[d].first &&
+                                  (int)rlist[2 * (rb * lay.cmax + d) + 1] == lst[d].second,
+                                  "%s: record list of block %u", what, b);
+                        CHECK(rec.cnt <= (uint32_t)lay.cmax, "%s: record list longer than cmax", what);
                     }
                     if (beta >= S) continue;
                     const uint64_t loc = (uint64_t)Bk + beta;
@@ -198,8 +217,11 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
                     for (auto &e : got) CHECK(e.first < nparent, "%s: parent %lu >= %lu", what, (unsigned long)e.first, (unsigned long)nparent);
                     CHECK(pf == pfe, "%s: pf %g vs %g", what, pf, pfe);
                 }
-            }
-        }
+              }
+          }
+        for (int gi = 0; gi < ngroups; gi++)
+            for (uint32_t it = 0; it < groups[gi].nitems; it++)
+                CHECK(seen[(size_t)gi][it] == 1, "%s: item %u of group %d not taken", what, it, gi);
         for (uint64_t i = 0; i < nchild; i++)
             CHECK(written[i] == 1, "%s: layer %d child %lu written %d times", what, k, (unsigned long)i, written[i]);
     }
@@ -216,7 +238,8 @@ static void emulate(int m, int nmax, int w)
     for (int j = 0; j < m; j++) perm[(size_t)j] = j;
     char what[64];
     snprintf(what, sizeof what, "m=%d n=%d w=%d", m, nmax, w);
-    replay(lay, img, layers, perm, what);
+    replay(lay, img, layers, perm, what, 16);
+    replay(lay, img, layers, perm, what, 2048);
 }
 
 // Masked: cond[j] >= 0 fixes mode j.  The expected layer k holds the states
@@ -263,7 +286,8 @@ static void emulate_masked(int m, int n, const std::vector<int32_t> &cond, int w
         printf("masked build failed %s\n", pcvl_last_error());
         exit(1);
     }
-    replay(lay, img, layers, perm, what);
+    replay(lay, img, layers, perm, what, 24);
+    replay(lay, img, layers, perm, what, 2048);
 }
 
 int main()
