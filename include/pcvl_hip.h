@@ -392,6 +392,11 @@ int pcvl_probe_fp64_fma(int blocks, int64_t iters, double *d_out, int64_t *flops
 int pcvl_probe_fp64_mix(int blocks, int64_t iters, int mode, double *d_out, int64_t *flops_mfma,
                         int64_t *flops_valu, hipStream_t stream);
 
+/* Measurement helper, no counterpart in the reference: the read rate of a
+ * device buffer of `bytes` bytes read `reps` times over by blocks x 256
+ * threads (16-byte loads).  d_out: blocks*256 doubles. */
+int pcvl_probe_read(const double *d_buf, uint64_t bytes, int reps, int blocks, double *d_out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

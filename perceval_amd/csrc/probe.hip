@@ -55,9 +55,35 @@ __global__ void __launch_bounds__(256) probe_mix_kernel(int64_t iters, double a,
     out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// Read bandwidth of a buffer that is read `reps` times over: a working set
+// that fits a cache level (4 MB L2 per XCD, 256 MB last-level cache) shows
+// that level's rate; a larger one the HBM rate.
+__global__ void __launch_bounds__(256) probe_read_kernel(const double2 *__restrict__ p, uint64_t n2, int reps,
+                                                         double *__restrict__ out)
+{
+    double s = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (int r = 0; r < reps; r++)
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
+            const double2 v = p[i];
+            s += v.x + v.y;
+        }
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 } // namespace pcvl
 
 using namespace pcvl;
+
+extern "C" int pcvl_probe_read(const double *d_buf, uint64_t bytes, int reps, int blocks, double *d_out,
+                               hipStream_t stream)
+{
+    PCVL_ARG_CHECK(d_buf && d_out && bytes >= 16 && reps > 0 && blocks > 0, "bad probe arguments");
+    hipLaunchKernelGGL(probe_read_kernel, dim3(blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const double2 *>(d_buf), bytes / 16, reps, d_out);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
 
 extern "C" int pcvl_probe_fp64_mix(int blocks, int64_t iters, int mode, double *d_out, int64_t *flops_mfma,
                                    int64_t *flops_valu, hipStream_t stream)

@@ -53,11 +53,15 @@ struct BlkGroup {
     uint32_t pad0, pad1, pad2;
 };
 
-// Precomputed walk of one block (groups with whole waves per block): the
-// block's base B, PE_h, prod alpha! and its occupied prefix modes.
+// Precomputed walk of one block (groups with whole waves per block), 128
+// bytes so that one scalar fetch brings all of it:
+//   w[0] B_k(alpha), w[1] PE_h(alpha), w[2] occupied prefix modes (<= 16),
+//   w[4..7] those modes (one byte each, increasing), w[8..9] prod alpha!
+//   (double), w[10..25] PE_j(alpha) of each occupied prefix mode j.
+constexpr int BLK_REC_WORDS = 32;
+constexpr int BLK_REC_MAX = 16;
 struct BlkRec {
-    uint32_t Bk, PEh, cnt, pad;
-    double pf;
+    uint32_t w[BLK_REC_WORDS];
 };
 
 // One sub-layer of layer k: herald configuration a.
@@ -82,8 +86,8 @@ struct BlkLayout {
     int grid_cap = 0;                            // workgroups of the persistent launch (a multiple of 8)
     std::vector<size_t> group_off;               // byte offset of layer k's groups in d_mem
     size_t off_th = 0, off_te = 0, off_cte = 0, off_f32 = 0, off_fd = 0, off_pf = 0, off_slot = 0, off_row = 0,
-           off_her = 0, off_rec = 0, off_rlist = 0;
-    int cmax = 1;             // entries per block in the record list: min(h, nmax)
+           off_her = 0, off_rec = 0;
+    int cmax = 1;             // most occupied prefix modes of a block: min(h, nmax)
     uint64_t nrec = 0;        // block records
     char *d_mem = nullptr;
     size_t bytes = 0;

@@ -82,152 +82,197 @@ __device__ __forceinline__ void blk_heralds(cd &acc, const cd *__restrict__ pare
     }
 }
 
-// One wave item of group g: one 64-child chunk of one block (g.nch != 0), or
-// 64 consecutive children of the group.  Lanes past the item's children
-// return at once.
+// Suffix parents of one child, from the group's slot table: slot d of beta
+// holds (rank(beta - e_j) << 5) | (j - h) for its d-th occupied suffix mode j,
+// or all ones past its last one.  Slot words are loaded eight at a time, then
+// their parents four at a time.
+__device__ __forceinline__ void blk_suffix(cd &acc, const char *ps, const uint32_t *__restrict__ ts, uint32_t S,
+                                           int smax, const cd *su, int h)
+{
+#pragma unroll
+    for (int h0 = 0; h0 < BLK_WMAX; h0 += 8) {
+        if (h0 < smax) {
+            uint32_t sw[8];
+#pragma unroll
+            for (int d = 0; d < 8; d++) sw[d] = h0 + d < smax ? ts[(size_t)(h0 + d) * S] : 0xffffffffu;
+#pragma unroll
+            for (int d0 = 0; d0 < 8; d0 += 4) {
+                if (h0 + d0 < smax) {
+                    cd pv[4], uv[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        pv[t] = cd{0.0, 0.0};
+                        uv[t] = cd{0.0, 0.0};
+                        if (sw[d0 + t] != 0xffffffffu) {
+                            pv[t] = *reinterpret_cast<const cd *>(ps + ((sw[d0 + t] >> 5) << 4));
+                            uv[t] = su[h + (int)(sw[d0 + t] & 31u)];
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);
+                }
+            }
+        }
+    }
+}
+
+// One wave item of a whole-wave group: one 64-child chunk of one block.  The
+// block's walk is its record, fetched with one scalar load; the loads are
+// issued in two rounds only: the record, the first eight suffix slot words
+// and the factorial product, then the parents of the first eight prefix modes
+// and the first eight suffix slots together.  Summation order is unchanged:
+// herald modes, prefix modes, suffix modes, each increasing.
+template <int MODE>
+__device__ __forceinline__ void blk_item_a(
+    const BlkGroup &g, uint32_t item, uint32_t lane, const cd *__restrict__ parent, void *__restrict__ out,
+    const cd *su, int h, int wsfx, int mfree, const uint2 *__restrict__ her, const double *__restrict__ spf,
+    const uint32_t *__restrict__ sslot, const BlkRec *__restrict__ recs, double inv_prod_in)
+{
+    const uint32_t b = item / g.nch;
+    const uint32_t beta = (item - b * g.nch) * 64u + lane;
+    const uint32_t *R = recs[g.rec0 + b].w;
+    uint32_t rw[26];
+#pragma unroll
+    for (int i = 0; i < 26; i++) rw[i] = R[i];
+    const uint32_t S = g.S;
+    if (beta >= S) return;
+    const int smax = min((int)g.r, wsfx);
+    const uint32_t *ts = sslot + g.tslot + beta;
+    uint32_t sw[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) sw[d] = d < smax ? ts[(size_t)d * S] : 0xffffffffu;
+    const double pfs = MODE == 1 ? spf[g.tpf + beta] : 0.0;
+    const uint32_t Bk = rw[0], PEh = rw[1];
+    const int cnt = (int)rw[2];
+    cd acc = {0.0, 0.0};
+    if (g.nher) blk_heralds(acc, parent, her + g.her_off, g.nher, su, mfree, Bk + beta);
+    const cd *pp = parent + g.par_base + Bk;
+    const char *ps = reinterpret_cast<const char *>(parent + g.par_base + (Bk - PEh));
+    const uint32_t boff = beta << 4;
+    // round two: first eight prefix parents and first eight suffix parents
+    cd pv[8], qv[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        pv[t] = cd{0.0, 0.0};
+        if (t < cnt) pv[t] = *reinterpret_cast<const cd *>(reinterpret_cast<const char *>(pp - rw[10 + t]) + boff);
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        qv[t] = cd{0.0, 0.0};
+        if (sw[t] != 0xffffffffu) qv[t] = *reinterpret_cast<const cd *>(ps + ((sw[t] >> 5) << 4));
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        if (t < cnt) cfma_acc(acc, pv[t], su[(rw[4 + t / 4] >> (8 * (t % 4))) & 255u]);
+    }
+#pragma unroll
+    for (int t = 8; t < BLK_REC_MAX; t++) {   // more than eight occupied prefix modes
+        if (t < cnt) {
+            const cd p = *reinterpret_cast<const cd *>(reinterpret_cast<const char *>(pp - rw[10 + t]) + boff);
+            cfma_acc(acc, p, su[(rw[4 + t / 4] >> (8 * (t % 4))) & 255u]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        if (sw[t] != 0xffffffffu) cfma_acc(acc, qv[t], su[h + (int)(sw[t] & 31u)]);
+    }
+    if (smax > 8) {   // suffix slots 8..15
+#pragma unroll
+        for (int t = 0; t < 8; t++) sw[t] = 8 + t < smax ? ts[(size_t)(8 + t) * S] : 0xffffffffu;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            qv[t] = cd{0.0, 0.0};
+            if (sw[t] != 0xffffffffu) qv[t] = *reinterpret_cast<const cd *>(ps + ((sw[t] >> 5) << 4));
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            if (sw[t] != 0xffffffffu) cfma_acc(acc, qv[t], su[h + (int)(sw[t] & 31u)]);
+    }
+    const size_t o = (size_t)g.out_base + Bk + beta;
+    if (MODE == 0) {
+        reinterpret_cast<cd *>(out)[o] = acc;
+    } else {
+        double pfa;
+        __builtin_memcpy(&pfa, &rw[8], sizeof(double));
+        reinterpret_cast<double *>(out)[o] = (acc.r * acc.r + acc.i * acc.i) * (pfa * pfs) * inv_prod_in;
+    }
+}
+
+// One wave item of a group whose blocks are smaller than a wave: 64
+// consecutive children of the group, each lane walking its own block (list of
+// occupied prefix modes in LDS, per thread).
 template <int MODE, bool PFI>
-__device__ __forceinline__ void blk_item(
+__device__ __forceinline__ void blk_item_b(
     const BlkGroup &g, uint32_t item, uint32_t lane, int tid, const cd *__restrict__ parent, void *__restrict__ out,
     const cd *su, uint32_t *lpe, uint8_t *lmd, int h, int wsfx, int mfree, const uint2 *__restrict__ her,
     const uint32_t *__restrict__ Th, const uint64_t *__restrict__ TE, const uint64_t *__restrict__ CTE, int wt,
     const uint32_t *__restrict__ f32, const double *__restrict__ fd, const double *__restrict__ spf,
-    const uint32_t *__restrict__ sslot, const BlkRec *__restrict__ recs, const uint2 *__restrict__ rlist, int cmax,
-    double inv_prod_in)
+    const uint32_t *__restrict__ sslot, double inv_prod_in)
 {
     using pf_t = typename std::conditional<PFI, uint32_t, double>::type;
     const int r = (int)g.r, K = (int)g.K;
     const int smax = min(r, wsfx);
     const uint32_t S = g.S;
-
-    uint32_t beta, Bk, PEh;
+    const uint32_t i = item * 64u + lane;
+    if (i >= g.nb * S) return;
+    const uint32_t b = i / S;
+    const uint32_t beta = i - b * S;
     pf_t pfp = (pf_t)1;
+    uint32_t rem = b, PE = 0, Bk = 0;
+    int P = K, cnt = 0;
+    for (int q = 0; q < h; q++) {
+        int x = 0;
+        if (q < h - 1) {
+            x = P;
+            uint32_t t = Th[q * wt + x];
+            while (t > rem) { --x; t = Th[q * wt + x]; }
+            rem -= t;
+        }
+        const uint64_t te = TE[q * wt + x + r];
+        Bk += (uint32_t)te;
+        if (P > x) {
+            lpe[cnt * BLK_THREADS + tid] = PE;
+            lmd[cnt * BLK_THREADS + tid] = (uint8_t)q;
+            cnt++;
+            if (PFI) pfp *= (pf_t)f32[P - x];
+            else pfp *= (pf_t)fd[P - x];
+        }
+        PE += (uint32_t)(te >> 32);
+        P = x;
+        if (P == 0) {
+            const uint64_t ce = CTE[r * (h + 1) + q + 1];
+            Bk += (uint32_t)ce;
+            PE += (uint32_t)(ce >> 32);
+            break;
+        }
+    }
+    const uint32_t PEh = PE;
+    const uint32_t cidx = Bk + beta;
     cd acc = {0.0, 0.0};
-    double pfd = 1.0;   // prod alpha! of a whole-wave block (from its record)
-    if (g.nch != 0) {
-        // ---- one chunk of one block: alpha is wave-uniform, its walk
-        // precomputed in the block's record
-        const uint32_t b = item / g.nch;
-        const uint32_t c = item - b * g.nch;
-        beta = c * 64u + lane;
-        const uint32_t rb = g.rec0 + b;
-        const BlkRec rec = recs[rb];
-        Bk = rec.Bk;
-        PEh = rec.PEh;
-        pfd = rec.pf;
-        const int cnt = (int)rec.cnt;
-        if (beta >= S) return;
-        if (g.nher) blk_heralds(acc, parent, her + g.her_off, g.nher, su, mfree, Bk + beta);
-        const uint32_t boff = beta << 4;
-        const cd *pp = parent + g.par_base;
-        const uint2 *rl = rlist + (size_t)rb * cmax;
-        for (int d0 = 0; d0 < cnt; d0 += 4) {
-            cd pv[4], uv[4];
+    if (g.nher) blk_heralds(acc, parent, her + g.her_off, g.nher, su, mfree, cidx);
+    const cd *pp = parent + g.par_base;
+    for (int d0 = 0; d0 < cnt; d0 += 4) {
+        cd pv[4], uv[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                pv[t] = cd{0.0, 0.0};
-                uv[t] = cd{0.0, 0.0};
-                if (d0 + t < cnt) {
-                    const uint2 e = rl[d0 + t];
-                    const char *pb = reinterpret_cast<const char *>(pp + (Bk - e.x));
-                    pv[t] = *reinterpret_cast<const cd *>(pb + boff);
-                    uv[t] = su[e.y];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);   // zero past the list: adds an exact 0
-        }
-    } else {
-        // ---- 64 consecutive children of the group, each lane its own block
-        const uint32_t i = item * 64u + lane;
-        if (i >= g.nb * S) return;
-        const uint32_t b = i / S;
-        beta = i - b * S;
-        uint32_t rem = b, PE = 0;
-        int P = K, cnt = 0;
-        Bk = 0;
-        for (int q = 0; q < h; q++) {
-            int x = 0;
-            if (q < h - 1) {
-                x = P;
-                uint32_t t = Th[q * wt + x];
-                while (t > rem) { --x; t = Th[q * wt + x]; }
-                rem -= t;
-            }
-            const uint64_t te = TE[q * wt + x + r];
-            Bk += (uint32_t)te;
-            if (P > x) {
-                lpe[cnt * BLK_THREADS + tid] = PE;
-                lmd[cnt * BLK_THREADS + tid] = (uint8_t)q;
-                cnt++;
-                if (PFI) pfp *= (pf_t)f32[P - x];
-                else pfp *= (pf_t)fd[P - x];
-            }
-            PE += (uint32_t)(te >> 32);
-            P = x;
-            if (P == 0) {
-                const uint64_t ce = CTE[r * (h + 1) + q + 1];
-                Bk += (uint32_t)ce;
-                PE += (uint32_t)(ce >> 32);
-                break;
+        for (int t = 0; t < 4; t++) {
+            pv[t] = cd{0.0, 0.0};
+            uv[t] = cd{0.0, 0.0};
+            if (d0 + t < cnt) {
+                pv[t] = pp[cidx - lpe[(d0 + t) * BLK_THREADS + tid]];
+                uv[t] = su[lmd[(d0 + t) * BLK_THREADS + tid]];
             }
         }
-        PEh = PE;
-        const uint32_t cidx = Bk + beta;
-        if (g.nher) blk_heralds(acc, parent, her + g.her_off, g.nher, su, mfree, cidx);
-        const cd *pp = parent + g.par_base;
-        for (int d0 = 0; d0 < cnt; d0 += 4) {
-            cd pv[4], uv[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                pv[t] = cd{0.0, 0.0};
-                uv[t] = cd{0.0, 0.0};
-                if (d0 + t < cnt) {
-                    pv[t] = pp[cidx - lpe[(d0 + t) * BLK_THREADS + tid]];
-                    uv[t] = su[lmd[(d0 + t) * BLK_THREADS + tid]];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);   // zero past the list: adds an exact 0
-        }
+        for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);   // zero past the list: adds an exact 0
     }
-
-    // ---- suffix parents, from the group's slot table (slot words loaded
-    // eight at a time, then their parents four at a time)
-    if (smax > 0) {
-        const uint32_t *ts = sslot + g.tslot + beta;
-        const char *ps = reinterpret_cast<const char *>(parent + g.par_base + (Bk - PEh));
-#pragma unroll
-        for (int h0 = 0; h0 < BLK_WMAX; h0 += 8) {
-            if (h0 < smax) {
-                uint32_t sw[8];
-#pragma unroll
-                for (int d = 0; d < 8; d++) sw[d] = h0 + d < smax ? ts[(size_t)(h0 + d) * S] : 0xffffffffu;
-#pragma unroll
-                for (int d0 = 0; d0 < 8; d0 += 4) {
-                    if (h0 + d0 < smax) {
-                        cd pv[4], uv[4];
-#pragma unroll
-                        for (int t = 0; t < 4; t++) {
-                            pv[t] = cd{0.0, 0.0};
-                            uv[t] = cd{0.0, 0.0};
-                            if (sw[d0 + t] != 0xffffffffu) {
-                                pv[t] = *reinterpret_cast<const cd *>(ps + ((sw[d0 + t] >> 5) << 4));
-                                uv[t] = su[h + (int)(sw[d0 + t] & 31u)];
-                            }
-                        }
-#pragma unroll
-                        for (int t = 0; t < 4; t++) cfma_acc(acc, pv[t], uv[t]);
-                    }
-                }
-            }
-        }
-    }
-
+    if (smax > 0)
+        blk_suffix(acc, reinterpret_cast<const char *>(parent + g.par_base + (Bk - PEh)), sslot + g.tslot + beta, S,
+                   smax, su, h);
     const size_t o = (size_t)g.out_base + Bk + beta;
     if (MODE == 0) {
         reinterpret_cast<cd *>(out)[o] = acc;
     } else {
-        const double pf = (g.nch != 0 ? pfd : (double)pfp) * spf[g.tpf + beta];
+        const double pf = (double)pfp * spf[g.tpf + beta];
         reinterpret_cast<double *>(out)[o] = (acc.r * acc.r + acc.i * acc.i) * pf * inv_prod_in;
     }
 }
@@ -238,14 +283,13 @@ __device__ __forceinline__ void blk_item(
 // same group at a time and each on a contiguous part of it (each XCD has its
 // own L2).  Waves take items at a stride of the XCD's waves.
 template <int MODE, bool PFI>
-__global__ void __launch_bounds__(BLK_THREADS) __attribute__((amdgpu_waves_per_eu(8))) slos_block_kernel(
+__global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
     const cd *__restrict__ parent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk, int k,
     uint32_t nchild, int h, int wsfx, int mfree, int mloc, const int32_t *__restrict__ rowmap,
     const BlkGroup *__restrict__ groups, int ngroups, const uint2 *__restrict__ her,
     const uint32_t *__restrict__ Th, const uint64_t *__restrict__ TE, const uint64_t *__restrict__ CTE, int wt,
     const uint32_t *__restrict__ f32, const double *__restrict__ fd, const double *__restrict__ spf,
-    const uint32_t *__restrict__ sslot, const BlkRec *__restrict__ recs, const uint2 *__restrict__ rlist, int cmax,
-    double inv_prod_in, int sentinel)
+    const uint32_t *__restrict__ sslot, const BlkRec *__restrict__ recs, double inv_prod_in, int sentinel)
 {
     extern __shared__ uint64_t s_blk[];
     cd *su = reinterpret_cast<cd *>(s_blk);                               // local mode -> U[row, mk]
@@ -264,9 +308,14 @@ __global__ void __launch_bounds__(BLK_THREADS) __attribute__((amdgpu_waves_per_e
         const BlkGroup g = groups[gi];
         const uint32_t share = (g.nitems + 7u) >> 3;
         const uint32_t lo = min(x * share, g.nitems), hi = min(lo + share, g.nitems);
-        for (uint32_t item = lo + j * (BLK_THREADS / 64) + wave; item < hi; item += stride)
-            blk_item<MODE, PFI>(g, item, lane, tid, parent, out, su, lpe, lmd, h, wsfx, mfree, her, Th, TE, CTE, wt,
-                                f32, fd, spf, sslot, recs, rlist, cmax, inv_prod_in);
+        if (g.nch != 0) {
+            for (uint32_t item = lo + j * (BLK_THREADS / 64) + wave; item < hi; item += stride)
+                blk_item_a<MODE>(g, item, lane, parent, out, su, h, wsfx, mfree, her, spf, sslot, recs, inv_prod_in);
+        } else {
+            for (uint32_t item = lo + j * (BLK_THREADS / 64) + wave; item < hi; item += stride)
+                blk_item_b<MODE, PFI>(g, item, lane, tid, parent, out, su, lpe, lmd, h, wsfx, mfree, her, Th, TE, CTE,
+                                      wt, f32, fd, spf, sslot, inv_prod_in);
+        }
     }
 }
 
@@ -389,39 +438,33 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
     // groups per layer: each sub-layer, suffix photons r = 0..kf; the blocks
     // of whole-wave groups get their walk precomputed (records)
     lay.cmax = std::max(1, std::min(h, nmax));
-    const int cmax = lay.cmax;
     std::vector<BlkRec> recs;
-    std::vector<uint32_t> rlist;  // (PE, mode) pairs, cmax per record
     auto add_records = [&](int K, int r) {
         std::vector<int> alpha((size_t)h, 0);
         alpha[0] = K;
         const uint64_t nb = nstates(h, K);
+        std::vector<int> after((size_t)h + 1, 0);
         for (uint64_t i = 0; i < nb; i++) {
             BlkRec rec{};
             uint64_t Bk = 0, PE = 0;
             double pf = 1.0;
-            int R = r;   // photons in modes q+1.. (prefix part plus the suffix)
-            std::vector<int> after((size_t)h + 1, 0);
             for (int q = h - 1; q >= 0; q--) after[(size_t)q] = after[(size_t)q + 1] + alpha[(size_t)q];
-            const size_t l0 = rlist.size();
-            rlist.resize(l0 + 2 * (size_t)cmax, 0);
             int cnt = 0;
             for (int q = 0; q < h; q++) {
-                R = r + after[(size_t)q + 1];
-                const uint64_t tev = te[(size_t)q * wt + R];
+                const uint64_t tev = te[(size_t)q * wt + r + after[(size_t)q + 1]];
                 Bk += tev & 0xffffffffull;
                 if (alpha[(size_t)q] > 0) {
-                    rlist[l0 + 2 * (size_t)cnt] = (uint32_t)PE;
-                    rlist[l0 + 2 * (size_t)cnt + 1] = (uint32_t)q;
+                    rec.w[4 + cnt / 4] |= (uint32_t)q << (8 * (cnt % 4));
+                    rec.w[10 + cnt] = (uint32_t)PE;
                     cnt++;
                     for (int v = 2; v <= alpha[(size_t)q]; v++) pf *= v;
                 }
                 PE += tev >> 32;
             }
-            rec.Bk = (uint32_t)Bk;
-            rec.PEh = (uint32_t)PE;
-            rec.cnt = (uint32_t)cnt;
-            rec.pf = pf;
+            rec.w[0] = (uint32_t)Bk;
+            rec.w[1] = (uint32_t)PE;
+            rec.w[2] = (uint32_t)cnt;
+            std::memcpy(&rec.w[8], &pf, sizeof(double));
             recs.push_back(rec);
             if (i + 1 < nb) {   // next alpha in descending lexicographic order
                 const int tail = alpha[(size_t)h - 1];
@@ -433,6 +476,8 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
             }
         }
     };
+    // records hold at most 16 occupied prefix modes and mode indices below 256
+    const bool use_records = lay.cmax <= BLK_REC_MAX && h <= 256;
     std::vector<uint32_t> hert;   // (herald index, parent sub-layer offset) pairs
     lay.groups.assign((size_t)lay.nlayers + 1, {});
     lay.items.assign((size_t)lay.nlayers + 1, 0);
@@ -453,13 +498,13 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
                 BlkGroup g{};
                 g.nb = (uint32_t)nstates(h, sub.kf - r);
                 g.S = (uint32_t)nstates(w, r);
-                if (g.S >= BLK_A_MIN) {
+                if (g.S >= BLK_A_MIN && use_records) {
                     g.nch = (g.S + 63) / 64;
                     g.nitems = g.nb * g.nch;
                     g.rec0 = (uint32_t)recs.size();
                     add_records(sub.kf - r, r);
-                    if (recs.size() > (64ull << 20) / sizeof(BlkRec)) {
-                        set_error("block SLOS layout: more than 2^21 block records (use a wider suffix)");
+                    if (recs.size() > (256ull << 20) / sizeof(BlkRec)) {
+                        set_error("block SLOS layout: block records above 256 MiB (use a wider suffix)");
                         return ERR_RANGE;
                     }
                 } else {
@@ -499,7 +544,6 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
     lay.off_slot = off; off += al(sizeof(uint32_t) * std::max<size_t>(sslot.size(), 1));
     lay.nrec = recs.size();
     lay.off_rec = off; off += al(sizeof(BlkRec) * std::max<size_t>(recs.size(), 1));
-    lay.off_rlist = off; off += al(sizeof(uint32_t) * std::max<size_t>(rlist.size(), 2));
     lay.bytes = off;
     std::vector<char> &host = *image;
     host.assign(off, 0);
@@ -516,7 +560,6 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
     std::memcpy(host.data() + lay.off_pf, spf.data(), sizeof(double) * spf.size());
     if (!sslot.empty()) std::memcpy(host.data() + lay.off_slot, sslot.data(), sizeof(uint32_t) * sslot.size());
     if (!recs.empty()) std::memcpy(host.data() + lay.off_rec, recs.data(), sizeof(BlkRec) * recs.size());
-    if (!rlist.empty()) std::memcpy(host.data() + lay.off_rlist, rlist.data(), sizeof(uint32_t) * rlist.size());
     *out = std::move(lay);
     return OK;
 }
@@ -706,8 +749,7 @@ int blk_layer_launch(const BlkLayout &lay, const cd *parent, void *out, const cd
                        reinterpret_cast<const double *>(base + lay.off_fd),
                        reinterpret_cast<const double *>(base + lay.off_pf),
                        reinterpret_cast<const uint32_t *>(base + lay.off_slot),
-                       reinterpret_cast<const BlkRec *>(base + lay.off_rec),
-                       reinterpret_cast<const uint2 *>(base + lay.off_rlist), lay.cmax, inv_prod_in, sentinel ? 1 : 0);
+                       reinterpret_cast<const BlkRec *>(base + lay.off_rec), inv_prod_in, sentinel ? 1 : 0);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
 }

@@ -17,6 +17,7 @@
 #include "../../perceval_amd/csrc/capi.cpp"
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <algorithm>
 #include <map>
 
@@ -76,7 +77,6 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
     const int32_t *rowmap = (const int32_t *)(img.data() + lay.off_row);
     const uint32_t *her = (const uint32_t *)(img.data() + lay.off_her);
     const BlkRec *recs = (const BlkRec *)(img.data() + lay.off_rec);
-    const uint32_t *rlist = (const uint32_t *)(img.data() + lay.off_rlist);
     const size_t n_th = (lay.off_te - lay.off_th) / 4, n_te = (lay.off_cte - lay.off_te) / 8;
     const size_t n_pf = (lay.off_slot - lay.off_pf) / 8, n_slot = (lay.bytes - lay.off_slot) / 4;
     const size_t n_cte = (lay.off_f32 - lay.off_cte) / 8, n_her = (lay.group_off[1] - lay.off_her) / 8;
@@ -156,14 +156,15 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
                         const uint64_t rb = (uint64_t)g.rec0 + b;
                         CHECK(rb < lay.nrec, "%s: record index", what);
                         if (rb >= lay.nrec) continue;
-                        const BlkRec &rec = recs[rb];
-                        CHECK(rec.Bk == Bk && rec.PEh == PE && rec.cnt == lst.size() && rec.pf == pfp,
+                        const uint32_t *rw = recs[rb].w;
+                        double rpf;
+                        std::memcpy(&rpf, &rw[8], sizeof(double));
+                        CHECK(rw[0] == Bk && rw[1] == PE && rw[2] == lst.size() && rpf == pfp,
                               "%s: record of block %u differs from its walk", what, b);
-                        for (size_t d = 0; d < lst.size() && d < (size_t)lay.cmax; d++)
-                            CHECK(rlist[2 * (rb * lay.cmax + d)] == lst[d].first &&
-                                  (int)rlist[2 * (rb * lay.cmax + d) + 1] == lst[d].second,
+                        CHECK(rw[2] <= (uint32_t)BLK_REC_MAX, "%s: record list longer than 16", what);
+                        for (size_t d = 0; d < lst.size() && d < (size_t)BLK_REC_MAX; d++)
+                            CHECK(rw[10 + d] == lst[d].first && (int)((rw[4 + d / 4] >> (8 * (d % 4))) & 255u) == lst[d].second,
                                   "%s: record list of block %u", what, b);
-                        CHECK(rec.cnt <= (uint32_t)lay.cmax, "%s: record list longer than cmax", what);
                     }
                     if (beta >= S) continue;
                     const uint64_t loc = (uint64_t)Bk + beta;
