@@ -39,6 +39,7 @@ import json
 import math
 import os
 import sys
+import statistics
 import time
 
 import numpy as np
@@ -337,38 +338,38 @@ def bench_slos(args, res):
     plan = SlosPlan(inp)
     dU = torch.from_numpy(U).cuda()
     out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
-    reps = 5
+
+    def timed(reps=10, batches=3):
+        # 3 untimed runs, then the median of 3 batches of 10
+        for _ in range(3):
+            plan.run(dU, PROBS, out)
+        ms = []
+        for _ in range(batches):
+            with Timer() as t:
+                for _ in range(reps):
+                    plan.run(dU, PROBS, out)
+            ms.append(t.ms / reps)
+        return statistics.median(ms)
+
     per_kernel = {}
     # 1 per state, 4 per state in two phases, 6 successor runs of 3 children
     # per thread with a transposed gather, 8 prefix/suffix blocks, 0 the
     # library's own pick (the headline number)
     for kernel in args.slos_kernels:
         plan.set_kernel(kernel)
-        plan.run(dU, PROBS, out)
-        with Timer() as t:
-            for _ in range(reps):
-                plan.run(dU, PROBS, out)
-        per_kernel[kernel] = t.ms / reps
+        per_kernel[kernel] = timed()
     # suffix width of the block kernel
     default_w = plan.block_width
     per_width = {}
     plan.set_kernel(8)
     for w in args.slos_widths:
         plan.set_block_width(w)
-        plan.run(dU, PROBS, out)
-        with Timer() as t:
-            for _ in range(reps):
-                plan.run(dU, PROBS, out)
-        per_width[w] = t.ms / reps
+        per_width[w] = timed()
     plan.set_block_width(default_w)
     plan.set_kernel(0)
-    plan.run(dU, PROBS, out)
     if 0 not in per_kernel:
-        plan.run(dU, PROBS, out)
-        with Timer() as t:
-            for _ in range(reps):
-                plan.run(dU, PROBS, out)
-        per_kernel[0] = t.ms / reps
+        per_kernel[0] = timed()
+    plan.run(dU, PROBS, out)
     ms = per_kernel[0]
     total = float(out.sum().item())
     plan.close()

@@ -50,6 +50,15 @@ __device__ __forceinline__ cd cmul(cd a, cd b)
 __device__ __forceinline__ cd cadd(cd a, cd b) { return {a.r + b.r, a.i + b.i}; }
 
 // acc += a * b with two fused steps per component.
+// |a|^2 * pf * inv in one fixed rounding order: left to the compiler, the
+// sum of squares may be contracted into either fused form, so two kernels
+// turning the same coefficient into a probability could differ in the last
+// bit.  Every SLOS epilogue goes through this.
+__device__ __forceinline__ double prob_from_coef(cd a, double pf, double inv)
+{
+    return fma(a.r, a.r, a.i * a.i) * pf * inv;
+}
+
 __device__ __forceinline__ void cfma_acc(cd &acc, cd a, cd b)
 {
     acc.r = fma(a.r, b.r, acc.r);

@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(SLOS_THREADS) slos_layer_kernel(
         if (MODE == 0) {
             reinterpret_cast<cd *>(out)[idx] = acc;
         } else {
-            reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * pfact * inv_prod_in;
+            reinterpret_cast<double *>(out)[idx] = prob_from_coef(acc, pfact, inv_prod_in);
         }
     }
 }
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(SLOS4_THREADS) slos_layer2p_kernel(
         if (MODE == 0) {
             reinterpret_cast<cd *>(out)[idx] = acc;
         } else {
-            reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * pfact * inv_prod_in;
+            reinterpret_cast<double *>(out)[idx] = prob_from_coef(acc, pfact, inv_prod_in);
         }
     }
 }
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(SLOS7_THREADS) slos_layer_succ2_kernel(
             if (MODE == 0) {
                 reinterpret_cast<cd *>(out)[idx] = acc;
             } else {
-                reinterpret_cast<double *>(out)[idx] = (acc.r * acc.r + acc.i * acc.i) * (double)spf[cs] * inv_prod_in;
+                reinterpret_cast<double *>(out)[idx] = prob_from_coef(acc, (double)spf[cs], inv_prod_in);
             }
         }
         __syncthreads();
@@ -900,10 +900,6 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         PCVL_HIP_CHECK(hipGetLastError());
         return OK;
     }
-    // Layer 0 is the vacuum coefficient 1; layer 1 reads it from bufA[0].
-    hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(bufA), 2);
-    PCVL_HIP_CHECK(hipGetLastError());
-    const cd *prev = bufA;
     // Auto takes the block kernel where its layout was built (every layer
     // below 2^32 states), else the compact successor kernel where its packed
     // parent ranks and LDS fit (parent layers below 2^26 states), else the
@@ -913,6 +909,10 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         set_error("SLOS kernel not applicable to this plan's layer sizes");
         return ERR_ARG;
     }
+    // Layer 0 is the vacuum coefficient 1; layer 1 reads it from bufA[0].
+    hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(bufA), 2);
+    PCVL_HIP_CHECK(hipGetLastError());
+    const cd *prev = bufA;
     for (int k = 1; k <= p.n; k++) {
         const uint64_t nc = p.counts[k];
         const bool last = (k == p.n);

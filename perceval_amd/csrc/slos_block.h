@@ -80,6 +80,7 @@ struct BlkLayout {
     int w = 0, h = 0, wt = 0;
     int nlayers = 0;          // layers 1..nlayers
     int mloc = 0;             // U rows used: free modes then herald modes
+    bool row_identity = false; // local mode i is U row i (no herald modes)
     std::vector<uint64_t> counts;                // states of layer k (compact when masked), k = 0..nlayers
     std::vector<std::vector<BlkGroup>> groups;   // per layer k (index k)
     std::vector<uint64_t> items;                 // wave items per layer
@@ -94,7 +95,7 @@ struct BlkLayout {
 };
 
 // Suffix width used when the caller does not choose one (0): the widest
-// w <= 10 below m whose suffix tables stay under 16 MiB.
+// w <= 12 below m whose suffix tables stay under 32 MiB.
 int blk_default_width(int m, int nmax);
 
 // The layout of layers 1..n of the full space F(m, k), suffix width w

@@ -38,6 +38,13 @@ constexpr int BLK_THREADS = 256;
 constexpr int BLK_WMAX = 16;        // widest suffix
 constexpr uint32_t BLK_A_MIN = 32;  // smallest block given whole waves
 constexpr int BLK_WG_PER_CU = 8;    // workgroups per CU of the persistent launch
+constexpr int BLK_GROUP_LDS = 64;   // groups of a layer cached in LDS (more: read from global memory)
+
+// The walk tables (Th, TE, CTE, f32, fd: consecutive in the layout, from
+// off_th): byte offsets from Th and the total, a multiple of 256.
+struct BlkTabOffsets {
+    uint32_t te, cte, f32, fd, bytes;
+};
 
 static uint64_t blk_binom(int64_t a, int64_t b)
 {
@@ -62,6 +69,16 @@ static uint64_t nstates(int modes, int x)
 // Herald parents of the child at position loc of its sub-layer: entry (i, base)
 // gives parent[base + loc] times U[H_i, mk] (su[mfree + i]).  They come first
 // in the summation order (herald modes lead the local mode order).
+// Children are written once and read again only by the next layer's launch
+// (after the whole layer): nontemporal stores keep them from displacing the
+// parent lines that the running layer still re-reads from L2.
+__device__ __forceinline__ void blk_store(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void blk_store(cd *p, cd v)
+{
+    __builtin_nontemporal_store(v.r, &p->r);
+    __builtin_nontemporal_store(v.i, &p->i);
+}
+
 __device__ __forceinline__ void blk_heralds(cd &acc, const cd *__restrict__ parent, const uint2 *__restrict__ her,
                                             uint32_t nher, const cd *su, int mfree, uint32_t loc)
 {
@@ -190,11 +207,11 @@ __device__ __forceinline__ void blk_item_a(
     }
     const size_t o = (size_t)g.out_base + Bk + beta;
     if (MODE == 0) {
-        reinterpret_cast<cd *>(out)[o] = acc;
+        blk_store(reinterpret_cast<cd *>(out) + o, acc);
     } else {
         double pfa;
         __builtin_memcpy(&pfa, &rw[8], sizeof(double));
-        reinterpret_cast<double *>(out)[o] = (acc.r * acc.r + acc.i * acc.i) * (pfa * pfs) * inv_prod_in;
+        blk_store(reinterpret_cast<double *>(out) + o, prob_from_coef(acc, pfa * pfs, inv_prod_in));
     }
 }
 
@@ -270,10 +287,10 @@ __device__ __forceinline__ void blk_item_b(
                    smax, su, h);
     const size_t o = (size_t)g.out_base + Bk + beta;
     if (MODE == 0) {
-        reinterpret_cast<cd *>(out)[o] = acc;
+        blk_store(reinterpret_cast<cd *>(out) + o, acc);
     } else {
         const double pf = (double)pfp * spf[g.tpf + beta];
-        reinterpret_cast<double *>(out)[o] = (acc.r * acc.r + acc.i * acc.i) * pf * inv_prod_in;
+        blk_store(reinterpret_cast<double *>(out) + o, prob_from_coef(acc, pf, inv_prod_in));
     }
 }
 
@@ -287,16 +304,35 @@ __global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
     const cd *__restrict__ parent, void *__restrict__ out, const cd *__restrict__ U, int m, int mk, int k,
     uint32_t nchild, int h, int wsfx, int mfree, int mloc, const int32_t *__restrict__ rowmap,
     const BlkGroup *__restrict__ groups, int ngroups, const uint2 *__restrict__ her,
-    const uint32_t *__restrict__ Th, const uint64_t *__restrict__ TE, const uint64_t *__restrict__ CTE, int wt,
-    const uint32_t *__restrict__ f32, const double *__restrict__ fd, const double *__restrict__ spf,
+    const char *__restrict__ tabs, BlkTabOffsets to, int wt, const double *__restrict__ spf,
     const uint32_t *__restrict__ sslot, const BlkRec *__restrict__ recs, double inv_prod_in, int sentinel)
 {
     extern __shared__ uint64_t s_blk[];
-    cd *su = reinterpret_cast<cd *>(s_blk);                               // local mode -> U[row, mk]
-    uint32_t *lpe = reinterpret_cast<uint32_t *>(su + mloc);              // k x THREADS (lists of waves over several blocks)
+    BlkGroup *sg = reinterpret_cast<BlkGroup *>(s_blk);                   // the layer's groups (up to BLK_GROUP_LDS)
+    cd *su = reinterpret_cast<cd *>(sg + BLK_GROUP_LDS);                  // local mode -> U[row, mk]
+    char *stab = reinterpret_cast<char *>(su + mloc);                     // walk tables of the per-lane walk
+    uint32_t *lpe = reinterpret_cast<uint32_t *>(stab + to.bytes);        // k x THREADS (lists of waves over several blocks)
     uint8_t *lmd = reinterpret_cast<uint8_t *>(lpe + (size_t)k * BLK_THREADS);
     const int tid = threadIdx.x;
-    for (int e = tid; e < mloc; e += BLK_THREADS) su[e] = U[(size_t)rowmap[e] * m + mk];
+    // the per-lane walk reads its tables once per prefix mode, one read
+    // depending on the last: from LDS, not through the memory hierarchy
+    for (int e = tid; e < (int)(to.bytes / sizeof(uint4)); e += BLK_THREADS)
+        reinterpret_cast<uint4 *>(stab)[e] = reinterpret_cast<const uint4 *>(tabs)[e];
+    const uint32_t *Th = reinterpret_cast<const uint32_t *>(stab);
+    const uint64_t *TE = reinterpret_cast<const uint64_t *>(stab + to.te);
+    const uint64_t *CTE = reinterpret_cast<const uint64_t *>(stab + to.cte);
+    const uint32_t *f32 = reinterpret_cast<const uint32_t *>(stab + to.f32);
+    const double *fd = reinterpret_cast<const double *>(stab + to.fd);
+    // rowmap null: local mode e is row e (unmasked layouts), one dependent load less
+    for (int e = tid; e < mloc; e += BLK_THREADS) su[e] = U[(size_t)(rowmap ? rowmap[e] : e) * m + mk];
+    // the group table is read once per group by every wave: one parallel
+    // fetch into LDS instead of a dependent global load per group
+    const bool glds = ngroups <= BLK_GROUP_LDS;
+    if (glds) {
+        const uint4 *gsrc = reinterpret_cast<const uint4 *>(groups);
+        uint4 *gdst = reinterpret_cast<uint4 *>(sg);
+        for (int e = tid; e < ngroups * (int)(sizeof(BlkGroup) / sizeof(uint4)); e += BLK_THREADS) gdst[e] = gsrc[e];
+    }
     if (sentinel && blockIdx.x == 0 && tid == 0) reinterpret_cast<cd *>(out)[nchild] = cd{0.0, 0.0};
     __syncthreads();
 
@@ -304,15 +340,30 @@ __global__ void __launch_bounds__(BLK_THREADS) slos_block_kernel(
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t x = blockIdx.x & 7u, j = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;
     const uint32_t stride = per_xcd * (BLK_THREADS / 64);
+    const uint32_t wl = j * (BLK_THREADS / 64) + wave;   // the wave's index among its XCD's waves
+    uint32_t cum = 0;                                      // XCD items of the previous groups, mod stride
     for (int gi = 0; gi < ngroups; gi++) {
-        const BlkGroup g = groups[gi];
+        BlkGroup g;
+        if (glds) {   // uniform: keep the fields in scalar registers
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(sg + gi);
+            uint32_t *dst = reinterpret_cast<uint32_t *>(&g);
+#pragma unroll
+            for (int f = 0; f < (int)(sizeof(BlkGroup) / 4); f++) dst[f] = __builtin_amdgcn_readfirstlane(src[f]);
+        } else {
+            g = groups[gi];
+        }
         const uint32_t share = (g.nitems + 7u) >> 3;
         const uint32_t lo = min(x * share, g.nitems), hi = min(lo + share, g.nitems);
+        // the XCD's waves take its items round robin, starting where the
+        // previous group stopped: a layer with few items per group gives
+        // each wave about one item in all, not one in every group
+        const uint32_t first = lo + (wl + stride - cum) % stride;
+        cum = (cum + (hi - lo)) % stride;
         if (g.nch != 0) {
-            for (uint32_t item = lo + j * (BLK_THREADS / 64) + wave; item < hi; item += stride)
+            for (uint32_t item = first; item < hi; item += stride)
                 blk_item_a<MODE>(g, item, lane, parent, out, su, h, wsfx, mfree, her, spf, sslot, recs, inv_prod_in);
         } else {
-            for (uint32_t item = lo + j * (BLK_THREADS / 64) + wave; item < hi; item += stride)
+            for (uint32_t item = first; item < hi; item += stride)
                 blk_item_b<MODE, PFI>(g, item, lane, tid, parent, out, su, lpe, lmd, h, wsfx, mfree, her, Th, TE, CTE,
                                       wt, f32, fd, spf, sslot, inv_prod_in);
         }
@@ -338,8 +389,8 @@ static uint64_t suffix_table_words(int w, int nmax)
 
 int blk_default_width(int m, int nmax)
 {
-    int w = std::min(10, m - 1);
-    while (w > 1 && suffix_table_words(w, nmax) * 4 > (16u << 20)) w--;
+    int w = std::min(12, m - 1);
+    while (w > 1 && suffix_table_words(w, nmax) * 4 > (32u << 20)) w--;
     return std::max(w, 1);
 }
 
@@ -371,6 +422,8 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
     lay.m = m; lay.nmax = nmax; lay.w = w; lay.h = m - w; lay.wt = nmax + 2;
     lay.nlayers = (int)subs.size() - 1;
     lay.mloc = (int)rowmap.size();
+    lay.row_identity = true;
+    for (size_t i = 0; i < rowmap.size(); i++) lay.row_identity = lay.row_identity && rowmap[i] == (int32_t)i;
     lay.counts = counts;
     const int h = lay.h, wt = lay.wt;
     // tables
@@ -586,7 +639,8 @@ int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<ch
 // whose lanes walk their own blocks.
 static size_t blk_lds_bytes(const BlkLayout &lay, int k)
 {
-    return sizeof(cd) * lay.mloc + (sizeof(uint32_t) + 1) * (size_t)k * BLK_THREADS;
+    return sizeof(BlkGroup) * BLK_GROUP_LDS + sizeof(cd) * lay.mloc + (lay.off_row - lay.off_th) +
+           (sizeof(uint32_t) + 1) * (size_t)k * BLK_THREADS;
 }
 
 int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out)
@@ -737,16 +791,16 @@ int blk_layer_launch(const BlkLayout &lay, const cd *parent, void *out, const cd
     const uint32_t nchild = (uint32_t)lay.counts[(size_t)k];
     const size_t lds = blk_lds_bytes(lay, k);
     const bool pfi = lay.nmax <= 12;
+    const BlkTabOffsets to = {(uint32_t)(lay.off_te - lay.off_th), (uint32_t)(lay.off_cte - lay.off_th),
+                              (uint32_t)(lay.off_f32 - lay.off_th), (uint32_t)(lay.off_fd - lay.off_th),
+                              (uint32_t)(lay.off_row - lay.off_th)};
     auto kp = probs ? (pfi ? slos_block_kernel<1, true> : slos_block_kernel<1, false>)
                     : (pfi ? slos_block_kernel<0, true> : slos_block_kernel<0, false>);
     hipLaunchKernelGGL(kp, dim3(grid), dim3(BLK_THREADS), lds, stream, parent, out, U, m, mk, k, nchild,
-                       lay.h, lay.w, lay.m, lay.mloc, reinterpret_cast<const int32_t *>(base + lay.off_row), groups,
+                       lay.h, lay.w, lay.m, lay.mloc,
+                       lay.row_identity ? nullptr : reinterpret_cast<const int32_t *>(base + lay.off_row), groups,
                        (int)lay.groups[(size_t)k].size(), reinterpret_cast<const uint2 *>(base + lay.off_her),
-                       reinterpret_cast<const uint32_t *>(base + lay.off_th),
-                       reinterpret_cast<const uint64_t *>(base + lay.off_te),
-                       reinterpret_cast<const uint64_t *>(base + lay.off_cte), lay.wt,
-                       reinterpret_cast<const uint32_t *>(base + lay.off_f32),
-                       reinterpret_cast<const double *>(base + lay.off_fd),
+                       base + lay.off_th, to, lay.wt,
                        reinterpret_cast<const double *>(base + lay.off_pf),
                        reinterpret_cast<const uint32_t *>(base + lay.off_slot),
                        reinterpret_cast<const BlkRec *>(base + lay.off_rec), inv_prod_in, sentinel ? 1 : 0);

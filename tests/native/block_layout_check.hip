@@ -97,14 +97,17 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
         const uint32_t per_xcd = grid >> 3, stride = per_xcd * 4;
         std::vector<std::vector<uint8_t>> seen((size_t)ngroups);
         for (int gi = 0; gi < ngroups; gi++) seen[(size_t)gi].assign(groups[gi].nitems, 0);
-        for (uint32_t wg = 0; wg < grid; wg++)
+        for (uint32_t wg = 0; wg < grid; wg++) {
+          uint32_t cum = 0;   // the kernel's rotation: XCD items of the previous groups, mod stride
           for (int gi = 0; gi < ngroups; gi++) {
             const BlkGroup g = groups[gi];
             const uint32_t x = wg & 7u, j = wg >> 3;
             const uint32_t share = (g.nitems + 7u) >> 3;
             const uint32_t lo = std::min(x * share, g.nitems), hi = std::min(lo + share, g.nitems);
+            const uint32_t cum_g = cum;
+            cum = (cum + (hi - lo)) % stride;
             for (uint32_t wave = 0; wave < 4; wave++)
-              for (uint32_t item = lo + j * 4 + wave; item < hi; item += stride) {
+              for (uint32_t item = lo + (j * 4 + wave + stride - cum_g) % stride; item < hi; item += stride) {
                 CHECK(seen[(size_t)gi][item] == 0, "%s: item %u of group %d taken twice", what, item, gi);
                 seen[(size_t)gi][item] = 1;
                 const int r = (int)g.r, K = (int)g.K, smax = std::min(r, w);
@@ -220,6 +223,7 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
                 }
               }
           }
+        }
         for (int gi = 0; gi < ngroups; gi++)
             for (uint32_t it = 0; it < groups[gi].nitems; it++)
                 CHECK(seen[(size_t)gi][it] == 1, "%s: item %u of group %d not taken", what, it, gi);
