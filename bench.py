@@ -24,6 +24,8 @@ configurations are reported beside it:
                 samples (samples/s)
   copy          1 GiB device copy: the HBM rate a copy attains, and the
                 reference that scales FETCH_SIZE / WRITE_SIZE readings
+  cache         (only with --sections cache) read rate against working-set
+                size, and the XCD of each workgroup of a 2,048-workgroup grid
 The perm section also times a pure-FMA FP64 probe, the vector rate the chip
 sustains at the clock it holds under FP64 load.
 
@@ -195,6 +197,40 @@ def bench_copy(args, res):
     ms = t.ms / 3
     res["copy_1GiB"] = {"ms": ms, "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}
     log(f"[bench] copy 1 GiB: {ms:.3f} ms, {2 * nbytes / (ms * 1e-3) / 1e9:.0f} GB/s (read + write)")
+
+
+def bench_cache(args, res):
+    """Probes behind the SLOS kernel's design (not run by default): the read
+    rate of a buffer read over and over against its size (where the last-level
+    caches stop holding a working set), and the XCD that runs each workgroup
+    of a 2,048-workgroup grid (the persistent launches assume blockIdx % 8)."""
+    import torch
+    from perceval_amd import _native
+    lib = _native.load()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    blocks = cus * 16
+    sink = torch.empty(blocks * 256, dtype=torch.float64, device="cuda")
+    rates = {}
+    for mb in (2, 16, 32, 64, 80, 96, 128, 256, 512):
+        nbytes = mb << 20
+        buf = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        reps = max(1, (8 << 30) // nbytes)
+
+        def run():
+            _native.check(lib.pcvl_probe_read(_native.ptr(buf), nbytes, reps, blocks, _native.ptr(sink),
+                                              _native.stream_handle()), "read probe")
+        run()
+        with Timer() as t:
+            run()
+        rates[str(mb)] = nbytes * reps / (t.ms * 1e-3) / 1e12
+        del buf
+    xcc = torch.empty(2048, dtype=torch.int32, device="cuda")
+    _native.check(lib.pcvl_probe_xcc(2048, _native.ptr(xcc), _native.stream_handle()), "xcc probe")
+    got = xcc.cpu().numpy()
+    match = int((got == (np.arange(2048) % 8)).sum())
+    res["cache_probe"] = {"read_TBps_by_MB": rates, "xcd_is_blockidx_mod_8": f"{match} of 2048 workgroups"}
+    log("[bench] read rate by working set (MB: TB/s): " + ", ".join(f"{k}: {v:.1f}" for k, v in rates.items()) +
+        f"; XCD = blockIdx % 8 for {match} of 2048 workgroups")
 
 
 def bench_gurvits(args, res):
@@ -675,7 +711,7 @@ def main():
     for s in sections:
         {"perm": bench_perm, "mixed": bench_mixed, "slos": bench_slos, "gradient": bench_gradient, "copy": bench_copy,
          "gurvits": bench_gurvits, "slos_tree": bench_slos_tree, "sampling_multi": bench_sampling_multi,
-         "sampling": bench_sampling}[s](args, res)
+         "sampling": bench_sampling, "cache": bench_cache}[s](args, res)
 
     perm = res.get("perm", {"ms": float("nan"), "perms_per_s": float("nan"), "tflops": float("nan")})
     line = {

@@ -397,6 +397,11 @@ int pcvl_probe_fp64_mix(int blocks, int64_t iters, int mode, double *d_out, int6
  * threads (16-byte loads).  d_out: blocks*256 doubles. */
 int pcvl_probe_read(const double *d_buf, uint64_t bytes, int reps, int blocks, double *d_out, hipStream_t stream);
 
+/* Measurement helper, no counterpart in the reference: the XCD that runs
+ * each of `blocks` workgroups (its XCC_ID hardware register), into
+ * d_out[0..blocks).  The persistent SLOS launches assume XCD = blockIdx % 8. */
+int pcvl_probe_xcc(int blocks, int32_t *d_out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,7 @@ SIGNATURES = {
     "pcvl_mis_gather": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p]),
     "pcvl_probe_fp64_fma": (c_int, [c_int, c_i64, c_void_p, ctypes.POINTER(c_i64), c_void_p]),
     "pcvl_probe_read": (c_int, [c_void_p, c_u64, c_int, c_int, c_void_p, c_void_p]),
+    "pcvl_probe_xcc": (c_int, [c_int, c_void_p, c_void_p]),
     "pcvl_probe_fp64_mix": (c_int, [c_int, c_i64, c_int, c_void_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
                                     c_void_p]),
     "pcvl_gurvits_batch": (c_int, [c_void_p, P(c_i32), P(c_i64), c_i64, c_i64, c_u64, c_u64, c_void_p, c_void_p,

@@ -71,6 +71,13 @@ __global__ void __launch_bounds__(256) probe_read_kernel(const double2 *__restri
     out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// Which XCD (accelerator complex die) runs each workgroup: its hardware
+// XCC_ID register, read by lane 0 and stored by a vector store.
+__global__ void __launch_bounds__(64) probe_xcc_kernel(int32_t *__restrict__ out)
+{
+    if (threadIdx.x == 0) out[blockIdx.x] = (int32_t)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15);
+}
+
 } // namespace pcvl
 
 using namespace pcvl;
@@ -81,6 +88,14 @@ extern "C" int pcvl_probe_read(const double *d_buf, uint64_t bytes, int reps, in
     PCVL_ARG_CHECK(d_buf && d_out && bytes >= 16 && reps > 0 && blocks > 0, "bad probe arguments");
     hipLaunchKernelGGL(probe_read_kernel, dim3(blocks), dim3(256), 0, stream,
                        reinterpret_cast<const double2 *>(d_buf), bytes / 16, reps, d_out);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_probe_xcc(int blocks, int32_t *d_out, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(blocks > 0 && d_out, "bad probe arguments");
+    hipLaunchKernelGGL(probe_xcc_kernel, dim3(blocks), dim3(64), 0, stream, d_out);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
 }
