@@ -350,6 +350,7 @@ def bench_mixed(args, res):
         for _ in range(3):
             g.replay()
     graph_ms = t.ms / 3
+    g.close()
     mplan.close()
     # the project's autotuner on this batch shape (chunk length), cached per device
     from perceval_amd import autotune
@@ -405,6 +406,15 @@ def bench_slos(args, res):
     plan.set_kernel(0)
     if 0 not in per_kernel:
         per_kernel[0] = timed()
+    # the default plan captured in a HIP graph (10 runs per replay)
+    from perceval_amd.graphs import graphed_slos
+    g = graphed_slos(plan, dU, PROBS, out, repeats=10)
+    g.replay()
+    with Timer() as t:
+        for _ in range(3):
+            g.replay()
+    graph_ms = t.ms / 30
+    g.close()
     plan.run(dU, PROBS, out)
     ms = per_kernel[0]
     total = float(out.sum().item())
@@ -412,13 +422,13 @@ def bench_slos(args, res):
     gbytes = slos_bytes(m, n) / 1e9
     res["slos_m24_n10"] = {"ms": ms, "outputs": plan.count, "algorithmic_GB": gbytes,
                            "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,
-                           "sum_probs": total, "block_width": default_w,
+                           "sum_probs": total, "block_width": default_w, "ms_graph_replay": graph_ms,
                            "ms_by_kernel": {name: per_kernel[kid] for kid, name in
                                             ((1, "per_state"), (4, "two_phase"), (6, "successor_run3"),
                                              (8, "prefix_suffix_blocks")) if kid in per_kernel},
                            "ms_by_block_width": {str(w): v for w, v in per_width.items()}}
     log(f"[bench] SLOS m=24 n=10: {ms:.3f} ms, {gbytes / (ms * 1e-3):.0f} GB/s algorithmic, sum={total:.15f}; "
-        f"by kernel " + ", ".join(f"{k}: {v:.3f}" for k, v in per_kernel.items()) +
+        f"HIP-graph replay {graph_ms:.3f} ms; by kernel " + ", ".join(f"{k}: {v:.3f}" for k, v in per_kernel.items()) +
         "; block kernel by width " + ", ".join(f"{w}: {v:.3f}" for w, v in per_width.items()))
     if not args.no_cpu:
         import oracle

@@ -361,6 +361,26 @@ int pcvl_mis_gather(const int32_t *d_prop, const int64_t *d_state, int64_t B, in
                     int32_t *d_out, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * HIP graphs of plan executions.  No counterpart in the reference (its
+ * arithmetic has no device launches); for callers that replay a fixed
+ * workload -- the same plan on the same device buffers -- with one launch
+ * call instead of host work per kernel.  Capture runs on a private stream;
+ * the graph reads and writes the buffers given here at every launch, so they
+ * must outlive it.
+ * ------------------------------------------------------------------------- */
+typedef struct pcvl_graph_s *pcvl_graph;
+
+/* `repeats` (1..1024) executions of a permanent plan on fixed buffers. */
+int pcvl_perm_graph_create(pcvl_perm_plan plan, const double *d_mats, double *d_out, int repeats,
+                           pcvl_graph *out);
+/* `repeats` executions of a SLOS plan (arguments as pcvl_slos_plan_execute). */
+int pcvl_slos_graph_create(pcvl_slos_plan plan, const double *d_U, int mode, double *d_out, void *d_work,
+                           size_t work_bytes, int repeats, pcvl_graph *out);
+/* Enqueue the whole graph on `stream`. */
+int pcvl_graph_launch(pcvl_graph graph, hipStream_t stream);
+int pcvl_graph_destroy(pcvl_graph graph);
+
+/* ---------------------------------------------------------------------------
  * Approximate permanents (NaiveApprox backend).
  * ------------------------------------------------------------------------- */
 

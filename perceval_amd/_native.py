@@ -93,6 +93,11 @@ SIGNATURES = {
     "pcvl_probe_fp64_fma": (c_int, [c_int, c_i64, c_void_p, ctypes.POINTER(c_i64), c_void_p]),
     "pcvl_probe_read": (c_int, [c_void_p, c_u64, c_int, c_int, c_void_p, c_void_p]),
     "pcvl_probe_xcc": (c_int, [c_int, c_void_p, c_void_p]),
+    "pcvl_perm_graph_create": (c_int, [c_void_p, c_void_p, c_void_p, c_int, ctypes.POINTER(c_void_p)]),
+    "pcvl_slos_graph_create": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_int,
+                                       ctypes.POINTER(c_void_p)]),
+    "pcvl_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "pcvl_graph_destroy": (c_int, [c_void_p]),
     "pcvl_probe_fp64_mix": (c_int, [c_int, c_i64, c_int, c_void_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
                                     c_void_p]),
     "pcvl_gurvits_batch": (c_int, [c_void_p, P(c_i32), P(c_i64), c_i64, c_i64, c_u64, c_u64, c_void_p, c_void_p,

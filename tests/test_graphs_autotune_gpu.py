@@ -32,20 +32,40 @@ def test_graph_replays_match_eager(gpu):
 
 def test_graph_slos_replays(gpu):
     import torch
-    from perceval_amd.graphs import GraphedCall
-    from perceval_amd.slos import PROBS, SlosPlan
+    from perceval_amd.graphs import graphed_slos
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan
     from perceval_amd.utils.workloads import haar_unitary
-    plan = SlosPlan([1, 1, 1, 0, 0, 0, 0])
-    dU = torch.from_numpy(haar_unitary(7, seed=1)).cuda()
-    out = torch.empty(plan.count, dtype=torch.float64, device="cuda")
-    eager = plan.run(dU, PROBS).clone()
-    plan._workspace(dU.device)
-    g = GraphedCall(lambda: plan.run(dU, PROBS, out))
-    for _ in range(3):
-        out.fill_(-1)
-        g.replay()
-        torch.cuda.synchronize()
-        assert torch.equal(out, eager)
+    for inp, mode in (([1, 1, 1, 0, 0, 0, 0], PROBS), ([1, 0, 2, 0, 1, 1, 0, 0, 1], COEFS)):
+        plan = SlosPlan(inp)
+        dU = torch.from_numpy(haar_unitary(len(inp), seed=len(inp))).cuda()
+        eager = plan.run(dU, mode).clone()
+        out = torch.empty_like(eager)
+        g = graphed_slos(plan, dU, mode, out, repeats=2)
+        for rep in range(3):
+            out.fill_(-1)
+            _ = torch.randn(1000, device="cuda").sum().item()   # unrelated eager work between replays
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out, eager), f"replay {rep} differs"
+        g.close()
+        plan.close()
+
+
+def test_graph_rejects_bad_arguments(gpu):
+    import ctypes
+    import torch
+    from perceval_amd import _native
+    from perceval_amd.permanent import PermanentPlan
+    lib = _native.load()
+    plan = PermanentPlan([5, 6])
+    d = torch.zeros(61, dtype=torch.complex128, device="cuda")
+    out = torch.zeros(2, dtype=torch.complex128, device="cuda")
+    g = ctypes.c_void_p()
+    assert lib.pcvl_perm_graph_create(plan._plan, _native.ptr(d), _native.ptr(out), 0, ctypes.byref(g)) == -1
+    assert lib.pcvl_perm_graph_create(None, _native.ptr(d), _native.ptr(out), 1, ctypes.byref(g)) == -1
+    assert lib.pcvl_graph_launch(None, None) == -1
+    assert lib.pcvl_graph_destroy(None) == 0
+    plan.close()
 
 
 def test_autotune_picks_a_measured_config(gpu, tmp_path, monkeypatch):
