@@ -258,6 +258,20 @@ int pcvl_mesh_adjoint(const int32_t *h_kind, const int32_t *h_mode, const int32_
                       int nsets, const int32_t *h_in_occ, const int32_t *h_targets, int ntargets,
                       double *d_loss, double *d_grad, double *d_probs, hipStream_t stream);
 
+/* The same as a plan: pcvl_mesh_plan_create validates and uploads the mesh
+ * program, the input state and the targets once (allocates and synchronises);
+ * pcvl_mesh_plan_execute then only enqueues the kernel for nsets phase vectors
+ * (no allocation, no synchronisation; graph-capturable), for training loops
+ * that evaluate the same circuit and targets at new phases every step.
+ * pcvl_mesh_adjoint is create + execute + synchronise + destroy. */
+typedef struct pcvl_mesh_plan_s *pcvl_mesh_plan;
+int pcvl_mesh_plan_create(const int32_t *h_kind, const int32_t *h_mode, const int32_t *h_param,
+                          const double *h_const, int nops, int m, int nparams, const int32_t *h_in_occ,
+                          const int32_t *h_targets, int ntargets, pcvl_mesh_plan *out);
+int pcvl_mesh_plan_execute(pcvl_mesh_plan plan, const double *d_phases, int nsets, double *d_loss,
+                           double *d_grad, double *d_probs, hipStream_t stream);
+int pcvl_mesh_plan_destroy(pcvl_mesh_plan plan);
+
 /* ---------------------------------------------------------------------------
  * Learned Metropolis independence sampler -- no counterpart in the reference
  * (whose sampler is xq.Clifford2017, perceval/backends/_clifford2017.py:52-57).

@@ -104,6 +104,10 @@ SIGNATURES = {
                                    c_void_p]),
     "pcvl_mesh_adjoint": (c_int, [P(c_i32), P(c_i32), P(c_i32), P(c_double), c_int, c_int, c_void_p, c_int, c_int,
                                   P(c_i32), P(c_i32), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pcvl_mesh_plan_create": (c_int, [P(c_i32), P(c_i32), P(c_i32), P(c_double), c_int, c_int, c_int, P(c_i32),
+                                      P(c_i32), c_int, ctypes.POINTER(c_void_p)]),
+    "pcvl_mesh_plan_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pcvl_mesh_plan_destroy": (c_int, [c_void_p]),
     "pcvl_cc_samples": (c_int, [c_void_p, c_int, P(c_i32), c_i64, c_u64, c_u64, c_void_p, c_void_p, c_size_t,
                                 c_void_p]),
     "pcvl_cc_samples_multi": (c_int, [c_void_p, c_int, P(c_i32), P(c_i64), c_int, c_u64, c_u64, c_void_p, c_void_p]),

@@ -19,6 +19,10 @@
 #include "common.h"
 #include "../../include/pcvl_hip.h"
 
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <string>
 #include <vector>
 
 namespace pcvl {
@@ -36,40 +40,54 @@ struct MeshOp {
     double c[8];     // kind 0: c[0] = constant phase; kind 1: 2x2 block row-major (re, im)
 };
 
-struct AdjShared {
-    cd U[ADJ_MAX_M * ADJ_MAX_M];
-    cd Z[ADJ_MAX_M * ADJ_MAX_M];
-    cd amp[ADJ_MAX_TARGETS];
-    cd cof[ADJ_MAX_TARGETS * ADJ_MAX_N * ADJ_MAX_N];
-    double red[ADJ_THREADS];
-};
-
 __device__ __forceinline__ cd cexpi(double ph) { return {cos(ph), sin(ph)}; }
 __device__ __forceinline__ cd cconj(cd a) { return {a.r, -a.i}; }
 
-// Glynn with derivatives for one target row r of an n x n matrix M (row-major
-// in registers of the caller): returns perm(M) (from row r == 0's thread) and
-// d perm / d M[r][c] for c < n.
-template <int N>
-__device__ void glynn_with_row_derivs(const cd *M, int r, cd &perm, cd *dr)
+// Sum over the 64 lanes of a wave, every lane receiving it (butterfly: a
+// fixed order, so the result does not depend on timing).
+__device__ __forceinline__ double wave_sum(double v)
 {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Glynn with derivatives for target row r of M = U[rows, cols] (n = N; U in
+// LDS, row-major m x m, rows the target's modes, cols the input's): writes
+// nrm * d perm / d M[r][c] to cof[c] for c < N, and nrm * perm(M) to *amp
+// when r == 0.  The matrix is read from LDS as the Gray walk flips its rows,
+// so no N x N copy sits in registers.
+template <int N>
+__device__ void glynn_with_row_derivs(const cd *U, int m, const int32_t *rows, const int32_t *cols, int r, double nrm,
+                                      cd *cof, cd *amp)
+{
+    cd perm, dr[N];
+    int col[N];
+#pragma unroll
+    for (int c = 0; c < N; c++) col[c] = cols[c];
     cd cs[N];
-    for (int c = 0; c < N; c++) {
-        cd s = {0.0, 0.0};
-        for (int i = 0; i < N; i++) s = cadd(s, M[i * N + c]);
-        cs[c] = s;
+#pragma unroll
+    for (int c = 0; c < N; c++) cs[c] = {0.0, 0.0};
+    for (int i = 0; i < N; i++) {
+        const cd *row = U + (size_t)rows[i] * m;
+#pragma unroll
+        for (int c = 0; c < N; c++) cs[c] = cadd(cs[c], row[col[c]]);
     }
     perm = {0.0, 0.0};
+#pragma unroll
     for (int c = 0; c < N; c++) dr[c] = {0.0, 0.0};
     uint32_t g = 0;
     for (uint32_t t = 0; t < (1u << (N - 1)); t++) {
         if (t) {
             const int b = __builtin_ctz(t);
             g ^= 1u << b;
-            const double s = ((g >> b) & 1) ? -2.0 : 2.0;
+            const double sgn = ((g >> b) & 1) ? -2.0 : 2.0;
+            const cd *row = U + (size_t)rows[b + 1] * m;
+#pragma unroll
             for (int c = 0; c < N; c++) {
-                cs[c].r = fma(s, M[(b + 1) * N + c].r, cs[c].r);
-                cs[c].i = fma(s, M[(b + 1) * N + c].i, cs[c].i);
+                const cd v = row[col[c]];
+                cs[c].r = fma(sgn, v.r, cs[c].r);
+                cs[c].i = fma(sgn, v.i, cs[c].i);
             }
         }
         const double sign = (t & 1) ? -1.0 : 1.0;
@@ -77,8 +95,10 @@ __device__ void glynn_with_row_derivs(const cd *M, int r, cd &perm, cd *dr)
         const double dr_sign = (r == 0) ? 1.0 : (((g >> (r - 1)) & 1) ? -1.0 : 1.0);
         cd P[N];
         P[0] = {1.0, 0.0};
+#pragma unroll
         for (int c = 1; c < N; c++) P[c] = cmul(P[c - 1], cs[c - 1]);
         cd S = {sign, 0.0};
+#pragma unroll
         for (int c = N - 1; c >= 0; c--) {
             const cd ex = cmul(P[c], S);      // sign * prod_{c' != c} cs_c'
             dr[c].r = fma(dr_sign, ex.r, dr[c].r);
@@ -87,139 +107,239 @@ __device__ void glynn_with_row_derivs(const cd *M, int r, cd &perm, cd *dr)
         }
         perm = cadd(perm, S);                 // S = sign * prod_c cs_c
     }
-    const double sc = ldexp(1.0, -(N - 1));
-    perm = {perm.r * sc, perm.i * sc};
-    for (int c = 0; c < N; c++) dr[c] = {dr[c].r * sc, dr[c].i * sc};
+    const double sc = ldexp(1.0, -(N - 1)) * nrm;
+#pragma unroll
+    for (int c = 0; c < N; c++) cof[c] = {dr[c].r * sc, dr[c].i * sc};
+    if (r == 0) *amp = {perm.r * sc, perm.i * sc};
 }
 
 template <int N>
-__device__ __forceinline__ void dispatch_glynn(int n, const cd *M, int r, cd &perm, cd *dr)
+__device__ __forceinline__ void dispatch_glynn(int n, const cd *U, int m, const int32_t *rows, const int32_t *cols,
+                                               int r, double nrm, cd *cof, cd *amp)
 {
     if constexpr (N <= ADJ_MAX_N) {
-        if (n == N) { glynn_with_row_derivs<N>(M, r, perm, dr); return; }
-        dispatch_glynn<N + 1>(n, M, r, perm, dr);
+        if (n == N) { glynn_with_row_derivs<N>(U, m, rows, cols, r, nrm, cof, amp); return; }
+        dispatch_glynn<N + 1>(n, U, m, rows, cols, r, nrm, cof, amp);
     }
 }
 
-__global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
-    const MeshOp *__restrict__ ops, int nops, int m, const double *__restrict__ phases, int nparams, int nsets,
-    const int32_t *__restrict__ in_modes, int n, const int32_t *__restrict__ target_modes,
-    const double *__restrict__ target_norm, int ntargets, double *__restrict__ loss_out, double *__restrict__ grad_out,
-    double *__restrict__ probs_out)
+// Index tables of one call, one int32 array (offsets in entries):
+//   lop[nops]        ops in level order: a level's ops act on disjoint modes,
+//                    so they commute and may run side by side
+//   lstart[nlev+1]   first entry of each level in lop
+//   rstart[m+1], rlist[targets*n]   per mode i, the (s*n + r) with rows_s[r] = i
+//   cstart[m+1], clist[n]           per mode j, the c with in[c] = j
+//   tm[targets*n], in[n]            target modes, input modes
+struct AdjIdx {
+    int lop, lstart, rstart, rlist, cstart, clist, tm, in, total;
+    int nlev;
+    int glog;   // lanes per op slot in the sweeps: 2^glog >= m
+};
+
+// Dynamic LDS of the kernel, in order (byte offsets rounded to 16): the mesh
+// program (when staged), e^{i phi} of the set's phases, U (m x m), Z (m x m),
+// amplitudes, cofactors (targets x n x n), target norms, a reduction buffer of
+// ADJ_THREADS doubles, the index tables.
+struct AdjLds {
+    size_t ops, zph, U, Z, amp, cof, tn, red, idx, bytes;
+};
+
+__host__ __device__ inline size_t adj_al16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+__host__ __device__ inline AdjLds adj_lds(int nops, int nparams, int m, int n, int ntargets, int nidx)
 {
-    __shared__ AdjShared sh;
+    AdjLds L;
+    size_t o = 0;
+    L.ops = o; o = adj_al16(o + sizeof(MeshOp) * (size_t)(nops > 0 ? nops : 1));
+    L.zph = o; o = adj_al16(o + sizeof(cd) * (size_t)(nparams > 0 ? nparams : 1));
+    L.U = o; o = adj_al16(o + sizeof(cd) * (size_t)m * m);
+    L.Z = o; o = adj_al16(o + sizeof(cd) * (size_t)m * m);
+    L.amp = o; o = adj_al16(o + sizeof(cd) * (size_t)ntargets);
+    L.cof = o; o = adj_al16(o + sizeof(cd) * (size_t)ntargets * n * n);
+    L.tn = o; o = adj_al16(o + sizeof(double) * (size_t)ntargets);
+    L.red = o; o = adj_al16(o + sizeof(double) * ADJ_THREADS);
+    L.idx = o; o = adj_al16(o + sizeof(int32_t) * (size_t)nidx);
+    L.bytes = o;
+    return L;
+}
+
+// One DPP step of a lane-group sum: v + v[lane ^ pattern] on the two 32-bit
+// halves of a double (the DPP lane moves are 32-bit).
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+    return v + __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Sum over each group of 2^glog lanes, every lane of the group receiving it,
+// in a fixed order: pairs within quads (quad_perm 1,0,3,2 then 2,3,0,1), then
+// the two quads of each 8 (row_half_mirror), the two 8s of each row of 16
+// (row_mirror), and across the two rows of a 32-lane group by a lane shuffle.
+// The DPP moves stay in registers; only the last step goes through the LDS
+// crossbar.
+__device__ __forceinline__ double group_sum(double v, int glog)
+{
+    if (glog >= 1) v = dpp_add<0xB1>(v);
+    if (glog >= 2) v = dpp_add<0x4E>(v);
+    if (glog >= 3) v = dpp_add<0x141>(v);
+    if (glog >= 4) v = dpp_add<0x140>(v);
+    if (glog >= 5) v += __shfl_xor(v, 16, 64);
+    return v;
+}
+
+// One workgroup per parameter set (grid-stride).  The forward product and the
+// backward sweep run in wave 0, level by level: the wave holds 64 >> glog op
+// slots of 2^glog lanes, lane `col` of a slot owning column col of U / F and
+// row col of Z.  An op touches one or two rows of F and the same columns of
+// Z, which each lane updates in its own column / row, and the ops of a level
+// act on disjoint modes, so the sweeps need no barrier; the one cross-lane
+// quantity, (F Z)_qq, is a reduction within the slot.
+__global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
+    const MeshOp *__restrict__ ops_g, int nops, int m, const double *__restrict__ phases, int nparams, int nsets,
+    int n, const int32_t *__restrict__ idx_g, AdjIdx X, const double *__restrict__ tn_g, int ntargets,
+    double *__restrict__ loss_out, double *__restrict__ grad_out, double *__restrict__ probs_out, int stage_ops)
+{
+    extern __shared__ uint64_t s_adj[];
+    const AdjLds L = adj_lds(stage_ops ? nops : 0, nparams, m, n, ntargets, X.total);
+    char *base = reinterpret_cast<char *>(s_adj);
+    cd *zph = reinterpret_cast<cd *>(base + L.zph);
+    MeshOp *ops_l = reinterpret_cast<MeshOp *>(base + L.ops);
+    const MeshOp *ops = stage_ops ? ops_l : ops_g;   // long programs are read from global memory
+    cd *U = reinterpret_cast<cd *>(base + L.U);
+    cd *Z = reinterpret_cast<cd *>(base + L.Z);
+    cd *amp = reinterpret_cast<cd *>(base + L.amp);
+    cd *cof = reinterpret_cast<cd *>(base + L.cof);
+    double *tn = reinterpret_cast<double *>(base + L.tn);
+    double *red = reinterpret_cast<double *>(base + L.red);
+    int32_t *idx = reinterpret_cast<int32_t *>(base + L.idx);
+    const int32_t *lop = idx + X.lop, *lstart = idx + X.lstart, *rstart = idx + X.rstart, *rlist = idx + X.rlist;
+    const int32_t *cstart = idx + X.cstart, *clist = idx + X.clist, *tm = idx + X.tm, *in_modes = idx + X.in;
     const int tid = threadIdx.x;
+    // the program and index tables, once per workgroup
+    if (stage_ops)
+        for (int e = tid; e < nops * (int)(sizeof(MeshOp) / 8); e += ADJ_THREADS)
+            reinterpret_cast<uint64_t *>(ops_l)[e] = reinterpret_cast<const uint64_t *>(ops_g)[e];
+    for (int e = tid; e < X.total; e += ADJ_THREADS) idx[e] = idx_g[e];
+    for (int e = tid; e < ntargets; e += ADJ_THREADS) tn[e] = tn_g[e];
+    __syncthreads();
+    const bool w0 = tid < 64;
+    const int glog = X.glog, slots = 64 >> glog;
+    const int slot = tid >> glog, col = tid & ((1 << glog) - 1);   // in wave 0
     for (int set = blockIdx.x; set < nsets; set += gridDim.x) {
         const double *ph = phases + (size_t)set * nparams;
-        // 1. U = L_K ... L_1: start from the identity, apply ops in circuit order
-        for (int e = tid; e < m * m; e += ADJ_THREADS) sh.U[e] = {(e / m == e % m) ? 1.0 : 0.0, 0.0};
+        for (int e = tid; e < nparams; e += ADJ_THREADS) zph[e] = cexpi(ph[e]);
         __syncthreads();
-        for (int p = 0; p < nops; p++) {
-            const MeshOp op = ops[p];
-            if (op.kind == 0) {
-                const cd z = cexpi(op.param >= 0 ? ph[op.param] : op.c[0]);
-                if (tid < m) sh.U[op.q * m + tid] = cmul(z, sh.U[op.q * m + tid]);
-            } else if (tid < m) {
-                const cd a = {op.c[0], op.c[1]}, b = {op.c[2], op.c[3]}, c = {op.c[4], op.c[5]}, d = {op.c[6], op.c[7]};
-                const cd x = sh.U[op.q * m + tid], y = sh.U[(op.q + 1) * m + tid];
-                sh.U[op.q * m + tid] = cadd(cmul(a, x), cmul(b, y));
-                sh.U[(op.q + 1) * m + tid] = cadd(cmul(c, x), cmul(d, y));
+        // 1. U = L_K ... L_1: start from the identity, apply the levels in circuit order
+        if (w0) {
+            if (slot == 0 && col < m)
+                for (int i = 0; i < m; i++) U[i * m + col] = {i == col ? 1.0 : 0.0, 0.0};
+            for (int lv = 0; lv < X.nlev; lv++) {
+                const int e1 = lstart[lv + 1];
+                for (int k = lstart[lv] + slot; k - slot < e1; k += slots) {
+                    if (k >= e1 || col >= m) continue;
+                    const MeshOp &op = ops[lop[k]];
+                    const int q = op.q;
+                    if (op.kind == 0) {
+                        const cd z = op.param >= 0 ? zph[op.param] : cexpi(op.c[0]);
+                        U[q * m + col] = cmul(z, U[q * m + col]);
+                    } else {
+                        const cd a = {op.c[0], op.c[1]}, b = {op.c[2], op.c[3]}, c = {op.c[4], op.c[5]},
+                                 d = {op.c[6], op.c[7]};
+                        const cd x = U[q * m + col], y = U[(q + 1) * m + col];
+                        U[q * m + col] = cadd(cmul(a, x), cmul(b, y));
+                        U[(q + 1) * m + col] = cadd(cmul(c, x), cmul(d, y));
+                    }
+                }
             }
-            __syncthreads();
         }
+        __syncthreads();
         // 2. amplitudes and cofactors: thread (target s, row r)
         for (int w = tid; w < ntargets * n; w += ADJ_THREADS) {
-            const int s = w / n, r = w % n;
-            cd M[ADJ_MAX_N * ADJ_MAX_N];
-            for (int i = 0; i < n; i++)
-                for (int c = 0; c < n; c++) M[i * n + c] = sh.U[target_modes[s * n + i] * m + in_modes[c]];
-            cd perm, dr[ADJ_MAX_N];
-            dispatch_glynn<1>(n, M, r, perm, dr);
-            const double nrm = target_norm[s];   // 1 / sqrt(prod s! prod t!)
-            for (int c = 0; c < n; c++) sh.cof[(s * n + r) * n + c] = {dr[c].r * nrm, dr[c].i * nrm};
-            if (r == 0) sh.amp[s] = {perm.r * nrm, perm.i * nrm};
+            const int s = w / n, r = w - (w / n) * n;
+            // scaled by 1 / sqrt(prod s! prod t!)
+            dispatch_glynn<1>(n, U, m, tm + s * n, in_modes, r, tn[s], cof + (size_t)(s * n + r) * n, amp + s);
         }
         __syncthreads();
         // loss and per-target probabilities
         if (tid < ntargets) {
-            const cd a = sh.amp[tid];
+            const cd a = amp[tid];
             const double pr = a.r * a.r + a.i * a.i;
             if (probs_out) probs_out[(size_t)set * ntargets + tid] = pr;
-            sh.red[tid] = pr;
+            red[tid] = pr;
         }
         __syncthreads();
         if (tid == 0) {
             double l = 0.0;
-            for (int s = 0; s < ntargets; s++) l += sh.red[s];
+            for (int s = 0; s < ntargets; s++) l += red[s];
             loss_out[set] = l;
         }
-        // G^T into Z: Z[j][i] = G[i][j] = sum_s conj(A_s) sum_{(r,c): rows_s[r]=i, in[c]=j} cof_src
+        // G^T into Z: Z[j][i] = G[i][j] = sum over (s, r) with rows_s[r] = i (increasing s, r) of
+        // conj(A_s) sum over c with in[c] = j of cof[s][r][c]
         for (int e = tid; e < m * m; e += ADJ_THREADS) {
-            const int j = e / m, i = e % m;
+            const int j = e / m, i = e - (e / m) * m;
             cd g = {0.0, 0.0};
-            for (int c = 0; c < n; c++) {
-                if (in_modes[c] != j) continue;
-                for (int s = 0; s < ntargets; s++) {
-                    const cd ca = cconj(sh.amp[s]);
-                    for (int r = 0; r < n; r++)
-                        if (target_modes[s * n + r] == i) cfma_acc(g, ca, sh.cof[(s * n + r) * n + c]);
+            const int c0 = cstart[j], c1 = cstart[j + 1];
+            if (c0 < c1)
+                for (int u = rstart[i]; u < rstart[i + 1]; u++) {
+                    const int sr = rlist[u];
+                    const cd ca = cconj(amp[sr / n]);
+                    for (int v = c0; v < c1; v++) cfma_acc(g, ca, cof[(size_t)sr * n + clist[v]]);
                 }
-            }
-            sh.Z[j * m + i] = g;
+            Z[j * m + i] = g;
         }
         __syncthreads();
-        // 3. backward sweep: F (in sh.U) <- L_p^H F, grad, Z <- Z L_p
-        for (int p = nops - 1; p >= 0; p--) {
-            const MeshOp op = ops[p];
-            if (op.kind == 0) {
-                const double phi = op.param >= 0 ? ph[op.param] : op.c[0];
-                const cd z = cexpi(phi);
-                if (tid < m) sh.U[op.q * m + tid] = cmul(cconj(z), sh.U[op.q * m + tid]);   // F <- L^H F
-                __syncthreads();
-                if (op.param >= 0) {
-                    // (F Z)_qq = sum_k F[q][k] Z[k][q]
-                    double re = 0.0, im = 0.0;
-                    if (tid < m) {
-                        const cd v = cmul(sh.U[op.q * m + tid], sh.Z[tid * m + op.q]);
-                        re = v.r; im = v.i;
+        // 3. backward sweep (wave 0), levels in reverse: F (in U) <- L_p^H F, grad, Z <- Z L_p
+        if (w0) {
+            for (int lv = X.nlev - 1; lv >= 0; lv--) {
+                const int e0 = lstart[lv], e1 = lstart[lv + 1];
+                for (int k0 = e0; k0 < e1; k0 += slots) {
+                    const int k = k0 + slot;
+                    const int pidx = k < e1 ? lop[k] : -1;
+                    const int kind = pidx >= 0 ? ops[pidx].kind : -1;
+                    const bool mine = pidx >= 0 && col < m;
+                    bool graded = false;
+                    cd z = {1.0, 0.0}, f = {0.0, 0.0}, zq = {0.0, 0.0};
+                    int q = 0, prm = -1;
+                    if (kind == 0) {
+                        const MeshOp &op = ops[pidx];
+                        q = op.q;
+                        prm = op.param;
+                        z = prm >= 0 ? zph[prm] : cexpi(op.c[0]);
+                        graded = prm >= 0;
+                        if (mine) {
+                            f = cmul(cconj(z), U[q * m + col]);   // F <- L^H F
+                            U[q * m + col] = f;
+                            zq = Z[col * m + q];
+                        }
                     }
-                    sh.red[tid] = re;
-                    __syncthreads();
-                    if (tid == 0) {
-                        double sr = 0.0, si = 0.0;
-                        for (int k = 0; k < m; k++) sr += sh.red[k];
-                        sh.red[ADJ_THREADS - 1] = sr;
-                        (void)si;
+                    // (F Z)_qq = sum_k F[q][k] Z[k][q]; dL/dphi = 2 Re(i z (F Z)_qq)
+                    if (__ballot(graded) != 0) {   // uniform: every lane takes part in the reduction
+                        const cd v = cmul(f, zq);
+                        const double fr = group_sum(v.r, glog), fi = group_sum(v.i, glog);
+                        if (graded && col == 0) grad_out[(size_t)set * nparams + prm] = -2.0 * (z.r * fi + z.i * fr);
                     }
-                    __syncthreads();
-                    const double fr = sh.red[ADJ_THREADS - 1];
-                    __syncthreads();
-                    sh.red[tid] = im;
-                    __syncthreads();
-                    if (tid == 0) {
-                        double si = 0.0;
-                        for (int k = 0; k < m; k++) si += sh.red[k];
-                        // dL/dphi = 2 Re(i z (F Z)_qq) = -2 (z.r * si + z.i * fr)
-                        grad_out[(size_t)set * nparams + op.param] = -2.0 * (z.r * si + z.i * fr);
+                    if (kind == 0 && mine) Z[col * m + q] = cmul(zq, z);   // Z <- Z L
+                    if (kind == 1 && mine) {
+                        const MeshOp &op = ops[pidx];
+                        const int q1 = op.q;
+                        const cd a = {op.c[0], op.c[1]}, b = {op.c[2], op.c[3]}, c = {op.c[4], op.c[5]},
+                                 d = {op.c[6], op.c[7]};
+                        // F <- L^H F on rows q, q+1: L^H = [[a*, c*], [b*, d*]]
+                        const cd x = U[q1 * m + col], y = U[(q1 + 1) * m + col];
+                        U[q1 * m + col] = cadd(cmul(cconj(a), x), cmul(cconj(c), y));
+                        U[(q1 + 1) * m + col] = cadd(cmul(cconj(b), x), cmul(cconj(d), y));
+                        // Z <- Z L on columns q, q+1
+                        const cd u = Z[col * m + q1], v = Z[col * m + q1 + 1];
+                        Z[col * m + q1] = cadd(cmul(u, a), cmul(v, c));
+                        Z[col * m + q1 + 1] = cadd(cmul(u, b), cmul(v, d));
                     }
-                    __syncthreads();
-                }
-                if (tid < m) sh.Z[tid * m + op.q] = cmul(sh.Z[tid * m + op.q], z);             // Z <- Z L
-            } else {
-                const cd a = {op.c[0], op.c[1]}, b = {op.c[2], op.c[3]}, c = {op.c[4], op.c[5]}, d = {op.c[6], op.c[7]};
-                if (tid < m) {
-                    // F <- L^H F on rows q, q+1: L^H = [[a*, c*], [b*, d*]]
-                    const cd x = sh.U[op.q * m + tid], y = sh.U[(op.q + 1) * m + tid];
-                    sh.U[op.q * m + tid] = cadd(cmul(cconj(a), x), cmul(cconj(c), y));
-                    sh.U[(op.q + 1) * m + tid] = cadd(cmul(cconj(b), x), cmul(cconj(d), y));
-                    // Z <- Z L on columns q, q+1
-                    const cd u = sh.Z[tid * m + op.q], v = sh.Z[tid * m + op.q + 1];
-                    sh.Z[tid * m + op.q] = cadd(cmul(u, a), cmul(v, c));
-                    sh.Z[tid * m + op.q + 1] = cadd(cmul(u, b), cmul(v, d));
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();   // U and Z are rewritten by the next set
     }
 }
 
@@ -227,18 +347,29 @@ __global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
 
 using namespace pcvl;
 
-extern "C" int pcvl_mesh_adjoint(const int32_t *h_kind, const int32_t *h_mode, const int32_t *h_param,
-                                 const double *h_const, int nops, int m, const double *d_phases, int nparams,
-                                 int nsets, const int32_t *h_in_occ, const int32_t *h_targets, int ntargets,
-                                 double *d_loss, double *d_grad, double *d_probs, hipStream_t stream)
+struct pcvl_mesh_plan_s {
+    int m = 0, n = 0, nops = 0, nparams = 0, ntargets = 0;
+    AdjIdx X{};
+    bool stage_ops = false;
+    size_t lds = 0;
+    char *d_blk = nullptr;            // program, index tables, target norms
+    const MeshOp *d_ops = nullptr;
+    const int32_t *d_idx = nullptr;
+    const double *d_tn = nullptr;
+    int grid_cap = 0;
+};
+
+extern "C" int pcvl_mesh_plan_create(const int32_t *h_kind, const int32_t *h_mode, const int32_t *h_param,
+                                     const double *h_const, int nops, int m, int nparams, const int32_t *h_in_occ,
+                                     const int32_t *h_targets, int ntargets, pcvl_mesh_plan *out)
 {
+    PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
+    *out = nullptr;
     PCVL_ARG_CHECK(m >= 1 && m <= ADJ_MAX_M, "mode count outside [1, 32]");
     PCVL_ARG_CHECK(nops >= 0 && (nops == 0 || (h_kind && h_mode && h_param && h_const)), "bad mesh program");
-    PCVL_ARG_CHECK(nsets >= 0 && nparams >= 0, "negative sizes");
+    PCVL_ARG_CHECK(nparams >= 0, "negative sizes");
     PCVL_ARG_CHECK(ntargets >= 1 && ntargets <= ADJ_MAX_TARGETS, "target count outside [1, 64]");
     PCVL_ARG_CHECK(h_in_occ && h_targets, "null input/target states");
-    if (nsets == 0) return OK;
-    PCVL_ARG_CHECK(d_phases && d_loss && d_grad, "null device buffer");
     std::vector<MeshOp> ops(nops);
     for (int p = 0; p < nops; p++) {
         MeshOp &o = ops[p];
@@ -274,28 +405,146 @@ extern "C" int pcvl_mesh_adjoint(const int32_t *h_kind, const int32_t *h_mode, c
         PCVL_ARG_CHECK(ns == n, "target photon number differs from the input's");
         tnorm[s] = 1.0 / sqrt(ps * pin);
     }
-    // small device copies of the program and index tables
-    const size_t b_ops = sizeof(MeshOp) * std::max(1, nops);
-    const size_t b_in = sizeof(int32_t) * n, b_t = sizeof(int32_t) * tmodes.size(), b_n = sizeof(double) * ntargets;
-    char *blk = nullptr;
-    PCVL_HIP_CHECK(hipMallocAsync((void **)&blk, b_ops + b_in + b_t + b_n + 64, stream));
-    MeshOp *d_ops = reinterpret_cast<MeshOp *>(blk);
-    int32_t *d_in = reinterpret_cast<int32_t *>(blk + b_ops);
-    int32_t *d_tm = reinterpret_cast<int32_t *>(blk + b_ops + b_in);
-    double *d_tn = reinterpret_cast<double *>(blk + ((b_ops + b_in + b_t + 7) & ~size_t(7)));
-    if (nops) PCVL_HIP_CHECK(hipMemcpyAsync(d_ops, ops.data(), sizeof(MeshOp) * nops, hipMemcpyHostToDevice, stream));
-    PCVL_HIP_CHECK(hipMemcpyAsync(d_in, in_modes.data(), b_in, hipMemcpyHostToDevice, stream));
-    PCVL_HIP_CHECK(hipMemcpyAsync(d_tm, tmodes.data(), b_t, hipMemcpyHostToDevice, stream));
-    PCVL_HIP_CHECK(hipMemcpyAsync(d_tn, tnorm.data(), b_n, hipMemcpyHostToDevice, stream));
+    // levels: an op goes one level above the last op on any of its modes
+    std::vector<int> last(m, 0), level(nops);
+    int nlev = 0;
+    for (int p = 0; p < nops; p++) {
+        const int q = ops[p].q, q2 = ops[p].kind == 1 ? q + 1 : q;
+        const int lv = std::max(last[q], last[q2]);
+        level[p] = lv;
+        last[q] = last[q2] = lv + 1;
+        nlev = std::max(nlev, lv + 1);
+    }
+    AdjIdx X;
+    X.nlev = nlev;
+    X.glog = 0;
+    while ((1 << X.glog) < m) X.glog++;
+    std::vector<int32_t> idx;
+    X.lop = (int)idx.size();
+    std::vector<int32_t> lstart(nlev + 1, 0);
+    for (int p = 0; p < nops; p++) lstart[level[p] + 1]++;
+    for (int lv = 0; lv < nlev; lv++) lstart[lv + 1] += lstart[lv];
+    {
+        std::vector<int32_t> lop(nops), fill(lstart.begin(), lstart.end() - 1);
+        for (int p = 0; p < nops; p++) lop[fill[level[p]]++] = p;   // circuit order within a level
+        idx.insert(idx.end(), lop.begin(), lop.end());
+    }
+    X.lstart = (int)idx.size();
+    idx.insert(idx.end(), lstart.begin(), lstart.end());
+    // per mode i the (s, r) whose target row is i, per mode j the c whose input column is j
+    X.rstart = (int)idx.size();
+    std::vector<int32_t> rstart(m + 1, 0), rlist;
+    for (int i = 0; i < m; i++) {
+        rstart[i] = (int32_t)rlist.size();
+        for (int e = 0; e < ntargets * n; e++)
+            if (tmodes[(size_t)e] == i) rlist.push_back(e);
+    }
+    rstart[m] = (int32_t)rlist.size();
+    idx.insert(idx.end(), rstart.begin(), rstart.end());
+    X.rlist = (int)idx.size();
+    idx.insert(idx.end(), rlist.begin(), rlist.end());
+    X.cstart = (int)idx.size();
+    std::vector<int32_t> cstart(m + 1, 0), clist;
+    for (int j = 0; j < m; j++) {
+        cstart[j] = (int32_t)clist.size();
+        for (int c = 0; c < n; c++)
+            if (in_modes[(size_t)c] == j) clist.push_back(c);
+    }
+    cstart[m] = (int32_t)clist.size();
+    idx.insert(idx.end(), cstart.begin(), cstart.end());
+    X.clist = (int)idx.size();
+    idx.insert(idx.end(), clist.begin(), clist.end());
+    X.tm = (int)idx.size();
+    idx.insert(idx.end(), tmodes.begin(), tmodes.end());
+    X.in = (int)idx.size();
+    idx.insert(idx.end(), in_modes.begin(), in_modes.end());
+    X.total = (int)idx.size();
+    // the program goes to LDS when it fits beside the rest (64 KiB budget), else
+    // it is read from global memory
+    const bool stage_ops = adj_lds(nops, nparams, m, n, ntargets, X.total).bytes <= 64 * 1024;
+    const AdjLds L = adj_lds(stage_ops ? nops : 0, nparams, m, n, ntargets, X.total);
+    PCVL_ARG_CHECK(L.bytes <= 160 * 1024, "too many phases or targets for the kernel's LDS");
+    auto *plan = new (std::nothrow) pcvl_mesh_plan_s();
+    if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
+    plan->m = m; plan->n = n; plan->nops = nops; plan->nparams = nparams; plan->ntargets = ntargets;
+    plan->X = X;
+    plan->stage_ops = stage_ops;
+    plan->lds = L.bytes;
+    // one device block: program, index tables, norms
+    const size_t b_ops = adj_al16(sizeof(MeshOp) * std::max(1, nops));
+    const size_t b_idx = adj_al16(sizeof(int32_t) * idx.size()), b_n = sizeof(double) * ntargets;
+    hipError_t e = hipMalloc((void **)&plan->d_blk, b_ops + b_idx + b_n);
+    if (e != hipSuccess) {
+        delete plan;
+        set_error(std::string("hipMalloc(mesh plan): ") + hipGetErrorString(e));
+        return ERR_ALLOC;
+    }
+    plan->d_ops = reinterpret_cast<const MeshOp *>(plan->d_blk);
+    plan->d_idx = reinterpret_cast<const int32_t *>(plan->d_blk + b_ops);
+    plan->d_tn = reinterpret_cast<const double *>(plan->d_blk + b_ops + b_idx);
+    if (nops) e = hipMemcpy(plan->d_blk, ops.data(), sizeof(MeshOp) * nops, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(plan->d_blk + b_ops, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(plan->d_blk + b_ops + b_idx, tnorm.data(), b_n, hipMemcpyHostToDevice);
     int dev = 0, cus = 0;
-    PCVL_HIP_CHECK(hipGetDevice(&dev));
-    PCVL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int grid = std::min(nsets, cus * 4);
-    hipLaunchKernelGGL(mesh_adjoint_kernel, dim3(grid), dim3(ADJ_THREADS), 0, stream, d_ops, nops, m, d_phases,
-                       nparams, nsets, d_in, n, d_tm, d_tn, ntargets, d_loss, d_grad, d_probs);
-    PCVL_HIP_CHECK(hipGetLastError());
-    // host vectors above are the sources of the async copies
-    PCVL_HIP_CHECK(hipStreamSynchronize(stream));
-    PCVL_HIP_CHECK(hipFreeAsync(blk, stream));
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) {
+        (void)hipFree(plan->d_blk);
+        delete plan;
+        set_error(std::string("mesh plan upload: ") + hipGetErrorString(e));
+        return ERR_HIP;
+    }
+    plan->grid_cap = cus * 4;
+    *out = plan;
     return OK;
+}
+
+extern "C" int pcvl_mesh_plan_execute(pcvl_mesh_plan plan, const double *d_phases, int nsets, double *d_loss,
+                                      double *d_grad, double *d_probs, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(plan != nullptr, "plan is null");
+    PCVL_ARG_CHECK(nsets >= 0, "negative set count");
+    if (nsets == 0) return OK;
+    PCVL_ARG_CHECK(d_phases && d_loss && d_grad, "null device buffer");
+    const pcvl_mesh_plan_s &P = *plan;
+    const int grid = std::min(nsets, P.grid_cap);
+    hipLaunchKernelGGL(mesh_adjoint_kernel, dim3(grid), dim3(ADJ_THREADS), P.lds, stream, P.d_ops, P.nops, P.m,
+                       d_phases, P.nparams, nsets, P.n, P.d_idx, P.X, P.d_tn, P.ntargets, d_loss, d_grad, d_probs,
+                       P.stage_ops ? 1 : 0);
+    PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mesh_plan_destroy(pcvl_mesh_plan plan)
+{
+    if (!plan) return OK;
+    hipError_t e = plan->d_blk ? hipFree(plan->d_blk) : hipSuccess;
+    delete plan;
+    if (e != hipSuccess) {
+        set_error(std::string("hipFree(mesh plan): ") + hipGetErrorString(e));
+        return ERR_HIP;
+    }
+    return OK;
+}
+
+extern "C" int pcvl_mesh_adjoint(const int32_t *h_kind, const int32_t *h_mode, const int32_t *h_param,
+                                 const double *h_const, int nops, int m, const double *d_phases, int nparams,
+                                 int nsets, const int32_t *h_in_occ, const int32_t *h_targets, int ntargets,
+                                 double *d_loss, double *d_grad, double *d_probs, hipStream_t stream)
+{
+    PCVL_ARG_CHECK(nsets >= 0 && nparams >= 0, "negative sizes");
+    pcvl_mesh_plan plan = nullptr;
+    int rc = pcvl_mesh_plan_create(h_kind, h_mode, h_param, h_const, nops, m, nparams, h_in_occ, h_targets, ntargets,
+                                   &plan);
+    if (rc != OK) return rc;
+    rc = pcvl_mesh_plan_execute(plan, d_phases, nsets, d_loss, d_grad, d_probs, stream);
+    if (rc == OK) {
+        const hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+            rc = ERR_HIP;
+        }
+    }
+    const int rc2 = pcvl_mesh_plan_destroy(plan);
+    return rc != OK ? rc : rc2;
 }

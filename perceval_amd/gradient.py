@@ -93,30 +93,66 @@ def _flatten(circuit: ACircuit, offset: int = 0):
         yield tuple(range(offset, offset + circuit.m)), circuit
 
 
+class MeshPlan:
+    """A mesh program, an input state and target outputs uploaded once
+    (pcvl_mesh_plan_create); each call enqueues the adjoint kernel for a batch
+    of phase vectors, with no allocation or synchronisation in the library."""
+
+    def __init__(self, program: MeshProgram, in_occ, targets):
+        self._lib = _native.load()
+        self.program = program
+        self.in_occ = np.ascontiguousarray(list(in_occ), dtype=np.int32)
+        self.targets = np.ascontiguousarray(np.asarray([list(t) for t in targets], dtype=np.int32))
+        self.ntargets = len(self.targets)
+        self._plan = ctypes.c_void_p()
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        _native.check(self._lib.pcvl_mesh_plan_create(
+            program.kind.ctypes.data_as(i32), program.mode.ctypes.data_as(i32), program.param.ctypes.data_as(i32),
+            program.const.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), program.nops, program.m,
+            program.nparams, self.in_occ.ctypes.data_as(i32), self.targets.ctypes.data_as(i32), self.ntargets,
+            ctypes.byref(self._plan)), "pcvl_mesh_plan_create")
+
+    def __call__(self, phases, want_probs: bool = False, stream=None):
+        import torch
+        if isinstance(phases, np.ndarray):
+            phases = torch.from_numpy(np.ascontiguousarray(phases, dtype=np.float64)).cuda()
+        phases = phases.to(torch.float64).contiguous()
+        B, Pn = phases.shape
+        if Pn != self.program.nparams:
+            raise ValueError(f"expected {self.program.nparams} phases per set, got {Pn}")
+        loss = torch.empty(B, dtype=torch.float64, device=phases.device)
+        grad = torch.empty((B, Pn), dtype=torch.float64, device=phases.device)
+        probs = torch.empty((B, self.ntargets), dtype=torch.float64, device=phases.device) if want_probs else None
+        _native.check(self._lib.pcvl_mesh_plan_execute(
+            self._plan, _native.ptr(phases), B, _native.ptr(loss), _native.ptr(grad),
+            _native.ptr(probs) if probs is not None else None, _native.stream_handle(stream)),
+            "pcvl_mesh_plan_execute")
+        return (loss, grad, probs) if want_probs else (loss, grad)
+
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            _native.check(self._lib.pcvl_mesh_plan_destroy(self._plan), "pcvl_mesh_plan_destroy")
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def mesh_adjoint(program: MeshProgram, phases, in_occ, targets, want_probs: bool = False):
     """Run the kernel.  phases: (B, P) float64 CUDA tensor or numpy.
-    Returns (loss (B,), grad (B, P)[, probs (B, T)]) as CUDA tensors."""
-    import torch
-    lib = _native.load()
-    if isinstance(phases, np.ndarray):
-        phases = torch.from_numpy(np.ascontiguousarray(phases, dtype=np.float64)).cuda()
-    phases = phases.to(torch.float64).contiguous()
-    B, Pn = phases.shape
-    if Pn != program.nparams:
-        raise ValueError(f"expected {program.nparams} phases per set, got {Pn}")
-    in_occ = np.ascontiguousarray(list(in_occ), dtype=np.int32)
+    Returns (loss (B,), grad (B, P)[, probs (B, T)]) as CUDA tensors.  The
+    program keeps one MeshPlan per (input state, targets)."""
+    in_occ = tuple(int(v) for v in in_occ)
     targets = np.ascontiguousarray(np.asarray([list(t) for t in targets], dtype=np.int32))
-    loss = torch.empty(B, dtype=torch.float64, device=phases.device)
-    grad = torch.empty((B, Pn), dtype=torch.float64, device=phases.device)
-    probs = torch.empty((B, len(targets)), dtype=torch.float64, device=phases.device) if want_probs else None
-    i32 = ctypes.POINTER(ctypes.c_int32)
-    _native.check(lib.pcvl_mesh_adjoint(
-        program.kind.ctypes.data_as(i32), program.mode.ctypes.data_as(i32), program.param.ctypes.data_as(i32),
-        program.const.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), program.nops, program.m,
-        _native.ptr(phases), Pn, B, in_occ.ctypes.data_as(i32), targets.ctypes.data_as(i32), len(targets),
-        _native.ptr(loss), _native.ptr(grad), _native.ptr(probs) if probs is not None else None,
-        _native.stream_handle()), "pcvl_mesh_adjoint")
-    return (loss, grad, probs) if want_probs else (loss, grad)
+    key = (in_occ, targets.shape, targets.tobytes())
+    plans = program.__dict__.setdefault("_plans", {})
+    plan = plans.get(key)
+    if plan is None:
+        plan = plans[key] = MeshPlan(program, in_occ, targets)
+    return plan(phases, want_probs)
 
 
 def _autograd_function():

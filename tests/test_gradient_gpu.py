@@ -85,3 +85,74 @@ def test_autograd_function(gpu):
     (loss * w).sum().backward()
     _, grad = mesh_adjoint(prog, phases, IN, targets)
     assert torch.allclose(x.grad, grad * w[:, None], rtol=0, atol=1e-15)
+
+
+def _random_targets(m, n, count, rng):
+    """`count` distinct n-photon occupation vectors of m modes, bunched ones included."""
+    seen, out = set(), []
+    while len(out) < count:
+        occ = np.bincount(rng.randint(0, m, size=n), minlength=m)
+        key = tuple(int(v) for v in occ)
+        if key not in seen:
+            seen.add(key)
+            out.append(occ.astype(np.int32))
+    return np.array(out, dtype=np.int32)
+
+
+@pytest.mark.parametrize("m,inp", [(4, [0, 1, 0, 0]), (32, [2, 1, 1, 1, 1, 1, 1] + [0] * 25)])
+def test_limits_bunched_and_long_program(gpu, m, inp):
+    """One photon on 4 modes (two of its outputs as targets), and the limits: 32 modes,
+    8 photons with a bunched input mode, 64 targets with bunched outputs, and a
+    Clements program of 992 phases, too long for the kernel to stage in LDS (read
+    from global memory instead).  Loss against the oracle, gradient against
+    central differences of the oracle loss on a few phases."""
+    from perceval_amd.gradient import MeshProgram, mesh_adjoint
+    rng = np.random.RandomState(m)
+    prog = MeshProgram(clements_mesh(m))
+    phases = rng.uniform(0, 2 * math.pi, size=(3, prog.nparams))
+    n = sum(inp)
+    targets = oracle.fock_enumerate(m, n)[:2] if m == 4 else _random_targets(m, n, 64, rng)
+
+    def loss_at(ph):
+        U = prog.unitary(ph)
+        return sum(abs(oracle.perm(oracle.submatrix(U, inp, t))) ** 2 /
+                   (oracle.prodnfact(t) * oracle.prodnfact(inp)) for t in targets)
+
+    loss, grad = mesh_adjoint(prog, phases, inp, targets)
+    loss, grad = loss.cpu().numpy(), grad.cpu().numpy()
+    h = 1e-6
+    for b in range(3):
+        assert abs(loss[b] - loss_at(phases[b])) < 1e-12
+        for k in rng.choice(prog.nparams, 4, replace=False):
+            pp = phases[b].copy(); pp[k] += h
+            pm = phases[b].copy(); pm[k] -= h
+            fd = (loss_at(pp) - loss_at(pm)) / (2 * h)
+            assert abs(grad[b, k] - fd) < 1e-12 + 1e-6 * abs(fd), (b, k, grad[b, k], fd)
+
+
+def test_one_shot_entry_matches_the_plan(gpu):
+    """pcvl_mesh_adjoint (create, execute, synchronise, destroy in one call)
+    against a MeshPlan executed twice on new phases: the same bits."""
+    import ctypes
+    import torch
+    from perceval_amd import _native
+    from perceval_amd.gradient import MeshPlan
+    _, prog, phases, targets = _setup(batch=64, seed=9)
+    plan = MeshPlan(prog, IN, targets)
+    lib = _native.load()
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    tg = np.ascontiguousarray(targets, dtype=np.int32)
+    inp = np.ascontiguousarray(IN, dtype=np.int32)
+    for shift in (0.0, 0.25):
+        ph = torch.from_numpy(phases + shift).cuda()
+        loss_p, grad_p, probs_p = plan(ph, want_probs=True)
+        loss = torch.empty_like(loss_p)
+        grad = torch.empty_like(grad_p)
+        probs = torch.empty_like(probs_p)
+        _native.check(lib.pcvl_mesh_adjoint(
+            prog.kind.ctypes.data_as(i32), prog.mode.ctypes.data_as(i32), prog.param.ctypes.data_as(i32),
+            prog.const.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), prog.nops, prog.m, _native.ptr(ph),
+            prog.nparams, 64, inp.ctypes.data_as(i32), tg.ctypes.data_as(i32), len(tg), _native.ptr(loss),
+            _native.ptr(grad), _native.ptr(probs), _native.stream_handle()), "pcvl_mesh_adjoint")
+        assert torch.equal(loss, loss_p) and torch.equal(grad, grad_p) and torch.equal(probs, probs_p)
+    plan.close()
