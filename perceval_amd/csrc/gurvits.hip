@@ -117,6 +117,119 @@ __global__ void __launch_bounds__(64) gurvits_kernel(const cd *__restrict__ mats
     }
 }
 
+// The same estimate with y = A x formed on the FP64 matrix cores, 16 samples
+// at a time: Y (NP x 16) = A (NP x NP) X (NP x 16, entries +-1), real and
+// imaginary parts as two products, by v_mfma_f64_16x16x4f64 over 16-row
+// blocks rb and 4-column blocks kb.  Operand layouts of that instruction
+// (checked on the device): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]
+// of the 16 x 4 and 4 x 16 operands, and D[l / 16 + 4 v][l % 16] in register v
+// of the 16 x 16 result.  So lane l works on sample s0 + l % 16, takes
+// x_j = bit j of that sample's signs for j = 4 kb + l / 16, and ends holding
+// rows l / 16 + 4 v of every row block of y; the product over i < n is taken
+// over those rows, then across the four lanes of the sample (lane ^ 16,
+// lane ^ 32).  Lanes 0-15 accumulate.  A's fragments stay in registers up to
+// NP = 32 and are read from LDS above.
+typedef double gv_d4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+__global__ void __launch_bounds__(64) gurvits_mfma_kernel(const cd *__restrict__ mats,
+                                                          const GurvitsItem *__restrict__ items, int64_t count,
+                                                          int parts, int64_t iters, int64_t per, uint64_t seed,
+                                                          uint64_t stream0, double *__restrict__ partial)
+{
+    constexpr int RB = NP / 16, KB = NP / 4;
+    constexpr bool AREG = NP <= 32;
+    __shared__ cd sA[NP * NP];
+    const int lane = threadIdx.x;
+    const int col = lane & 15, quad = lane >> 4;
+    const int64_t nwork = count * parts;
+    for (int64_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const int64_t b = wi / parts;
+        const int64_t q = wi % parts;
+        const int n = __builtin_amdgcn_readfirstlane(items[b].n);
+        const int64_t off = items[b].off;
+        for (int e = lane; e < NP * NP; e += 64) {
+            const int i = e / NP, j = e % NP;
+            sA[e] = (i < n && j < n) ? mats[off + (int64_t)i * n + j] : cd{0.0, 0.0};
+        }
+        __syncthreads();
+        double ar[AREG ? RB * KB : 1], ai[AREG ? RB * KB : 1];
+        if constexpr (AREG) {
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                for (int kb = 0; kb < KB; kb++) {
+                    const cd v = sA[(16 * rb + col) * NP + 4 * kb + quad];
+                    ar[rb * KB + kb] = v.r;
+                    ai[rb * KB + kb] = v.i;
+                }
+        }
+        const uint64_t mask = n >= 64 ? ~0ull : ((1ull << n) - 1);
+        const int64_t s_lo = q * per;
+        const int64_t s_hi = (s_lo + per < iters) ? s_lo + per : iters;
+        cd sum = {0.0, 0.0};
+        double sq = 0.0;
+        for (int64_t s0 = s_lo; s0 < s_hi; s0 += 16) {
+            const int64_t s = s0 + col;
+            const bool valid = s < s_hi;
+            const uint64_t bits = valid ? rand64(seed, stream0 + (uint64_t)b, (uint64_t)s) : 0ull;
+            gv_d4 yr[RB], yi[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                yr[rb] = gv_d4{0.0, 0.0, 0.0, 0.0};
+                yi[rb] = gv_d4{0.0, 0.0, 0.0, 0.0};
+            }
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) {
+                const double x = ((bits >> (4 * kb + quad)) & 1) ? -1.0 : 1.0;   // columns j >= n are zero in A
+#pragma unroll
+                for (int rb = 0; rb < RB; rb++) {
+                    double a_r, a_i;
+                    if constexpr (AREG) {
+                        a_r = ar[rb * KB + kb];
+                        a_i = ai[rb * KB + kb];
+                    } else {
+                        const cd v = sA[(16 * rb + col) * NP + 4 * kb + quad];
+                        a_r = v.r;
+                        a_i = v.i;
+                    }
+                    yr[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r, x, yr[rb], 0, 0, 0);
+                    yi[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i, x, yi[rb], 0, 0, 0);
+                }
+            }
+            // this lane's rows quad + 4 v of each block, then the sample's four lanes
+            cd p = {1.0, 0.0};
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (16 * rb + quad + 4 * v < n) p = cmul(p, cd{yr[rb][v], yi[rb][v]});
+            cd o = {__shfl_xor(p.r, 16, 64), __shfl_xor(p.i, 16, 64)};
+            p = cmul(p, o);
+            o = {__shfl_xor(p.r, 32, 64), __shfl_xor(p.i, 32, 64)};
+            p = cmul(p, o);
+            if (__builtin_popcountll(bits & mask) & 1) p = {-p.r, -p.i};
+            if (valid && quad == 0) {
+                sum.r += p.r;
+                sum.i += p.i;
+                sq = fma(p.r, p.r, fma(p.i, p.i, sq));
+            }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            sum.r += __shfl_xor(sum.r, o, 64);
+            sum.i += __shfl_xor(sum.i, o, 64);
+            sq += __shfl_xor(sq, o, 64);
+        }
+        if (lane == 0) {
+            partial[3 * wi] = sum.r;
+            partial[3 * wi + 1] = sum.i;
+            partial[3 * wi + 2] = sq;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void gurvits_finalize_kernel(const double *__restrict__ partial, int64_t count, int parts, int64_t iters,
                                         cd *__restrict__ est, double *__restrict__ err)
 {
@@ -143,12 +256,19 @@ static int launch_gurvits(const cd *mats, const GurvitsItem *items, int64_t coun
     int dev = 0, cus = 0, per_cu = 0;
     PCVL_HIP_CHECK(hipGetDevice(&dev));
     PCVL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&gurvits_kernel<NP>),
-                                                                64, 0));
+    // orders above 8: y = A x on the matrix cores (NP a multiple of 16)
+    constexpr bool MF = NP >= 16 && NP % 16 == 0;
+    const void *kp = MF ? reinterpret_cast<const void *>(&gurvits_mfma_kernel<MF ? NP : 16>)
+                        : reinterpret_cast<const void *>(&gurvits_kernel<NP>);
+    PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, 64, 0));
     if (per_cu < 1) per_cu = 1;
     const int grid = (int)std::min<int64_t>(count * parts, (int64_t)cus * per_cu);
-    hipLaunchKernelGGL((gurvits_kernel<NP>), dim3(grid), dim3(64), 0, st, mats, items, count, parts, iters, per, seed,
-                       stream0, partial);
+    if constexpr (MF)
+        hipLaunchKernelGGL((gurvits_mfma_kernel<NP>), dim3(grid), dim3(64), 0, st, mats, items, count, parts, iters,
+                           per, seed, stream0, partial);
+    else
+        hipLaunchKernelGGL((gurvits_kernel<NP>), dim3(grid), dim3(64), 0, st, mats, items, count, parts, iters, per,
+                           seed, stream0, partial);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
 }
@@ -194,9 +314,7 @@ extern "C" int pcvl_gurvits_batch(const double *d_mats, const int32_t *h_n, cons
         const cd *mats = reinterpret_cast<const cd *>(d_mats);
         if (nmax <= 8) rc = launch_gurvits<8>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
         else if (nmax <= 16) rc = launch_gurvits<16>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
-        else if (nmax <= 24) rc = launch_gurvits<24>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
         else if (nmax <= 32) rc = launch_gurvits<32>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
-        else if (nmax <= 40) rc = launch_gurvits<40>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
         else rc = launch_gurvits<48>(mats, d_items, count, parts, iters, per, seed, stream0, d_part, stream);
         if (rc == OK) {
             hipLaunchKernelGGL(gurvits_finalize_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
