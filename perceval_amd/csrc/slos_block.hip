@@ -8,20 +8,18 @@
 // Work.  The children of layer k are split into groups by r = photons in
 // the w suffix modes; group r holds N(h, k-r) blocks of S(r) = N(w, r)
 // children.  A wave is one work item:
-//   * S(r) >= 32: one 64-child chunk of one block.  The block index is
-//     wave-uniform, so the walk that unranks alpha runs on the scalar unit
-//     and leaves the parent base of every occupied prefix mode in a VGPR
-//     (lane d holds entry d).  Each prefix parent load is then a uniform base
-//     plus the lane's beta: one contiguous 1 KiB run per load instruction.
+//   * S(r) >= 32: one 64-child chunk of one block.  The block's walk
+//     (B_k(alpha), PE_h, its occupied prefix modes with their PE_j, and
+//     prod alpha!) is precomputed as a 128-byte record that one scalar load
+//     brings in; each prefix parent load is then a uniform base plus the
+//     lane's beta: one contiguous 1 KiB run per load instruction.
 //   * S(r) < 32: 64 consecutive children of the group, across blocks; each
-//     lane walks its own alpha and keeps its list in LDS.
+//     lane walks its own alpha (walk tables in LDS) and keeps its list in LDS.
 // The suffix parents come from a table per r (every block with the same r
 // has the same suffix structure): slot d of beta holds
 // (rank(beta - e_j) << 5) | (j - h) for its d-th occupied suffix mode j, or
-// all ones past its last one.  Groups run in increasing r, so the parent
-// groups r and r - 1 that a group reads stay in the last-level cache, and
-// each group gives XCD x a contiguous share of its workgroups (each XCD has
-// its own L2).
+// all ones past its last one.  Groups run in increasing r, and each group
+// gives XCD x a contiguous share of its items (each XCD has its own L2).
 #include "common.h"
 #include "slos_block.h"
 #include "../../include/pcvl_hip.h"
