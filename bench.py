@@ -26,6 +26,8 @@ configurations are reported beside it:
                 reference that scales FETCH_SIZE / WRITE_SIZE readings
   cache         (only with --sections cache) read rate against working-set
                 size, and the XCD of each workgroup of a 2,048-workgroup grid
+  cc            (only with --sections cc) plain Clifford-Clifford alone,
+                m = 60, n = 20
 The perm section also times a pure-FMA FP64 probe, the vector rate the chip
 sustains at the clock it holds under FP64 load.
 
@@ -231,6 +233,23 @@ def bench_cache(args, res):
     res["cache_probe"] = {"read_TBps_by_MB": rates, "xcd_is_blockidx_mod_8": f"{match} of 2048 workgroups"}
     log("[bench] read rate by working set (MB: TB/s): " + ", ".join(f"{k}: {v:.1f}" for k, v in rates.items()) +
         f"; XCD = blockIdx % 8 for {match} of 2048 workgroups")
+
+
+def bench_cc(args, res):
+    """Plain GPU Clifford-Clifford alone (not run by default; the sampling
+    section runs it too, with the learned sampler): m = 60, n = 20,
+    --samples samples, for profiling the sampler by itself."""
+    import torch
+    from perceval_amd.sampling import cc_samples_device
+    from perceval_amd.utils.workloads import haar_unitary
+    m, n, N = 60, 20, args.samples
+    dU = torch.from_numpy(haar_unitary(m, seed=60)).cuda()
+    inp = [1] * n + [0] * (m - n)
+    cc_samples_device(dU, inp, 256, seed=99)
+    with Timer() as t:
+        cc_samples_device(dU, inp, N, seed=1)
+    res["cc_m60_n20"] = {"samples": N, "ms": t.ms, "samples_per_s": N / (t.ms * 1e-3)}
+    log(f"[bench] plain CC alone: {N} samples in {t.ms:.1f} ms = {N / (t.ms * 1e-3):.0f} samples/s")
 
 
 def bench_gurvits(args, res):
@@ -721,7 +740,7 @@ def main():
     for s in sections:
         {"perm": bench_perm, "mixed": bench_mixed, "slos": bench_slos, "gradient": bench_gradient, "copy": bench_copy,
          "gurvits": bench_gurvits, "slos_tree": bench_slos_tree, "sampling_multi": bench_sampling_multi,
-         "sampling": bench_sampling, "cache": bench_cache}[s](args, res)
+         "sampling": bench_sampling, "cache": bench_cache, "cc": bench_cc}[s](args, res)
 
     perm = res.get("perm", {"ms": float("nan"), "perms_per_s": float("nan"), "tflops": float("nan")})
     line = {
