@@ -170,10 +170,33 @@ def make_backends(backends_module, types=None) -> dict:
             assert 0 <= idx < len(coefs)
             return complex(coefs[idx] * math.sqrt(output_state.prodnfact() / self._input_state.prodnfact()))
 
+        def _herald_conditions(self, istate):
+            """Per-mode conditions of a single exact-count mask (heralds), or
+            None when the mask needs the selection path (several masks,
+            at-least modes, fewer than two free modes)."""
+            if self._masks_str is None or len(self._masks_str) != 1 or self._no_limit_modes:
+                return None
+            cond = _slos.mask_conditions(self._masks_str, istate.m)[0]
+            if (cond < 0).sum() < 2 or istate.n < 1 or istate.n > 30:
+                return None
+            if self._mask_n is not None and self._mask_n != istate.n:
+                return None
+            return cond
+
         def all_prob(self, input_state=None):
             if input_state is not None:
                 self.set_input_state(input_state)
             istate = self._input_state
+            cond = self._herald_conditions(istate)
+            if cond is not None:
+                # heralds: every layer restricted to the states that can still
+                # match (the masked FSArray of _slos.py:161-164), kept outputs in
+                # FSArray order
+                key = ("masked", istate, tuple(int(v) for v in cond))
+                if key not in self._results:
+                    plan = _slos.SlosPlan(list(istate), mask_cond=cond)
+                    self._results[key] = plan.run(self._umat, _slos.PROBS).cpu().numpy()
+                return self._results[key].tolist()
             idx = self._ranks(istate)
             if idx is None:
                 return self._host(istate, _slos.PROBS).tolist()

@@ -37,16 +37,26 @@ def _forbid(name):
 
 
 class _FakePlan:
-    """SlosPlan on the oracle: coefficients and probabilities in FSArray order."""
+    """SlosPlan on the oracle: coefficients and probabilities in FSArray order
+    (with mask conditions: the states with exactly c_j photons in every mode
+    whose condition is c_j >= 0, in FSArray order)."""
 
-    def __init__(self, in_occ):
+    def __init__(self, in_occ, mask_cond=None):
         self.in_occ = list(in_occ)
+        self.mask_cond = None if mask_cond is None else np.asarray(mask_cond)
+        _FakePlan.masked_runs += self.mask_cond is not None
+
+    masked_runs = 0
 
     def run(self, U, mode=slos.PROBS, out=None, stream=None):
         U = np.asarray(U)
-        if mode == slos.COEFS:
-            return torch.from_numpy(oracle.slos_coefs(U, self.in_occ))
-        return torch.from_numpy(oracle.slos_probs(U, self.in_occ))
+        r = oracle.slos_coefs(U, self.in_occ) if mode == slos.COEFS else oracle.slos_probs(U, self.in_occ)
+        if self.mask_cond is not None:
+            states = oracle.fock_enumerate(len(self.in_occ), sum(self.in_occ))
+            fixed = self.mask_cond >= 0
+            keep = np.all(states[:, fixed] == self.mask_cond[fixed], axis=1)
+            r = r[keep]
+        return torch.from_numpy(r)
 
 
 class _FakeTree:
@@ -222,3 +232,13 @@ def test_bound_clifford_samples_from_the_library(bound):
     random_seed(3)
     assert b.samples(50) == s1 and len(s1) == 50 and all(s.n == 2 for s in s1)
     assert b.sample().m == 4
+
+
+def test_bound_slos_runs_heralds_as_a_masked_plan(bound):
+    """A single exact-count mask goes through a masked plan (every layer
+    restricted to the states that can still match), not through selection
+    from the full output layer; the reference's masked values come out."""
+    case = next(c for c in _CASES if c["id"] == "strong_sim_with_mask")
+    before = _FakePlan.masked_runs
+    test_bound_backends_reproduce_reference_values(bound, case, "SLOS")
+    assert _FakePlan.masked_runs > before
