@@ -178,9 +178,11 @@ __device__ __forceinline__ cd glynn_wave_item(const double *__restrict__ mats, c
 
     // Row 1 flips every other step: keep it in VGPRs (R1 = 1, 3), read it
     // from LDS like the other rows (R1 = 2, 4: fewer VGPRs and more waves), or
-    // pick by order (R1 = 0).  R1 = 3, 4 fuse each flip into the preceding
-    // term.
-    constexpr bool kRow1InRegs = (R1 == 1) || (R1 == 3) || (R1 == 0 && N <= 22);
+    // pick by order (R1 = 0: in VGPRs up to order 26, the largest that fits
+    // without spilling; at orders 23-24 that is 5% faster than three waves
+    // per SIMD reading it from LDS).  R1 = 3, 4 fuse each flip into the
+    // preceding term.
+    constexpr bool kRow1InRegs = (R1 == 1) || (R1 == 3) || (R1 == 0 && N <= 26);
     constexpr bool kFused = (R1 >= 3);
     cd r1[kRow1InRegs ? N : 1];
     if constexpr (kRow1InRegs) {
