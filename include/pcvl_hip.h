@@ -168,6 +168,18 @@ int pcvl_slos_plan_destroy(pcvl_slos_plan plan);
  * modes and 1..30 photons.  Summation runs in herald-modes-first order, so
  * results match the unmasked plan to rounding, not to the bit. */
 int pcvl_slos_plan_create_masked(int m, const int32_t *h_in_occ, const int32_t *h_cond, pcvl_slos_plan *out);
+/* The general masks of set_mask(masks, n, at_least_modes): nrows condition
+ * rows (h_conds, nrows x m, -1 any count, c >= 0 a count), a state matching
+ * any row; h_at_least (m flags, or NULL) makes a mode's count a floor (at
+ * least c).  Layer k keeps the states whose constrained-mode counts can still
+ * reach some row with the n - k photons to come (tests/utils/test_mask.py:
+ * 33-45, partial match).  The final layer may then hold several herald
+ * configurations; execute writes the matching states in FSArray order (a
+ * final reordering pass, whose buffer pcvl_slos_plan_info's work_bytes
+ * includes).  Needs at least two modes that no row constrains, 1..30
+ * photons, at most 64 rows and 10^6 herald configurations. */
+int pcvl_slos_plan_create_masks(int m, const int32_t *h_in_occ, int nrows, const int32_t *h_conds,
+                                const int32_t *h_at_least, pcvl_slos_plan *out);
 
 /* Several input states in one plan: the _Path tree of
  * SLOSBackend.preprocess([inputs]) (perceval/backends/_slos.py:37-86,170-185),

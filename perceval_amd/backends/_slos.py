@@ -14,6 +14,7 @@ import math
 
 import numpy as np
 
+from .._native import NativeError
 from ..slos import COEFS, PROBS, SlosPlan, SlosTree, fock_mask_select, gather_f64, mask_conditions
 from ..utils.states import BSDistribution, FockState, StateVector, fock_space
 from ._abstract_backends import AStrongSimulationBackend, _as_circuit
@@ -146,23 +147,32 @@ class SLOSBackend(AStrongSimulationBackend):
         assert 0 <= idx < len(coefs)
         return complex(coefs[idx] * math.sqrt(output_state.prodnfact() / self._input_state.prodnfact()))
 
-    def _herald_conditions(self, istate):
-        """Per-mode conditions of a single exact-count mask (the heralds), or
-        None when the mask needs the general path (several masks, at-least
-        modes, fewer than two free modes, no photon)."""
-        if self._mask is None or len(self._masks_str) != 1 or self._no_limit_modes:
+    def _mask_plan_args(self, istate):
+        """(condition rows, at-least modes) of the mask for a masked plan, or
+        None when the mask goes through selection from the full output layer
+        (no mode constrained, fewer than two modes free in every row, more
+        than 64 rows, a photon count outside 1..30 or another than the
+        mask's)."""
+        if self._mask is None or len(self._masks_str) > 64:
             return None
-        cond = mask_conditions(self._masks_str, istate.m)[0]
-        if (cond < 0).sum() < 2 or istate.n < 1 or istate.n > 30 or (cond < 0).all():
+        cond = mask_conditions(self._masks_str, istate.m)
+        free = (cond < 0).all(axis=0)
+        if free.sum() < 2 or free.all() or istate.n < 1 or istate.n > 30:
             return None
         if self._mask_n is not None and self._mask_n != istate.n:
             return None
-        return cond
+        return cond, tuple(int(q) for q in (self._no_limit_modes or ()))
 
-    def _masked_plan(self, istate, cond):
-        key = ("masked_plan", istate, tuple(int(v) for v in cond))
+    def _masked_plan(self, istate, args):
+        cond, at_least = args
+        key = ("masked_plan", istate, cond.tobytes(), at_least)
         if key not in self._plans_masked:
-            self._plans_masked[key] = SlosPlan(list(istate), mask_cond=cond)
+            try:
+                self._plans_masked[key] = SlosPlan(list(istate), mask_cond=cond, at_least=at_least)
+            except NativeError as e:
+                if "herald configurations" not in str(e) and "2^32" not in str(e):
+                    raise
+                self._plans_masked[key] = None   # too many constrained configurations: select instead
         return self._plans_masked[key]
 
     def all_prob(self, input_state=None):
@@ -171,16 +181,16 @@ class SLOSBackend(AStrongSimulationBackend):
             self.set_input_state(input_state)
         if self._mask is None:
             return self._compute(self._input_state, PROBS).tolist()
-        cond = self._herald_conditions(self._input_state)
-        if cond is not None:
-            # heralds: layers restricted to the states that can still match,
-            # the kept outputs in FSArray order
+        args = self._mask_plan_args(self._input_state)
+        plan = None if args is None else self._masked_plan(self._input_state, args)
+        if plan is not None:
+            # every layer restricted to the states that can still match some
+            # mask row, the matching outputs in FSArray order
             key = ("masked_probs", self._input_state)
             if key not in self._results:
-                plan = self._masked_plan(self._input_state, cond)
                 self._results[key] = plan.run(self._umat, PROBS).cpu().numpy()
             return self._results[key].tolist()
-        # other masks: select on the GPU and copy back only the kept probabilities
+        # otherwise: select on the GPU and copy back only the kept probabilities
         idx = self._masked_indices_device(self._input_state)
         return gather_f64(self.all_prob_tensor(), idx).cpu().numpy().tolist()
 

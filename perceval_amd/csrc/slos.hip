@@ -682,8 +682,9 @@ struct SlosPlanImpl {
     int kernel = 0;                   // 0 auto, 1 per state, 4 per state in two phases,
                                       // 6 successor runs with a transposed gather, 8 prefix/suffix blocks
     BlkLayout blk;                    // block layout (kernel 8), built when the layers allow it
-    bool masked = false;              // layers restricted to a herald mask (kernel 8 only)
+    bool masked = false;              // layers restricted to a mask (kernel 8 only)
     BlkMasked mask;                   // their sub-layers
+    uint32_t *d_perm = nullptr;       // final layer in FSArray order (masks with several final sub-layers)
 };
 
 static size_t twophase_lds(int m, int w, int k)
@@ -817,19 +818,30 @@ extern "C" int pcvl_slos_plan_create(int m, const int32_t *h_in_occ, pcvl_slos_p
 extern "C" int pcvl_slos_plan_create_masked(int m, const int32_t *h_in_occ, const int32_t *h_cond,
                                             pcvl_slos_plan *out)
 {
+    return pcvl_slos_plan_create_masks(m, h_in_occ, 1, h_cond, nullptr, out);
+}
+
+extern "C" int pcvl_slos_plan_create_masks(int m, const int32_t *h_in_occ, int nrows, const int32_t *h_conds,
+                                           const int32_t *h_at_least, pcvl_slos_plan *out)
+{
     PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
-    PCVL_ARG_CHECK(h_in_occ != nullptr && h_cond != nullptr, "input state or mask is null");
+    PCVL_ARG_CHECK(h_in_occ != nullptr && h_conds != nullptr, "input state or mask is null");
     PCVL_ARG_CHECK(m >= 2 && m <= SLOS_MAX_MODES, "mode count outside [2, 64]");
+    PCVL_ARG_CHECK(nrows >= 1 && nrows <= 64, "mask row count outside [1, 64]");
     *out = nullptr;
     int n = 0, nfree = 0;
     for (int i = 0; i < m; i++) {
         PCVL_ARG_CHECK(h_in_occ[i] >= 0, "negative occupation");
-        PCVL_ARG_CHECK(h_cond[i] >= -1, "mask condition below -1");
         n += h_in_occ[i];
-        nfree += h_cond[i] < 0 ? 1 : 0;
+        bool free_mode = true;
+        for (int r = 0; r < nrows; r++) {
+            PCVL_ARG_CHECK(h_conds[(size_t)r * m + i] >= -1, "mask condition below -1");
+            free_mode = free_mode && h_conds[(size_t)r * m + i] < 0;
+        }
+        nfree += free_mode ? 1 : 0;
     }
     PCVL_ARG_CHECK(n >= 1 && n <= 30, "masked plans take 1 to 30 photons");
-    PCVL_ARG_CHECK(nfree >= 2, "a masked plan needs at least two modes without a condition");
+    PCVL_ARG_CHECK(nfree >= 2, "a masked plan needs at least two modes that no mask row constrains");
     auto *plan = new (std::nothrow) pcvl_slos_plan_s();
     if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
     SlosPlanImpl &p = plan->impl;
@@ -846,23 +858,49 @@ extern "C" int pcvl_slos_plan_create_masked(int m, const int32_t *h_in_occ, cons
         }
         double pin = 1.0;
         for (int i = 0; i < m; i++) for (int v = 2; v <= h_in_occ[i]; v++) pin *= v;
-        int rc = blk_masked_layers(m, n, h_cond, &p.mask);
+        int rc = device_grid_cap(&p.grid_cap);
+        if (rc == OK) rc = blk_masked_layers_ex(m, n, nrows, h_conds, h_at_least, &p.mask);
         if (rc != OK) { delete plan; return rc; }
-        // the herald modes' factorials are the same for every kept state
-        p.inv_prod_in = p.mask.herald_fact / pin;
+        // the herald modes' factorials go with each final sub-layer (its groups)
+        p.inv_prod_in = 1.0 / pin;
         p.counts = p.mask.counts;
         for (int k = 0; k < n; k++) p.max_inter = std::max<uint64_t>(p.max_inter, p.counts[(size_t)k]);
+        if (!p.mask.perm.empty()) {
+            const size_t b = sizeof(uint32_t) * p.mask.perm.size();
+            hipError_t e = hipMalloc((void **)&p.d_perm, b);
+            if (e == hipSuccess) e = hipMemcpy(p.d_perm, p.mask.perm.data(), b, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                if (p.d_perm) (void)hipFree(p.d_perm);
+                delete plan;
+                set_error(std::string("masked SLOS output order upload: ") + hipGetErrorString(e));
+                return ERR_HIP;
+            }
+            std::vector<uint32_t>().swap(p.mask.perm);   // the host copy is no longer needed
+        }
         if (!p.mask.empty) {
             rc = masked_layout(p, 0);
-            if (rc != OK) { delete plan; return rc; }
+            if (rc != OK) { if (p.d_perm) (void)hipFree(p.d_perm); delete plan; return rc; }
         }
     } catch (const std::bad_alloc &) {
+        if (p.d_perm) (void)hipFree(p.d_perm);
         delete plan;
         set_error("host allocation failed while building the masked plan");
         return ERR_ALLOC;
     }
     *out = plan;
     return OK;
+}
+
+// Complex entries of the two layer buffers at the start of the workspace.
+static uint64_t slos_work_main(const SlosPlanImpl &p) { return p.n >= 2 ? 2 * (p.max_inter + 1) : 2; }
+
+// Masked final layer in FSArray order: dst[i] = src[perm[i]].
+template <typename T>
+__global__ void slos_perm_gather_kernel(const T *__restrict__ src, const uint32_t *__restrict__ perm, uint64_t count,
+                                        T *__restrict__ dst)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = src[perm[i]];
 }
 
 extern "C" int pcvl_slos_plan_info(pcvl_slos_plan plan, int *n, int64_t *out_count, size_t *work_bytes,
@@ -872,9 +910,10 @@ extern "C" int pcvl_slos_plan_info(pcvl_slos_plan plan, int *n, int64_t *out_cou
     const SlosPlanImpl &p = plan->impl;
     if (n) *n = p.n;
     if (out_count) *out_count = (int64_t)p.counts[p.n];
-    // two ping-pong buffers of the largest intermediate layer (complex)
-    // two layer buffers, each with one zero coefficient after its states
-    if (work_bytes) *work_bytes = p.n >= 2 ? 2 * (p.max_inter + 1) * sizeof(cd) : sizeof(cd) * 2;
+    // two layer buffers of the largest intermediate layer (complex), each
+    // with one zero coefficient after its states; masks with several final
+    // sub-layers add the final layer in sub-layer order (reordered at the end)
+    if (work_bytes) *work_bytes = sizeof(cd) * (slos_work_main(p) + (p.d_perm ? p.counts[(size_t)p.n] : 0));
     if (h_path) for (int k = 0; k < p.n; k++) h_path[k] = p.mk[k];
     return OK;
 }
@@ -913,10 +952,13 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
     hipLaunchKernelGGL(slos_vacuum_kernel, dim3(1), dim3(64), 0, stream, reinterpret_cast<double *>(bufA), 2);
     PCVL_HIP_CHECK(hipGetLastError());
     const cd *prev = bufA;
+    // masks with several final sub-layers: the last layer goes to the end of
+    // the workspace in sub-layer order, then to d_out in FSArray order
+    void *last_dst = p.d_perm ? static_cast<void *>(bufA + slos_work_main(p)) : static_cast<void *>(d_out);
     for (int k = 1; k <= p.n; k++) {
         const uint64_t nc = p.counts[k];
         const bool last = (k == p.n);
-        void *dst = last ? static_cast<void *>(d_out) : static_cast<void *>(prev == bufA ? bufB : bufA);
+        void *dst = last ? last_dst : static_cast<void *>(prev == bufA ? bufB : bufA);
         const bool probs = last && mode == PCVL_SLOS_PROBS;
         if (kernel == 8) {
             const int rc = blk_layer_launch(p.blk, prev, dst, U, p.m, p.mk[k - 1], k, probs, p.inv_prod_in, !last,
@@ -949,6 +991,17 @@ extern "C" int pcvl_slos_plan_execute(pcvl_slos_plan plan, const double *d_U, in
         }
         PCVL_HIP_CHECK(hipGetLastError());
         prev = reinterpret_cast<const cd *>(dst);
+    }
+    if (p.d_perm) {
+        const uint64_t cnt = p.counts[(size_t)p.n];
+        const int grid = (int)std::min<uint64_t>((cnt + 255) / 256, (uint64_t)p.grid_cap);
+        if (mode == PCVL_SLOS_PROBS)
+            hipLaunchKernelGGL(slos_perm_gather_kernel<double>, dim3(grid), dim3(256), 0, stream,
+                               reinterpret_cast<const double *>(last_dst), p.d_perm, cnt, d_out);
+        else
+            hipLaunchKernelGGL(slos_perm_gather_kernel<cd>, dim3(grid), dim3(256), 0, stream,
+                               reinterpret_cast<const cd *>(last_dst), p.d_perm, cnt, reinterpret_cast<cd *>(d_out));
+        PCVL_HIP_CHECK(hipGetLastError());
     }
     return OK;
 }
@@ -1007,6 +1060,10 @@ extern "C" int pcvl_slos_plan_destroy(pcvl_slos_plan plan)
     if (!plan) return OK;
     hipError_t e = hipSuccess;
     if (plan->impl.d_T) e = hipFree(plan->impl.d_T);
+    if (plan->impl.d_perm) {
+        const hipError_t e2 = hipFree(plan->impl.d_perm);
+        if (e == hipSuccess) e = e2;
+    }
     blk_layout_destroy(&plan->impl.blk);
     delete plan;
     if (e != hipSuccess) { set_error(std::string("hipFree(slos plan): ") + hipGetErrorString(e)); return ERR_HIP; }

@@ -50,8 +50,10 @@ struct BlkGroup {
     uint32_t her_off;   // herald parents: entries [her_off, her_off + nher) of the herald table
     uint32_t nher;
     uint32_t rec0;      // record of the group's block 0 (groups with nch != 0)
-    uint32_t pad0, pad1, pad2;
+    uint32_t hf_lo, hf_hi;   // the sub-layer's prod a_j! (a double), factor of its probabilities
+    uint32_t pad0;
 };
+static_assert(sizeof(BlkGroup) == 64, "BlkGroup is one 64-byte row");
 
 // Precomputed walk of one block (groups with whole waves per block), 128
 // bytes so that one scalar fetch brings all of it:
@@ -72,6 +74,7 @@ struct BlkSub {
     // herald parents, in herald mode order: (herald index i, offset of
     // sub-layer a - e_{H_i} in layer k - 1)
     std::vector<std::pair<uint32_t, uint32_t>> her;
+    double hfact = 1.0;      // prod a_j! (probabilities of the final layer)
 };
 
 struct BlkLayout {
@@ -112,18 +115,31 @@ int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<ch
 int blk_layout_build_host_ex(int m_free, int nmax_free, int w, const std::vector<std::vector<BlkSub>> &subs,
                              const std::vector<uint64_t> &counts, const std::vector<int32_t> &rowmap,
                              BlkLayout *out, std::vector<char> *image);
-// The masked layers of an n-photon plan over m modes whose mask fixes
-// cond[j] >= 0 photons in mode j (cond[j] < 0: any count).
+// The masked layers of an n-photon plan over m modes for a mask of `nrows`
+// condition rows (FSMask, perceval/backends/_abstract_backends.py:103-146):
+// conds[r * m + j] >= 0 asks for that many photons in mode j (at least that
+// many when at_least[j] != 0), < 0 for any count; a state is kept when it
+// matches some row.  Herald modes are the modes some row constrains; layer k
+// keeps the states whose herald part a can still reach some row (partial
+// match, tests/utils/test_mask.py:33-38): for that row, a_j <= c_j on its
+// exact modes and its missing photons (c_j - a_j, at-least modes counting
+// max(0, c_j - a_j)) fit in the n - k photons still to come.  Every parent
+// of a kept state is kept (same row).  The final layer is the concatenation
+// of its sub-layers (herald configurations in descending lexicographic
+// order); when it has more than one, `perm` lists, in FSArray order of the
+// kept states (descending lexicographic over the original modes), each
+// one's position in the final layer.
 struct BlkMasked {
     int m_free = 0, nher = 0;
-    int nmax_free = 0;                          // photons in the free modes of the final layer
-    bool empty = false;                         // the heralds need more photons than the input has
+    int nmax_free = 0;                          // most photons in the free modes of any sub-layer
+    bool empty = false;                         // no state of the final layer matches
     std::vector<int32_t> rowmap;                // free modes, then herald modes (original indices)
     std::vector<std::vector<BlkSub>> subs;      // per layer k = 1..n
     std::vector<uint64_t> counts;               // states per layer k = 0..n
-    double herald_fact = 1.0;                   // prod over herald modes of c_j!
+    std::vector<uint32_t> perm;                 // final layer in FSArray order (empty: already in order)
 };
 int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out);
+int blk_masked_layers_ex(int m, int n, int nrows, const int32_t *conds, const int32_t *at_least, BlkMasked *out);
 
 // Uploads a host-built layout (allocates d_mem).
 int blk_layout_upload(BlkLayout *lay, const std::vector<char> &image);

@@ -67,6 +67,12 @@ static uint64_t nstates(int modes, int x)
 // Herald parents of the child at position loc of its sub-layer: entry (i, base)
 // gives parent[base + loc] times U[H_i, mk] (su[mfree + i]).  They come first
 // in the summation order (herald modes lead the local mode order).
+// The group's herald factorial prod a_j! (1 outside masked final layers).
+__device__ __forceinline__ double blk_hfact(const BlkGroup &g)
+{
+    return __builtin_bit_cast(double, ((uint64_t)g.hf_hi << 32) | g.hf_lo);
+}
+
 // Children are written once and read again only by the next layer's launch
 // (after the whole layer): nontemporal stores keep them from displacing the
 // parent lines that the running layer still re-reads from L2.
@@ -209,7 +215,7 @@ __device__ __forceinline__ void blk_item_a(
     } else {
         double pfa;
         __builtin_memcpy(&pfa, &rw[8], sizeof(double));
-        blk_store(reinterpret_cast<double *>(out) + o, prob_from_coef(acc, pfa * pfs, inv_prod_in));
+        blk_store(reinterpret_cast<double *>(out) + o, prob_from_coef(acc, pfa * pfs * blk_hfact(g), inv_prod_in));
     }
 }
 
@@ -287,7 +293,7 @@ __device__ __forceinline__ void blk_item_b(
     if (MODE == 0) {
         blk_store(reinterpret_cast<cd *>(out) + o, acc);
     } else {
-        const double pf = (double)pfp * spf[g.tpf + beta];
+        const double pf = (double)pfp * spf[g.tpf + beta] * blk_hfact(g);
         blk_store(reinterpret_cast<double *>(out) + o, prob_from_coef(acc, pf, inv_prod_in));
     }
 }
@@ -530,6 +536,7 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
     // records hold at most 16 occupied prefix modes and mode indices below 256
     const bool use_records = lay.cmax <= BLK_REC_MAX && h <= 256;
     std::vector<uint32_t> hert;   // (herald index, parent sub-layer offset) pairs
+    std::map<std::pair<int, int>, uint32_t> rec_of;   // (K, r) -> first record
     lay.groups.assign((size_t)lay.nlayers + 1, {});
     lay.items.assign((size_t)lay.nlayers + 1, 0);
     for (int k = 1; k <= lay.nlayers; k++) {
@@ -547,13 +554,20 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
             }
             for (int r = 0; r <= sub.kf; r++) {
                 BlkGroup g{};
+                std::memcpy(&g.hf_lo, &sub.hfact, sizeof(double));   // hf_lo, hf_hi
                 g.nb = (uint32_t)nstates(h, sub.kf - r);
                 g.S = (uint32_t)nstates(w, r);
                 if (g.S >= BLK_A_MIN && use_records) {
                     g.nch = (g.S + 63) / 64;
                     g.nitems = g.nb * g.nch;
-                    g.rec0 = (uint32_t)recs.size();
-                    add_records(sub.kf - r, r);
+                    // a block's record depends on (K, r) only: sub-layers share them
+                    const auto key = std::make_pair(sub.kf - r, r);
+                    auto rit = rec_of.find(key);
+                    if (rit == rec_of.end()) {
+                        rit = rec_of.emplace(key, (uint32_t)recs.size()).first;
+                        add_records(sub.kf - r, r);
+                    }
+                    g.rec0 = rit->second;
                     if (recs.size() > (256ull << 20) / sizeof(BlkRec)) {
                         set_error("block SLOS layout: block records above 256 MiB (use a wider suffix)");
                         return ERR_RANGE;
@@ -643,62 +657,87 @@ static size_t blk_lds_bytes(const BlkLayout &lay, int k)
 
 int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out)
 {
+    return blk_masked_layers_ex(m, n, 1, cond, nullptr, out);
+}
+
+int blk_masked_layers_ex(int m, int n, int nrows, const int32_t *conds, const int32_t *at_least, BlkMasked *out)
+{
+    if (m < 1 || n < 0 || nrows < 1 || !conds) { set_error("masked SLOS: bad mask arguments"); return ERR_ARG; }
     BlkMasked mk;
-    std::vector<int32_t> H, c;
+    std::vector<int32_t> H;
     for (int j = 0; j < m; j++) {
-        if (cond[j] >= 0) {
-            H.push_back(j);
-            c.push_back(cond[j]);
-        } else {
-            mk.rowmap.push_back(j);
-        }
+        bool constrained = false;
+        for (int r = 0; r < nrows; r++) constrained = constrained || conds[(size_t)r * m + j] >= 0;
+        if (constrained) H.push_back(j);
+        else mk.rowmap.push_back(j);
     }
     mk.m_free = (int)mk.rowmap.size();
     mk.nher = (int)H.size();
     for (int32_t j : H) mk.rowmap.push_back(j);
-    int C = 0;
-    double hf = 1.0;
-    uint64_t configs = 1;
-    for (int32_t v : c) {
-        C += v;
-        for (int t = 2; t <= v; t++) hf *= t;
-        configs *= (uint64_t)std::min(v, n) + 1;
-        if (configs > 1000000) { set_error("masked SLOS: more than 10^6 herald configurations"); return ERR_RANGE; }
-    }
-    mk.herald_fact = hf;
-    mk.subs.assign((size_t)n + 1, {});
-    mk.counts.assign((size_t)n + 1, 0);
-    if (C > n) {
-        mk.empty = true;
-        *out = std::move(mk);
-        return OK;
-    }
-    mk.nmax_free = n - C;
-    // every herald configuration a <= c (componentwise, capped at n), in
-    // descending lexicographic order
+    const int nh = (int)H.size();
+    // per row, over the herald modes: the count asked (-1 any) and whether it is a floor
+    std::vector<int32_t> rc((size_t)nrows * nh), rl((size_t)nrows * nh);
+    for (int r = 0; r < nrows; r++)
+        for (int i = 0; i < nh; i++) {
+            rc[(size_t)r * nh + i] = conds[(size_t)r * m + H[(size_t)i]];
+            rl[(size_t)r * nh + i] = at_least && at_least[H[(size_t)i]] ? 1 : 0;
+        }
+    // largest useful value of each herald count
+    std::vector<int32_t> cap((size_t)nh, 0);
+    for (int i = 0; i < nh; i++)
+        for (int r = 0; r < nrows; r++) {
+            const int32_t c = rc[(size_t)r * nh + i];
+            cap[(size_t)i] = std::max(cap[(size_t)i], (c < 0 || rl[(size_t)r * nh + i]) ? n : std::min(c, n));
+        }
+    // the missing photons of configuration a for row r (-1: a has more than row r allows)
+    auto missing = [&](const std::vector<int32_t> &a, int r) {
+        int d = 0;
+        for (int i = 0; i < nh; i++) {
+            const int32_t c = rc[(size_t)r * nh + i];
+            if (c < 0) continue;
+            if (rl[(size_t)r * nh + i]) d += std::max(0, c - a[(size_t)i]);
+            else if (a[(size_t)i] > c) return -1;
+            else d += c - a[(size_t)i];
+        }
+        return d;
+    };
+    // layer k keeps configuration a when some row still fits in the n - k photons to come
+    auto kept = [&](const std::vector<int32_t> &a, int k) {
+        for (int r = 0; r < nrows; r++) {
+            const int d = missing(a, r);
+            if (d >= 0 && d <= n - k) return true;
+        }
+        return false;
+    };
+    // every configuration a <= cap with |a| <= n, in descending lexicographic order
     std::vector<std::vector<int32_t>> all;
     {
-        std::vector<int32_t> a(H.size(), 0);
-        for (size_t i = 0; i < H.size(); i++) a[i] = std::min(c[i], n);
+        std::vector<int32_t> a(cap);
         while (true) {
-            all.push_back(a);
-            // next in descending lexicographic order of a box: decrement the
-            // last coordinate that is above 0, reset the ones after it
-            int i = (int)a.size() - 1;
+            int sa = 0;
+            for (int32_t v : a) sa += v;
+            if (sa <= n) {
+                all.push_back(a);
+                if (all.size() > 1000000) { set_error("masked SLOS: more than 10^6 herald configurations"); return ERR_RANGE; }
+            }
+            int i = nh - 1;
             while (i >= 0 && a[(size_t)i] == 0) i--;
             if (i < 0) break;
             a[(size_t)i]--;
-            for (size_t t = (size_t)i + 1; t < a.size(); t++) a[t] = std::min(c[t], n);
+            for (int t = i + 1; t < nh; t++) a[(size_t)t] = cap[(size_t)t];
         }
     }
     auto size_of = [](const std::vector<int32_t> &a) { int s = 0; for (int32_t v : a) s += v; return s; };
+    mk.subs.assign((size_t)n + 1, {});
+    mk.counts.assign((size_t)n + 1, 0);
     std::vector<std::map<std::vector<int32_t>, uint32_t>> base((size_t)n + 1);
     for (int k = 0; k <= n; k++) {
         uint64_t off = 0;
         for (const auto &a : all) {
             const int sa = size_of(a);
-            if (sa > k || k - sa > n - C) continue;
+            if (sa > k || !kept(a, k)) continue;
             base[(size_t)k][a] = (uint32_t)off;
+            mk.nmax_free = std::max(mk.nmax_free, k - sa);
             off += nstates(mk.m_free, k - sa);
             if (off >= (1ull << 32)) { set_error("masked SLOS: a layer holds 2^32 states or more"); return ERR_RANGE; }
         }
@@ -710,15 +749,91 @@ int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out)
             BlkSub sub;
             sub.out_base = it->second;
             sub.kf = k - size_of(a);
-            if (sub.kf >= 1) sub.par_base = base[(size_t)k - 1].at(a);
-            for (size_t i = 0; i < a.size(); i++) {
-                if (a[i] == 0) continue;
-                auto b = a;
-                b[i]--;
-                sub.her.push_back({(uint32_t)i, base[(size_t)k - 1].at(b)});
+            if (sub.kf >= 1) {
+                auto pit = base[(size_t)k - 1].find(a);
+                if (pit == base[(size_t)k - 1].end()) { set_error("masked SLOS: internal: parent sub-layer missing"); return ERR_RANGE; }
+                sub.par_base = pit->second;
             }
+            for (int i = 0; i < nh; i++) {
+                if (a[(size_t)i] == 0) continue;
+                auto b = a;
+                b[(size_t)i]--;
+                auto pit = base[(size_t)k - 1].find(b);
+                if (pit == base[(size_t)k - 1].end()) { set_error("masked SLOS: internal: herald parent missing"); return ERR_RANGE; }
+                sub.her.push_back({(uint32_t)i, pit->second});
+            }
+            if (k == n)
+                for (int32_t v : a)
+                    for (int t = 2; t <= v; t++) sub.hfact *= t;
             mk.subs[(size_t)k].push_back(std::move(sub));
         }
+    }
+    if (mk.counts[(size_t)n] == 0) {
+        mk.empty = true;
+        *out = std::move(mk);
+        return OK;
+    }
+    // several final sub-layers: the FSArray order of the kept states, by a
+    // walk over the original modes (each mode's count from the photons left
+    // down to 0) that keeps only the final configurations still reachable:
+    // the same herald values so far, their remaining herald photons within
+    // the photons left (exactly, once no free mode is left)
+    if (base[(size_t)n].size() > 1) {
+        if (mk.counts[(size_t)n] > (1ull << 28)) { set_error("masked SLOS: more than 2^28 kept outputs"); return ERR_RANGE; }
+        std::vector<int32_t> hidx((size_t)m, -1), fidx((size_t)m, -1);
+        for (int i = 0; i < nh; i++) hidx[(size_t)H[(size_t)i]] = i;
+        for (int e = 0; e < mk.m_free; e++) fidx[(size_t)mk.rowmap[(size_t)e]] = e;
+        std::vector<std::pair<std::vector<int32_t>, uint32_t>> finals(base[(size_t)n].begin(), base[(size_t)n].end());
+        const size_t nf = finals.size();
+        // suf[t * (m + 1) + q]: herald photons of final configuration t on modes q..m-1
+        std::vector<int> suf(nf * ((size_t)m + 1), 0), free_after((size_t)m + 1, 0);
+        for (size_t t = 0; t < nf; t++)
+            for (int q = m - 1; q >= 0; q--)
+                suf[t * ((size_t)m + 1) + q] = suf[t * ((size_t)m + 1) + q + 1] +
+                                               (hidx[(size_t)q] >= 0 ? finals[t].first[(size_t)hidx[(size_t)q]] : 0);
+        for (int q = m - 1; q >= 0; q--) free_after[(size_t)q] = free_after[(size_t)q + 1] + (fidx[(size_t)q] >= 0);
+        std::vector<int32_t> f((size_t)mk.m_free, 0);
+        std::vector<std::vector<int>> cands((size_t)m + 1);
+        cands[0].resize(nf);
+        for (size_t t = 0; t < nf; t++) cands[0][t] = (int)t;
+        mk.perm.reserve((size_t)mk.counts[(size_t)n]);
+        std::vector<int> val((size_t)m, 0), left((size_t)m + 1, 0);
+        left[0] = n;
+        int q = 0;
+        bool start = true;
+        while (q >= 0) {
+            if (start) {
+                val[(size_t)q] = left[(size_t)q];
+                start = false;
+            } else {
+                if (q == m - 1 || val[(size_t)q] == 0) { q--; continue; }
+                val[(size_t)q]--;
+            }
+            const int v = val[(size_t)q];
+            if (q == m - 1 && v != left[(size_t)q]) { q--; continue; }
+            const int rest = left[(size_t)q] - v;
+            auto &next = cands[(size_t)q + 1];
+            next.clear();
+            for (int t : cands[(size_t)q]) {
+                if (hidx[(size_t)q] >= 0 && finals[(size_t)t].first[(size_t)hidx[(size_t)q]] != v) continue;
+                const int need = suf[(size_t)t * ((size_t)m + 1) + q + 1];
+                if (need > rest || (free_after[(size_t)q + 1] == 0 && need != rest)) continue;
+                next.push_back(t);
+            }
+            if (next.empty()) continue;   // the next smaller value
+            if (fidx[(size_t)q] >= 0) f[(size_t)fidx[(size_t)q]] = v;
+            if (q == m - 1) {
+                // complete: the herald part is exactly one final configuration
+                const auto &fc = finals[(size_t)next[0]];
+                std::vector<int> fv(f.begin(), f.end());
+                mk.perm.push_back(fc.second + (uint32_t)local_rank(fv));
+                continue;   // the last mode has a single value: this pops
+            }
+            left[(size_t)q + 1] = rest;
+            q++;
+            start = true;
+        }
+        if (mk.perm.size() != mk.counts[(size_t)n]) { set_error("masked SLOS: internal: output order walk"); return ERR_RANGE; }
     }
     *out = std::move(mk);
     return OK;

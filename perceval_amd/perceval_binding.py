@@ -34,6 +34,7 @@ import types as _types
 
 import numpy as np
 
+from . import _native
 from . import permanent as _permanent
 from . import sampling as _sampling
 from . import slos as _slos
@@ -170,33 +171,43 @@ def make_backends(backends_module, types=None) -> dict:
             assert 0 <= idx < len(coefs)
             return complex(coefs[idx] * math.sqrt(output_state.prodnfact() / self._input_state.prodnfact()))
 
-        def _herald_conditions(self, istate):
-            """Per-mode conditions of a single exact-count mask (heralds), or
-            None when the mask needs the selection path (several masks,
-            at-least modes, fewer than two free modes)."""
-            if self._masks_str is None or len(self._masks_str) != 1 or self._no_limit_modes:
+        def _mask_plan_args(self, istate):
+            """(condition rows, at-least modes) for a masked plan, or None when
+            the mask goes through selection from the full output layer (no
+            mode constrained, fewer than two modes free in every row, more
+            than 64 rows, a photon count outside 1..30 or another than the
+            mask's)."""
+            if self._masks_str is None or len(self._masks_str) > 64:
                 return None
-            cond = _slos.mask_conditions(self._masks_str, istate.m)[0]
-            if (cond < 0).sum() < 2 or istate.n < 1 or istate.n > 30:
+            cond = _slos.mask_conditions(self._masks_str, istate.m)
+            free = (cond < 0).all(axis=0)
+            if free.sum() < 2 or free.all() or istate.n < 1 or istate.n > 30:
                 return None
             if self._mask_n is not None and self._mask_n != istate.n:
                 return None
-            return cond
+            return cond, tuple(int(q) for q in (self._no_limit_modes or ()))
 
         def all_prob(self, input_state=None):
             if input_state is not None:
                 self.set_input_state(input_state)
             istate = self._input_state
-            cond = self._herald_conditions(istate)
-            if cond is not None:
-                # heralds: every layer restricted to the states that can still
-                # match (the masked FSArray of _slos.py:161-164), kept outputs in
-                # FSArray order
-                key = ("masked", istate, tuple(int(v) for v in cond))
+            args = self._mask_plan_args(istate)
+            if args is not None:
+                # every layer restricted to the states that can still match
+                # some mask row (the masked FSArray of _slos.py:161-164), the
+                # matching outputs in FSArray order
+                cond, at_least = args
+                key = ("masked", istate, cond.tobytes(), at_least)
                 if key not in self._results:
-                    plan = _slos.SlosPlan(list(istate), mask_cond=cond)
-                    self._results[key] = plan.run(self._umat, _slos.PROBS).cpu().numpy()
-                return self._results[key].tolist()
+                    try:
+                        plan = _slos.SlosPlan(list(istate), mask_cond=cond, at_least=at_least)
+                        self._results[key] = plan.run(self._umat, _slos.PROBS).cpu().numpy()
+                    except _native.NativeError as e:
+                        if "herald configurations" not in str(e) and "2^32" not in str(e):
+                            raise
+                        self._results[key] = None   # too many constrained configurations: select instead
+                if self._results[key] is not None:
+                    return self._results[key].tolist()
             idx = self._ranks(istate)
             if idx is None:
                 return self._host(istate, _slos.PROBS).tolist()

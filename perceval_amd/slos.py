@@ -20,29 +20,44 @@ PROBS = 1
 
 
 class SlosPlan:
-    """mask_cond: optional per-mode conditions (-1 any count, c >= 0 exactly
-    c photons; see mask_conditions).  A masked plan computes only the layers'
-    states that can still match (the masked FSArray of _slos.py:161-164) and
-    returns the matching output states only, in FSArray order."""
+    """mask_cond: optional conditions, one row per mask (-1 any count, c >= 0
+    c photons; a 1-D row is one mask; see mask_conditions); at_least: the
+    modes whose digit is a floor (at least c photons) instead of a count.  A
+    masked plan computes only the layers' states that can still match some
+    row (the masked FSArray of _slos.py:161-164) and returns the matching
+    output states only, in FSArray order."""
 
-    def __init__(self, in_occ, mask_cond=None):
+    def __init__(self, in_occ, mask_cond=None, at_least=None):
         lib = _native.load()
         self.in_occ = np.ascontiguousarray(np.asarray(list(in_occ), dtype=np.int32))
         self.m = int(self.in_occ.size)
         self._plan = ctypes.c_void_p()
         self.mask_cond = None
+        self.at_least = None
         i32p = ctypes.POINTER(ctypes.c_int32)
         if mask_cond is None:
+            if at_least:
+                raise ValueError("at-least modes need mask conditions")
             _native.check(lib.pcvl_slos_plan_create(self.m, self.in_occ.ctypes.data_as(i32p), ctypes.byref(self._plan)),
                           "pcvl_slos_plan_create")
         else:
-            cond = np.ascontiguousarray(np.asarray(list(mask_cond), dtype=np.int32))
-            if cond.size != self.m:
+            cond = np.asarray(mask_cond, dtype=np.int32)
+            if cond.ndim == 1:
+                cond = cond[None, :]
+            cond = np.ascontiguousarray(cond)
+            if cond.ndim != 2 or cond.shape[1] != self.m or cond.shape[0] < 1:
                 raise ValueError("mask conditions must have one entry per mode")
+            at = np.zeros(self.m, dtype=np.int32)
+            for q in (at_least or []):
+                if not 0 <= int(q) < self.m:
+                    raise ValueError("at-least mode outside the circuit")
+                at[int(q)] = 1
             self.mask_cond = cond
-            _native.check(lib.pcvl_slos_plan_create_masked(self.m, self.in_occ.ctypes.data_as(i32p),
-                                                           cond.ctypes.data_as(i32p), ctypes.byref(self._plan)),
-                          "pcvl_slos_plan_create_masked")
+            self.at_least = at
+            _native.check(lib.pcvl_slos_plan_create_masks(self.m, self.in_occ.ctypes.data_as(i32p), cond.shape[0],
+                                                          cond.ctypes.data_as(i32p), at.ctypes.data_as(i32p),
+                                                          ctypes.byref(self._plan)),
+                          "pcvl_slos_plan_create_masks")
         self._lib = lib
         n, count, work = ctypes.c_int(), ctypes.c_int64(), ctypes.c_size_t()
         path = np.zeros(max(1, int(self.in_occ.sum())), dtype=np.int32)
@@ -207,16 +222,27 @@ def fock_unrank(ranks, m: int, n: int) -> np.ndarray:
 
 
 def mask_conditions(masks, m: int) -> np.ndarray:
-    """FSMask strings -> (nmask, m) int32 conditions (-1 = any count)."""
+    """FSMask strings -> (nmask, m) int32 conditions (-1 = any count).  A
+    count c is the character chr(0x30 + c), so '0'..'9' and on up to 32
+    photons (perceval/simulators/simulator.py:650-662 writes heralds so)."""
     if isinstance(masks, str):
         masks = [masks]
     rows = []
     for mk in masks:
-        mk = mk.replace("*", " ")
         if len(mk) != m:
             raise ValueError("mask length must equal the number of modes")
-        rows.append([-1 if ch == " " else int(ch) for ch in mk])
+        rows.append([mask_char_count(ch) for ch in mk])
     return np.ascontiguousarray(np.asarray(rows, dtype=np.int32))
+
+
+def mask_char_count(ch: str) -> int:
+    """One mask character: ' ' or '*' any count (-1), else chr(0x30 + c) for c in 0..32."""
+    if ch in " *":
+        return -1
+    c = ord(ch) - 0x30
+    if not 0 <= c <= 32:
+        raise ValueError(f"mask character {ch!r} is neither a count nor a wildcard")
+    return c
 
 
 def fock_mask_select(m: int, n: int, masks, at_least_modes=None):

@@ -2,13 +2,21 @@
 tests/utils/test_mask.py:33-45 and AStrongSimulationBackend.set_mask,
 perceval/backends/_abstract_backends.py:103-146).
 
-A mask is a string with one character per mode: a digit fixes the photon count
-of that mode, a space or '*' accepts any count.  Several masks are OR-ed.  On
+A mask is a string with one character per mode: chr(0x30 + c) (a digit for
+c <= 9, up to 32 photons: perceval/simulators/simulator.py:650-662) fixes the
+photon count of that mode, a space or '*' accepts any count.  Several masks are OR-ed.  On
 `at_least_modes` a digit is a lower bound instead of an exact count.  With
 allow_partial=True a state may hold fewer photons than a digit requires (it
 can still be completed by photons added later).
 """
 from __future__ import annotations
+
+
+def _count(ch: str) -> int:
+    c = ord(ch) - 0x30
+    if not 0 <= c <= 32:
+        raise ValueError(f"mask character {ch!r} is neither a count nor a wildcard")
+    return c
 
 
 class FSMask:
@@ -20,10 +28,9 @@ class FSMask:
         self.at_least = set(at_least_modes or [])
         self.conditions = []
         for mk in masks:
-            mk = mk.replace("*", " ")
             if len(mk) != m:
                 raise ValueError("mask length must equal the number of modes")
-            self.conditions.append([None if ch == " " else int(ch) for ch in mk])
+            self.conditions.append([None if ch in " *" else _count(ch) for ch in mk])
 
     def _match_one(self, cond, state, allow_partial: bool) -> bool:
         for i, c in enumerate(cond):

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <set>
 
 using namespace pcvl;
 
@@ -220,6 +221,14 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
                     CHECK(got == exp, "%s: k=%d r=%d b=%u beta=%u: parent list differs", what, k, r, b, beta);
                     for (auto &e : got) CHECK(e.first < nparent, "%s: parent %lu >= %lu", what, (unsigned long)e.first, (unsigned long)nparent);
                     CHECK(pf == pfe, "%s: pf %g vs %g", what, pf, pfe);
+                    // the herald modes' factorials ride on the final layer's groups
+                    double hfe = 1, hfg;
+                    if (k == lay.nlayers)
+                        for (int e = mfree; e < lay.mloc; e++)
+                            for (int v = 2; v <= c[(size_t)perm[(size_t)e]]; v++) hfe *= v;
+                    const uint32_t hw[2] = {g.hf_lo, g.hf_hi};
+                    std::memcpy(&hfg, hw, sizeof(double));
+                    CHECK(hfg == hfe, "%s: k=%d herald factorials %g vs %g", what, k, hfg, hfe);
                 }
               }
           }
@@ -295,6 +304,89 @@ static void emulate_masked(int m, int n, const std::vector<int32_t> &cond, int w
     replay(lay, img, layers, perm, what, 2048);
 }
 
+// General masks: several condition rows (a state matches when it matches
+// one), at-least modes.  The expected final layer is the brute-force set of
+// matching n-photon states; layer k is its closure under photon removal (the
+// states some matching output extends), ordered by (constrained modes, free
+// modes) descending; the output order is the FSArray order of the final set.
+static void emulate_masks(int m, int n, const std::vector<std::vector<int32_t>> &rows,
+                          const std::vector<int32_t> &at_least, int w)
+{
+    const int nrows = (int)rows.size();
+    std::vector<int32_t> flat;
+    for (const auto &r : rows) flat.insert(flat.end(), r.begin(), r.end());
+    BlkMasked mk;
+    if (blk_masked_layers_ex(m, n, nrows, flat.data(), at_least.empty() ? nullptr : at_least.data(), &mk) != OK) {
+        printf("masks failed %s\n", pcvl_last_error());
+        exit(1);
+    }
+    char what[96];
+    snprintf(what, sizeof what, "masks m=%d n=%d rows=%d w=%d", m, n, nrows, w);
+    std::vector<std::vector<int>> fin;   // FSArray order
+    for (const auto &s : space(m, n)) {
+        bool any = false;
+        for (const auto &r : rows) {
+            bool ok = true;
+            for (int j = 0; j < m && ok; j++) {
+                if (r[(size_t)j] < 0) continue;
+                const bool floor = !at_least.empty() && at_least[(size_t)j];
+                ok = floor ? s[(size_t)j] >= r[(size_t)j] : s[(size_t)j] == r[(size_t)j];
+            }
+            any = any || ok;
+        }
+        if (any) fin.push_back(s);
+    }
+    CHECK(mk.empty == fin.empty(), "%s: empty flag", what);
+    if (mk.empty) return;
+    std::vector<std::set<std::vector<int>>> closure((size_t)n + 1);
+    closure[(size_t)n].insert(fin.begin(), fin.end());
+    for (int k = n; k >= 1; k--)
+        for (const auto &t : closure[(size_t)k])
+            for (int j = 0; j < m; j++)
+                if (t[(size_t)j] > 0) {
+                    auto u = t;
+                    u[(size_t)j]--;
+                    closure[(size_t)k - 1].insert(u);
+                }
+    const std::vector<int32_t> perm = mk.rowmap;
+    std::vector<std::vector<std::vector<int>>> layers;
+    for (int k = 0; k <= n; k++) {
+        std::vector<std::pair<std::vector<int>, std::vector<int>>> kept;
+        for (const auto &s : closure[(size_t)k]) {
+            std::vector<int> key;
+            for (int e = mk.m_free; e < (int)perm.size(); e++) key.push_back(s[(size_t)perm[(size_t)e]]);
+            for (int e = 0; e < mk.m_free; e++) key.push_back(s[(size_t)perm[(size_t)e]]);
+            kept.push_back({key, s});
+        }
+        std::sort(kept.begin(), kept.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::vector<std::vector<int>> L;
+        for (auto &e : kept) L.push_back(e.second);
+        CHECK(mk.counts[(size_t)k] == L.size(), "%s: count of layer %d: %lu vs %zu", what, k,
+              (unsigned long)mk.counts[(size_t)k], L.size());
+        layers.push_back(L);
+    }
+    if (fails) return;
+    // output order: perm[i] is the layer position of the i-th state in FSArray order
+    std::map<std::vector<int>, uint32_t> pos;
+    for (size_t i = 0; i < layers[(size_t)n].size(); i++) pos[layers[(size_t)n][i]] = (uint32_t)i;
+    if (mk.perm.empty()) {
+        CHECK(layers[(size_t)n] == fin, "%s: single configuration but the layer is not in FSArray order", what);
+    } else {
+        CHECK(mk.perm.size() == fin.size(), "%s: output order holds %zu entries, %zu expected", what, mk.perm.size(),
+              fin.size());
+        for (size_t i = 0; i < fin.size() && i < mk.perm.size(); i++)
+            CHECK(mk.perm[i] == pos[fin[i]], "%s: output %zu: order %u vs %u", what, i, mk.perm[i], pos[fin[i]]);
+    }
+    BlkLayout lay;
+    std::vector<char> img;
+    if (blk_layout_build_host_ex(mk.m_free, mk.nmax_free, w, mk.subs, mk.counts, mk.rowmap, &lay, &img) != OK) {
+        printf("masks build failed %s\n", pcvl_last_error());
+        exit(1);
+    }
+    replay(lay, img, layers, perm, what, 24);
+    replay(lay, img, layers, perm, what, 2048);
+}
+
 int main()
 {
     int cases = 0;
@@ -329,6 +421,32 @@ int main()
             }
     }
     emulate_masked(16, 7, {1, -1, -1, -1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0}, 8); cases++;
+    // several rows, at-least modes, rows that leave a herald mode free
+    struct MaskCase { int m; std::vector<std::vector<int32_t>> rows; std::vector<int32_t> at; };
+    const std::vector<MaskCase> general = {
+        {6, {{-1, -1, -1, 0, 1, 1}, {-1, -1, -1, 1, 1, 0}}, {}},
+        {6, {{-1, -1, 1, -1, -1, 1}}, {0, 0, 1, 0, 0, 0}},
+        {7, {{1, -1, -1, -1, -1, -1, 0}, {-1, -1, 2, -1, -1, -1, -1}}, {}},
+        {7, {{1, -1, -1, 2, -1, -1, -1}, {0, -1, -1, 0, -1, -1, -1}}, {1, 0, 0, 1, 0, 0, 0}},
+        {8, {{0, 1, -1, -1, -1, -1, -1, 1}, {1, 0, -1, -1, -1, -1, -1, 1}, {2, 2, -1, -1, -1, -1, -1, 0}}, {0, 0, 0, 0, 0, 0, 0, 1}},
+        {6, {{3, -1, -1, -1, -1, 3}}, {}},
+        {5, {{-1, 1, -1, -1, 1}, {-1, 1, -1, -1, 1}}, {}}};
+    for (const auto &mc : general) {
+        int nfree = 0;
+        for (int j = 0; j < mc.m; j++) {
+            bool fr = true;
+            for (const auto &r : mc.rows) fr = fr && r[(size_t)j] < 0;
+            nfree += fr;
+        }
+        for (int n = 1; n <= 6; n++)
+            for (int w = 1; w < nfree && w <= 16; w++) {
+                emulate_masks(mc.m, n, mc.rows, mc.at, w);
+                cases++;
+            }
+    }
+    emulate_masks(14, 6, {{1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0}, {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 1}},
+                  {}, 8);
+    cases++;
     printf("%d cases, %d failures\n", cases, fails);
     return fails ? 1 : 0;
 }

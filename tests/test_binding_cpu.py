@@ -38,12 +38,14 @@ def _forbid(name):
 
 class _FakePlan:
     """SlosPlan on the oracle: coefficients and probabilities in FSArray order
-    (with mask conditions: the states with exactly c_j photons in every mode
-    whose condition is c_j >= 0, in FSArray order)."""
+    (with mask conditions: the states that match some condition row, a row
+    asking exactly c_j photons of every mode whose condition is c_j >= 0, at
+    least c_j on the at-least modes, in FSArray order)."""
 
-    def __init__(self, in_occ, mask_cond=None):
+    def __init__(self, in_occ, mask_cond=None, at_least=None):
         self.in_occ = list(in_occ)
-        self.mask_cond = None if mask_cond is None else np.asarray(mask_cond)
+        self.mask_cond = None if mask_cond is None else np.atleast_2d(np.asarray(mask_cond))
+        self.at_least = list(at_least or [])
         _FakePlan.masked_runs += self.mask_cond is not None
 
     masked_runs = 0
@@ -53,8 +55,12 @@ class _FakePlan:
         r = oracle.slos_coefs(U, self.in_occ) if mode == slos.COEFS else oracle.slos_probs(U, self.in_occ)
         if self.mask_cond is not None:
             states = oracle.fock_enumerate(len(self.in_occ), sum(self.in_occ))
-            fixed = self.mask_cond >= 0
-            keep = np.all(states[:, fixed] == self.mask_cond[fixed], axis=1)
+            floor = np.zeros(len(self.in_occ), dtype=bool)
+            floor[self.at_least] = True
+            keep = np.zeros(len(states), dtype=bool)
+            for row in self.mask_cond:
+                ok = (row < 0) | np.where(floor, states >= row, states == row)
+                keep |= np.all(ok, axis=1)
             r = r[keep]
         return torch.from_numpy(r)
 

@@ -57,6 +57,19 @@ def test_mask_semantics():
     assert not mask.match(BasicState("|0,0,1,1,1,1>"))
 
 
+def test_mask_counts_above_nine():
+    """Herald masks write a count c as chr(0x30 + c), up to 32 photons
+    (perceval/simulators/simulator.py:650-662): ':' is 10, 'P' is 32."""
+    from perceval_amd.slos import mask_conditions
+    assert mask_conditions([" :*0P"], 5).tolist() == [[-1, 10, -1, 0, 32]]
+    mask = FSMask(3, 12, [" : "])
+    assert mask.match(BasicState([1, 10, 1])) and not mask.match(BasicState([0, 9, 3]))
+    with pytest.raises(ValueError):
+        mask_conditions(["Q  "], 3)
+    with pytest.raises(ValueError):
+        FSMask(3, 2, ["/  "])
+
+
 def test_bs_conventions_match_reference_formulas():
     th, tl, bl, tr, br = 1.1, 0.3, -0.4, 0.7, 0.2
     c, s = math.cos(th / 2), math.sin(th / 2)

@@ -397,6 +397,101 @@ def test_slos_masked_plan_m24_n10_heralds(gpu):
     plan.close()
 
 
+def _rows_keep(states, rows, at_least):
+    """States matching some condition row: exact counts, floors on the
+    at-least modes, -1 any count."""
+    rows = np.atleast_2d(np.asarray(rows))
+    floor = np.zeros(states.shape[1], dtype=bool)
+    floor[list(at_least)] = True
+    keep = np.zeros(len(states), dtype=bool)
+    for row in rows:
+        keep |= np.all((row < 0) | np.where(floor, states >= row, states == row), axis=1)
+    return keep
+
+
+@pytest.mark.parametrize("inp,rows,at_least", [
+    ([1, 1, 1, 1, 0, 0], [[-1, -1, -1, 0, 1, 1], [-1, -1, -1, 1, 1, 0]], []),       # two rows (test_mask.py:42)
+    ([1, 0, 1, 1, 0, 1], [[-1, -1, 1, -1, -1, 1]], [2]),                               # a floor
+    ([2, 0, 1, 0, 1, 0, 1], [[1, -1, -1, -1, -1, -1, 0], [-1, -1, 2, -1, -1, -1, -1]], []),
+    ([1, 1, 0, 1, 1, 0, 1], [[1, -1, -1, 2, -1, -1, -1], [0, -1, -1, 0, -1, -1, 1]], [0, 3]),
+    ([1, 1, 0, 1, 1, 0, 1], [[1, -1, -1, 2, -1, -1, -1], [0, -1, -1, 0, -1, -1, -1]], [0, 3]),  # row 2 matches all
+    ([1] * 5 + [0] * 3, [[0, 1, -1, -1, -1, -1, -1, 1], [1, 0, -1, -1, -1, -1, -1, 1],
+                         [2, 2, -1, -1, -1, -1, -1, 0]], [7]),
+    ([1, 1, 1, 0, 0, 0], [[1, 1, -1, -1, -1, 3], [2, 2, -1, -1, -1, -1]], []),         # no row reachable: empty
+    ([1] * 7 + [0] * 7, [[1] + [-1] * 12 + [0], [0] + [-1] * 12 + [1]], []),
+])
+def test_slos_masks_plan_vs_full(gpu, inp, rows, at_least):
+    """A plan restricted to several mask rows and at-least modes against the
+    full plan's output filtered by the same masks, 1e-12 absolute, every
+    block width: layers hold the states some matching output extends, the
+    final layer several herald configurations, put back in FSArray order."""
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan
+    m = len(inp)
+    U = haar_unitary(m, seed=7 * m + len(rows))
+    full = SlosPlan(inp)
+    keep = _rows_keep(oracle.fock_enumerate(m, sum(inp)), rows, at_least)
+    pf = full.run(U, PROBS).cpu().numpy()[keep]
+    cf = full.run(U, COEFS).cpu().numpy()[keep]
+    plan = SlosPlan(inp, mask_cond=rows, at_least=at_least)
+    assert plan.count == int(keep.sum())
+    if plan.count == 0:
+        assert plan.run(U, PROBS).numel() == 0
+        return
+    free = int((np.asarray(rows) < 0).all(axis=0).sum())
+    for w in range(1, min(free - 1, 16) + 1):
+        plan.set_block_width(w)
+        p = plan.run(U, PROBS).cpu().numpy()
+        c = plan.run(U, COEFS).cpu().numpy()
+        assert np.max(np.abs(p - pf)) < ABS_TOL, w
+        assert np.max(np.abs(c - cf)) < ABS_TOL, w
+    assert np.max(np.abs(pf - oracle.slos_probs(U, inp)[keep])) < ABS_TOL
+    plan.close()
+    full.close()
+
+
+def test_slos_masks_plan_m24_n10(gpu):
+    """Two herald patterns on modes 20..23 of the m=24, n=10 configuration,
+    one of them a floor on mode 23: every output of the masked plan within
+    1e-12 of the full plan's probability of the same state, in the order of
+    the GPU mask selection (FSArray order)."""
+    import torch
+    from perceval_amd.slos import PROBS, SlosPlan, fock_mask_select, gather_f64
+    U = haar_unitary(24, seed=25)
+    inp = [1] * 10 + [0] * 14
+    masks = [" " * 20 + "0101", " " * 20 + "1001"]
+    plan = SlosPlan(inp, mask_cond=[[-1] * 20 + [0, 1, 0, 1], [-1] * 20 + [1, 0, 0, 1]], at_least=[23])
+    idx = fock_mask_select(24, 10, masks, [23])
+    assert plan.count == idx.numel()
+    _say(f"masks plan: {plan.count} outputs, workspace {plan.work_bytes / 2**20:.0f} MiB")
+    got = plan.run(U, PROBS)
+    ref = gather_f64(SlosPlan(inp).run(U, PROBS), idx)
+    torch.cuda.synchronize()
+    err = float((got - ref).abs().max().item())
+    _say(f"masks vs full: max abs err {err:.3e}")
+    assert err < ABS_TOL
+    plan.close()
+
+
+def test_slos_backend_masks_use_a_masked_plan(gpu):
+    """SLOSBackend.all_prob with two masks and an at-least mode: the masked
+    plan's values equal the full distribution filtered by FSMask matching."""
+    from perceval_amd.backends import SLOSBackend
+    from perceval_amd.components import Unitary
+    from perceval_amd.utils.states import FockState
+    U = haar_unitary(7, seed=71)
+    b = SLOSBackend()
+    b.set_circuit(Unitary(U))
+    b.set_input_state(FockState([1, 1, 0, 1, 1, 0, 1]))
+    full = np.asarray(b.all_prob())
+    b.set_mask(["1  2   ", "0  0  1"], at_least_modes=[0, 3])
+    b.set_input_state(FockState([1, 1, 0, 1, 1, 0, 1]))
+    got = np.asarray(b.all_prob())
+    assert b._plans_masked and all(p is not None for p in b._plans_masked.values())
+    keep = _rows_keep(oracle.fock_enumerate(7, 5), [[1, -1, -1, 2, -1, -1, -1], [0, -1, -1, 0, -1, -1, 1]], [0, 3])
+    assert got.shape == (int(keep.sum()),)
+    assert np.max(np.abs(got - full[keep])) < ABS_TOL
+
+
 def test_slos_masked_plan_rejects_bad_masks(gpu):
     from perceval_amd.slos import SlosPlan
     with pytest.raises(ValueError):
