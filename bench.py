@@ -438,6 +438,24 @@ def bench_slos(args, res):
     ms = per_kernel[0]
     total = float(out.sum().item())
     plan.close()
+    # masked plans on the same input: heralds 0,1,0,1 on modes 20..23, and two
+    # herald rows with a floor on mode 23 (every layer restricted to the
+    # states that can still match)
+    masked = {}
+    for name, rows, at_least in (("heralds_0101", [[-1] * 20 + [0, 1, 0, 1]], []),
+                                 ("rows_0101_1001_floor23", [[-1] * 20 + [0, 1, 0, 1], [-1] * 20 + [1, 0, 0, 1]], [23])):
+        mplan = SlosPlan(inp, mask_cond=rows, at_least=at_least)
+        mout = torch.empty(max(1, mplan.count), dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            mplan.run(dU, PROBS, mout)
+        with Timer() as t:
+            for _ in range(10):
+                mplan.run(dU, PROBS, mout)
+        masked[name] = {"ms": t.ms / 10, "outputs": mplan.count}
+        mplan.close()
+    res["slos_m24_n10_masked"] = masked
+    log("[bench] SLOS m=24 n=10 masked: " + ", ".join(f"{k} {v['outputs']} outputs {v['ms']:.3f} ms"
+                                                       for k, v in masked.items()))
     gbytes = slos_bytes(m, n) / 1e9
     res["slos_m24_n10"] = {"ms": ms, "outputs": plan.count, "algorithmic_GB": gbytes,
                            "GB_per_s": gbytes / (ms * 1e-3), "frac_hbm_peak": gbytes / (ms * 1e-3) / HBM_PEAK_GBS,

@@ -126,6 +126,26 @@ def test_slos_successor_and_block_kernels_bit_identical(gpu, inp):
     plan.close()
 
 
+def test_slos_probs_thirteen_photons(gpu):
+    """13 photons in 8 modes (77,520 states): the kernels' probability
+    epilogues in the double-precision factorial form (products above 12! do
+    not fit 32 bits), against the oracle at 1e-12 and bit-identical to the
+    two-phase kernel."""
+    from perceval_amd.slos import PROBS, SlosPlan
+    U = haar_unitary(8, seed=13)
+    inp = [3, 0, 2, 1, 0, 4, 1, 2]
+    plan = SlosPlan(inp)
+    assert plan.count == 77520
+    got = {}
+    for kernel in (4, 6, 8, 0):
+        plan.set_kernel(kernel)
+        got[kernel] = plan.run(U, PROBS).cpu().numpy()
+    assert np.max(np.abs(got[0] - oracle.slos_probs(U, inp))) < ABS_TOL
+    for kernel in (6, 8, 0):
+        assert np.array_equal(got[4], got[kernel]), kernel
+    plan.close()
+
+
 @pytest.mark.parametrize("inp", [[1] * 6 + [0] * 6, [1, 1, 0, 2, 0, 0, 1, 0, 3, 0, 0, 1], [0] * 11 + [7],
                                  [3, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1], [9, 5, 0], [4, 3]])
 def test_slos_block_kernel_every_width(gpu, inp):
