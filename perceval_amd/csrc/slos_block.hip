@@ -27,6 +27,8 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <string>
+#include <unordered_map>
 #include <new>
 #include <type_traits>
 
@@ -689,84 +691,115 @@ int blk_masked_layers_ex(int m, int n, int nrows, const int32_t *conds, const in
             const int32_t c = rc[(size_t)r * nh + i];
             cap[(size_t)i] = std::max(cap[(size_t)i], (c < 0 || rl[(size_t)r * nh + i]) ? n : std::min(c, n));
         }
-    // the missing photons of configuration a for row r (-1: a has more than row r allows)
-    auto missing = [&](const std::vector<int32_t> &a, int r) {
-        int d = 0;
-        for (int i = 0; i < nh; i++) {
-            const int32_t c = rc[(size_t)r * nh + i];
-            if (c < 0) continue;
-            if (rl[(size_t)r * nh + i]) d += std::max(0, c - a[(size_t)i]);
-            else if (a[(size_t)i] > c) return -1;
-            else d += c - a[(size_t)i];
-        }
-        return d;
-    };
-    // layer k keeps configuration a when some row still fits in the n - k photons to come
-    auto kept = [&](const std::vector<int32_t> &a, int k) {
+    // the fewest photons configuration a still misses for some row (-1: a has
+    // more than every row allows on some exact mode)
+    auto dmin_of = [&](const std::vector<int32_t> &a) {
+        int best = -1;
         for (int r = 0; r < nrows; r++) {
-            const int d = missing(a, r);
-            if (d >= 0 && d <= n - k) return true;
+            int d = 0;
+            bool fits = true;
+            for (int i = 0; i < nh && fits; i++) {
+                const int32_t c = rc[(size_t)r * nh + i];
+                if (c < 0) continue;
+                if (rl[(size_t)r * nh + i]) d += std::max(0, c - a[(size_t)i]);
+                else if (a[(size_t)i] > c) fits = false;
+                else d += c - a[(size_t)i];
+            }
+            if (fits && (best < 0 || d < best)) best = d;
         }
-        return false;
+        return best;
     };
-    // every configuration a <= cap with |a| <= n, in descending lexicographic order
+    // every configuration a <= cap with |a| <= n, in descending lexicographic
+    // order: the successor takes one from the last nonzero count and refills
+    // the counts after it greedily.  The count is bounded so that the tables
+    // below stay within about 2^28 steps of host work.
+    const size_t max_conf = std::min<size_t>(1000000, ((size_t)1 << 28) / ((size_t)std::max(1, nh) * (nrows + n + 2)));
     std::vector<std::vector<int32_t>> all;
+    std::vector<int> asum, dmin;
     {
-        std::vector<int32_t> a(cap);
+        std::vector<int32_t> a((size_t)nh, 0);
+        int left = n;
+        for (int i = 0; i < nh; i++) { a[(size_t)i] = std::min(cap[(size_t)i], left); left -= a[(size_t)i]; }
         while (true) {
             int sa = 0;
             for (int32_t v : a) sa += v;
-            if (sa <= n) {
-                all.push_back(a);
-                if (all.size() > 1000000) { set_error("masked SLOS: more than 10^6 herald configurations"); return ERR_RANGE; }
+            all.push_back(a);
+            asum.push_back(sa);
+            dmin.push_back(dmin_of(a));
+            if (all.size() > max_conf) {
+                set_error("masked SLOS: more than " + std::to_string(max_conf) + " herald configurations");
+                return ERR_RANGE;
             }
             int i = nh - 1;
             while (i >= 0 && a[(size_t)i] == 0) i--;
             if (i < 0) break;
             a[(size_t)i]--;
-            for (int t = i + 1; t < nh; t++) a[(size_t)t] = cap[(size_t)t];
+            left = n - (sa - 1);
+            for (int t = i + 1; t < nh; t++) { a[(size_t)t] = std::min(cap[(size_t)t], left); left -= a[(size_t)t]; }
         }
     }
-    auto size_of = [](const std::vector<int32_t> &a) { int s = 0; for (int32_t v : a) s += v; return s; };
+    const size_t NA = all.size();
+    // down[t * nh + i]: the index of all[t] - e_i (every such vector is itself a configuration)
+    std::vector<int32_t> down(NA * (size_t)nh, -1);
+    {
+        struct VecHash {
+            size_t operator()(const std::vector<int32_t> &v) const {
+                uint64_t h = 1469598103934665603ull;
+                for (int32_t x : v) h = (h ^ (uint32_t)x) * 1099511628211ull;
+                return (size_t)h;
+            }
+        };
+        std::unordered_map<std::vector<int32_t>, int32_t, VecHash> idx_of;
+        idx_of.reserve(NA * 2);
+        for (size_t t = 0; t < NA; t++) idx_of.emplace(all[t], (int32_t)t);
+        for (size_t t = 0; t < NA; t++)
+            for (int i = 0; i < nh; i++) {
+                if (all[t][(size_t)i] == 0) continue;
+                auto b = all[t];
+                b[(size_t)i]--;
+                auto it = idx_of.find(b);
+                if (it == idx_of.end()) { set_error("masked SLOS: internal: configuration missing"); return ERR_RANGE; }
+                down[t * (size_t)nh + i] = it->second;
+            }
+    }
     mk.subs.assign((size_t)n + 1, {});
     mk.counts.assign((size_t)n + 1, 0);
-    std::vector<std::map<std::vector<int32_t>, uint32_t>> base((size_t)n + 1);
+    // pos[t]: offset of configuration t's sub-layer in the layer (-1: not kept);
+    // layer k keeps a when |a| <= k and some row still fits in the n - k photons to come
+    std::vector<int64_t> pos_prev(NA, -1), pos(NA, -1);
     for (int k = 0; k <= n; k++) {
         uint64_t off = 0;
-        for (const auto &a : all) {
-            const int sa = size_of(a);
-            if (sa > k || !kept(a, k)) continue;
-            base[(size_t)k][a] = (uint32_t)off;
-            mk.nmax_free = std::max(mk.nmax_free, k - sa);
-            off += nstates(mk.m_free, k - sa);
+        for (size_t t = 0; t < NA; t++) {
+            pos[t] = -1;
+            if (asum[t] > k || dmin[t] < 0 || dmin[t] > n - k) continue;
+            pos[t] = (int64_t)off;
+            mk.nmax_free = std::max(mk.nmax_free, k - asum[t]);
+            off += nstates(mk.m_free, k - asum[t]);
             if (off >= (1ull << 32)) { set_error("masked SLOS: a layer holds 2^32 states or more"); return ERR_RANGE; }
         }
         mk.counts[(size_t)k] = off;
-        if (k == 0) continue;
-        for (const auto &a : all) {
-            auto it = base[(size_t)k].find(a);
-            if (it == base[(size_t)k].end()) continue;
-            BlkSub sub;
-            sub.out_base = it->second;
-            sub.kf = k - size_of(a);
-            if (sub.kf >= 1) {
-                auto pit = base[(size_t)k - 1].find(a);
-                if (pit == base[(size_t)k - 1].end()) { set_error("masked SLOS: internal: parent sub-layer missing"); return ERR_RANGE; }
-                sub.par_base = pit->second;
+        if (k > 0)
+            for (size_t t = 0; t < NA; t++) {
+                if (pos[t] < 0) continue;
+                BlkSub sub;
+                sub.out_base = (uint32_t)pos[t];
+                sub.kf = k - asum[t];
+                if (sub.kf >= 1) {
+                    if (pos_prev[t] < 0) { set_error("masked SLOS: internal: parent sub-layer missing"); return ERR_RANGE; }
+                    sub.par_base = (uint32_t)pos_prev[t];
+                }
+                for (int i = 0; i < nh; i++) {
+                    if (all[t][(size_t)i] == 0) continue;
+                    const int32_t b = down[t * (size_t)nh + i];
+                    if (pos_prev[(size_t)b] < 0) { set_error("masked SLOS: internal: herald parent missing"); return ERR_RANGE; }
+                    sub.her.push_back({(uint32_t)i, (uint32_t)pos_prev[(size_t)b]});
+                }
+                if (k == n)
+                    for (int32_t v : all[t])
+                        for (int f = 2; f <= v; f++) sub.hfact *= f;
+                mk.subs[(size_t)k].push_back(std::move(sub));
             }
-            for (int i = 0; i < nh; i++) {
-                if (a[(size_t)i] == 0) continue;
-                auto b = a;
-                b[(size_t)i]--;
-                auto pit = base[(size_t)k - 1].find(b);
-                if (pit == base[(size_t)k - 1].end()) { set_error("masked SLOS: internal: herald parent missing"); return ERR_RANGE; }
-                sub.her.push_back({(uint32_t)i, pit->second});
-            }
-            if (k == n)
-                for (int32_t v : a)
-                    for (int t = 2; t <= v; t++) sub.hfact *= t;
-            mk.subs[(size_t)k].push_back(std::move(sub));
-        }
+        if (k < n) std::swap(pos_prev, pos);
     }
     if (mk.counts[(size_t)n] == 0) {
         mk.empty = true;
@@ -778,12 +811,19 @@ int blk_masked_layers_ex(int m, int n, int nrows, const int32_t *conds, const in
     // down to 0) that keeps only the final configurations still reachable:
     // the same herald values so far, their remaining herald photons within
     // the photons left (exactly, once no free mode is left)
-    if (base[(size_t)n].size() > 1) {
+    std::vector<std::pair<std::vector<int32_t>, uint32_t>> finals;   // (configuration, sub-layer offset)
+    for (size_t t = 0; t < NA; t++)
+        if (pos[t] >= 0) finals.push_back({all[t], (uint32_t)pos[t]});
+    if (finals.size() > 1) {
         if (mk.counts[(size_t)n] > (1ull << 28)) { set_error("masked SLOS: more than 2^28 kept outputs"); return ERR_RANGE; }
+        // the walk below filters the candidate configurations at every node
+        if ((double)mk.counts[(size_t)n] * (double)(finals.size() + (size_t)m) > 0x1p34) {
+            set_error("masked SLOS: too many outputs times final herald configurations to put in FSArray order");
+            return ERR_RANGE;
+        }
         std::vector<int32_t> hidx((size_t)m, -1), fidx((size_t)m, -1);
         for (int i = 0; i < nh; i++) hidx[(size_t)H[(size_t)i]] = i;
         for (int e = 0; e < mk.m_free; e++) fidx[(size_t)mk.rowmap[(size_t)e]] = e;
-        std::vector<std::pair<std::vector<int32_t>, uint32_t>> finals(base[(size_t)n].begin(), base[(size_t)n].end());
         const size_t nf = finals.size();
         // suf[t * (m + 1) + q]: herald photons of final configuration t on modes q..m-1
         std::vector<int> suf(nf * ((size_t)m + 1), 0), free_after((size_t)m + 1, 0);

@@ -447,6 +447,13 @@ int main()
     emulate_masks(14, 6, {{1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0}, {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 1}},
                   {}, 8);
     cases++;
+    {   // floors of 0 on 24 modes: C(34, 10) configurations, refused after the cap, not enumerated
+        std::vector<int32_t> row(26, 0), at(26, 1);
+        row[0] = row[1] = -1;
+        BlkMasked mk;
+        CHECK(blk_masked_layers_ex(26, 10, 1, row.data(), at.data(), &mk) == ERR_RANGE, "configuration cap");
+        cases++;
+    }
     printf("%d cases, %d failures\n", cases, fails);
     return fails ? 1 : 0;
 }
