@@ -109,14 +109,20 @@ int pcvl_fock_rank(const int32_t *d_occ, int64_t count, int m, int nmax, int64_t
 int pcvl_fock_unrank(const int64_t *d_rank, int64_t count, int m, int n, int32_t *d_occ,
                      hipStream_t stream);
 
-/* Ranks (FSArray order) of the n-photon, m-mode states that match any of the
- * nmask masks (AStrongSimulationBackend.set_mask / xq.FSMask,
- * perceval/backends/_abstract_backends.py:103-146): h_cond holds nmask rows of
- * m conditions, -1 for any count, else a count matched exactly, or reached at
- * least on the modes where h_at_least (m flags, may be NULL) is non-zero.
- * d_out must hold C(m+n-1, n) entries; *d_count (device) receives the number
- * of matches.  At most 16 masks and 2^34 states.  Synchronises `stream`. */
-int pcvl_fock_mask_select(int m, int n, const int32_t *h_cond, int nmask, const int32_t *h_at_least,
+/* Ranks (FSArray order) of the n-photon, m-mode states that the masks keep
+ * (AStrongSimulationBackend.set_mask / xq.FSMask / the masked xq.FSArray,
+ * perceval/backends/_abstract_backends.py:103-146): h_cond holds nmask rows
+ * of m conditions, -1 for any count, else a count matched exactly, or reached
+ * at least on the modes where h_at_least (m flags, may be NULL) is non-zero.
+ * mask_n is the photon count the masks were set for (set_mask's n): with
+ * mask_n <= n a state is kept when it matches a row; with mask_n > n, when
+ * the mask_n - n photons still to come can complete a row (no exact count
+ * exceeded, the photons missing from exact counts and floors at most
+ * mask_n - n), as the reference's simulator needs for the separated parts of
+ * a noisy input (perceval/simulators/simulator.py:412-470).  d_out must hold
+ * C(m+n-1, n) entries; *d_count (device) receives the number of states kept.
+ * At most 16 masks and 2^34 states.  Synchronises `stream`. */
+int pcvl_fock_mask_select(int m, int n, int mask_n, const int32_t *h_cond, int nmask, const int32_t *h_at_least,
                           int64_t *d_out, int64_t *d_count, hipStream_t stream);
 
 /* d_dst[k] = d_src[d_idx[k]] (float64), e.g. the probabilities of a mask's
@@ -171,15 +177,18 @@ int pcvl_slos_plan_create_masked(int m, const int32_t *h_in_occ, const int32_t *
 /* The general masks of set_mask(masks, n, at_least_modes): nrows condition
  * rows (h_conds, nrows x m, -1 any count, c >= 0 a count), a state matching
  * any row; h_at_least (m flags, or NULL) makes a mode's count a floor (at
- * least c).  Layer k keeps the states whose constrained-mode counts can still
- * reach some row with the n - k photons to come (tests/utils/test_mask.py:
- * 33-45, partial match).  The final layer may then hold several herald
- * configurations; execute writes the matching states in FSArray order (a
- * final reordering pass, whose buffer pcvl_slos_plan_info's work_bytes
- * includes).  Needs at least two modes that no row constrains, 1..30
- * photons, at most 64 rows and 10^6 herald configurations. */
+ * least c).  mask_n is set_mask's n (0: the input's photon count n; else at
+ * least n).  Layer k keeps the states whose constrained-mode counts can still
+ * reach some row with the mask_n - k photons to come (tests/utils/test_mask.py:
+ * 33-45, partial match), so with mask_n > n the output holds the states that
+ * photons from elsewhere can complete (the selection rule of
+ * pcvl_fock_mask_select).  The final layer may then hold several herald
+ * configurations; execute writes the kept states in FSArray order (a final
+ * reordering pass, whose buffer pcvl_slos_plan_info's work_bytes includes).
+ * Needs at least two modes that no row constrains, 1..30 photons, at most 64
+ * rows and 10^6 herald configurations. */
 int pcvl_slos_plan_create_masks(int m, const int32_t *h_in_occ, int nrows, const int32_t *h_conds,
-                                const int32_t *h_at_least, pcvl_slos_plan *out);
+                                const int32_t *h_at_least, int mask_n, pcvl_slos_plan *out);
 
 /* Several input states in one plan: the _Path tree of
  * SLOSBackend.preprocess([inputs]) (perceval/backends/_slos.py:37-86,170-185),

@@ -50,11 +50,11 @@ SIGNATURES = {
     "pcvl_fock_count": (c_i64, [c_int, c_int]),
     "pcvl_fock_rank": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
     "pcvl_fock_unrank": (c_int, [c_void_p, c_i64, c_int, c_int, c_void_p, c_void_p]),
-    "pcvl_fock_mask_select": (c_int, [c_int, c_int, P(c_i32), c_int, P(c_i32), c_void_p, c_void_p, c_void_p]),
+    "pcvl_fock_mask_select": (c_int, [c_int, c_int, c_int, P(c_i32), c_int, P(c_i32), c_void_p, c_void_p, c_void_p]),
     "pcvl_gather_f64": (c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_void_p]),
     "pcvl_slos_plan_create": (c_int, [c_int, P(c_i32), P(c_void_p)]),
     "pcvl_slos_plan_create_masked": (c_int, [c_int, P(c_i32), P(c_i32), P(c_void_p)]),
-    "pcvl_slos_plan_create_masks": (c_int, [c_int, P(c_i32), c_int, P(c_i32), P(c_i32), ctypes.POINTER(c_void_p)]),
+    "pcvl_slos_plan_create_masks": (c_int, [c_int, P(c_i32), c_int, P(c_i32), P(c_i32), c_int, ctypes.POINTER(c_void_p)]),
     "pcvl_slos_plan_info": (c_int, [c_void_p, P(c_int), P(c_i64), P(c_size_t), P(c_i32)]),
     "pcvl_slos_plan_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "pcvl_slos_plan_destroy": (c_int, [c_void_p]),

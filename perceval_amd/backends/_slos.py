@@ -131,7 +131,8 @@ class SLOSBackend(AStrongSimulationBackend):
             return None
         key = ("mask_idx", istate.m, istate.n)
         if key not in self._results:
-            self._results[key] = fock_mask_select(istate.m, istate.n, self._masks_str, self._no_limit_modes)
+            self._results[key] = fock_mask_select(istate.m, istate.n, self._masks_str, self._no_limit_modes,
+                                                  self._mask_n)
         return self._results[key]
 
     def _masked_indices(self, istate):
@@ -148,27 +149,28 @@ class SLOSBackend(AStrongSimulationBackend):
         return complex(coefs[idx] * math.sqrt(output_state.prodnfact() / self._input_state.prodnfact()))
 
     def _mask_plan_args(self, istate):
-        """(condition rows, at-least modes) of the mask for a masked plan, or
-        None when the mask goes through selection from the full output layer
-        (no mode constrained, fewer than two modes free in every row, more
-        than 64 rows, a photon count outside 1..30 or another than the
-        mask's)."""
+        """(condition rows, at-least modes, mask photon count) of the mask for
+        a masked plan, or None when the mask goes through selection from the
+        full output layer (no mode constrained, fewer than two modes free in
+        every row, more than 64 rows, a photon count outside 1..30, or a mask
+        set for fewer photons than the input's)."""
         if self._mask is None or len(self._masks_str) > 64:
             return None
         cond = mask_conditions(self._masks_str, istate.m)
         free = (cond < 0).all(axis=0)
         if free.sum() < 2 or free.all() or istate.n < 1 or istate.n > 30:
             return None
-        if self._mask_n is not None and self._mask_n != istate.n:
+        mask_n = self._mask_n or istate.n
+        if mask_n < istate.n:
             return None
-        return cond, tuple(int(q) for q in (self._no_limit_modes or ()))
+        return cond, tuple(int(q) for q in (self._no_limit_modes or ())), int(mask_n)
 
     def _masked_plan(self, istate, args):
-        cond, at_least = args
-        key = ("masked_plan", istate, cond.tobytes(), at_least)
+        cond, at_least, mask_n = args
+        key = ("masked_plan", istate, cond.tobytes(), at_least, mask_n)
         if key not in self._plans_masked:
             try:
-                self._plans_masked[key] = SlosPlan(list(istate), mask_cond=cond, at_least=at_least)
+                self._plans_masked[key] = SlosPlan(list(istate), mask_cond=cond, at_least=at_least, mask_n=mask_n)
             except NativeError as e:
                 if "herald configurations" not in str(e) and "2^32" not in str(e):
                     raise
@@ -217,7 +219,7 @@ class SLOSBackend(AStrongSimulationBackend):
         pin = istate.prodnfact()
         for r, c in zip(rows, coefs):
             s = FockState(r)
-            if self._mask is not None and not self._mask.match(s):
+            if self._mask is not None and not self._mask.keeps(s):
                 continue
             res += s * complex(c * math.sqrt(s.prodnfact() / pin))
         return res

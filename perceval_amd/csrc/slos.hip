@@ -571,11 +571,19 @@ constexpr int MASK_THREADS = 256;
 constexpr int MASK_MAX = 16;   // masks per selection
 
 // Walks rank rk of the n-photon space and tests every mask on the way; bit k
-// of the result is set when mask k matches.
+// of the result is set when mask k can be completed by `spare` more photons:
+// no exact count exceeded, the photons missing from the exact counts and the
+// floors at most `spare`, and exactly `spare` when mask k has no mode to take
+// the surplus (bit k of `absorbs`: a free or at-least mode; spare 0: an exact
+// match).
 __device__ __forceinline__ uint32_t mask_walk(uint64_t rk, int m, int n, const uint64_t *sT, int w,
-                                              const int32_t *cond, int nmask, const int32_t *at_least)
+                                              const int32_t *cond, int nmask, const int32_t *at_least, int spare,
+                                              uint32_t absorbs)
 {
     uint32_t ok = (nmask >= 32) ? 0xffffffffu : ((1u << nmask) - 1u);
+    int miss[MASK_MAX];
+#pragma unroll
+    for (int k = 0; k < MASK_MAX; k++) miss[k] = 0;
     uint64_t rem = rk;
     int R = n;
     for (int q = 0; q < m; q++) {
@@ -588,12 +596,20 @@ __device__ __forceinline__ uint32_t mask_walk(uint64_t rk, int m, int n, const u
             rem -= tv;
         }
         const int sq = R - Rn;
-        for (int k = 0; k < nmask; k++) {
+#pragma unroll
+        for (int k = 0; k < MASK_MAX; k++) {
+            if (k >= nmask) break;
             const int c = cond[k * m + q];
-            if (c >= 0 && !(at_least[q] ? sq >= c : sq == c)) ok &= ~(1u << k);
+            if (c < 0) continue;
+            if (at_least[q]) miss[k] += c > sq ? c - sq : 0;
+            else if (sq > c) ok &= ~(1u << k);
+            else miss[k] += c - sq;
         }
         R = Rn;
     }
+#pragma unroll
+    for (int k = 0; k < MASK_MAX; k++)
+        if (k < nmask && (miss[k] > spare || (miss[k] < spare && !((absorbs >> k) & 1u)))) ok &= ~(1u << k);
     return ok;
 }
 
@@ -601,7 +617,8 @@ __device__ __forceinline__ uint32_t mask_walk(uint64_t rk, int m, int n, const u
 __global__ void __launch_bounds__(MASK_THREADS) mask_count_kernel(uint64_t total, int m, int n,
                                                                   const uint64_t *__restrict__ gT, int w,
                                                                   const int32_t *__restrict__ gcond, int nmask,
-                                                                  const int32_t *__restrict__ gat,
+                                                                  const int32_t *__restrict__ gat, int spare,
+                                                                  uint32_t absorbs,
                                                                   uint8_t *__restrict__ flags,
                                                                   int32_t *__restrict__ counts)
 {
@@ -615,7 +632,7 @@ __global__ void __launch_bounds__(MASK_THREADS) mask_count_kernel(uint64_t total
     const uint64_t i = (uint64_t)blockIdx.x * MASK_THREADS + threadIdx.x;
     bool hit = false;
     if (i < total) {
-        hit = mask_walk(i, m, n, sT, w, scond, nmask, sat) != 0u;
+        hit = mask_walk(i, m, n, sT, w, scond, nmask, sat, spare, absorbs) != 0u;
         flags[i] = hit ? 1 : 0;
     }
     const int c = __syncthreads_count(hit);
@@ -818,11 +835,11 @@ extern "C" int pcvl_slos_plan_create(int m, const int32_t *h_in_occ, pcvl_slos_p
 extern "C" int pcvl_slos_plan_create_masked(int m, const int32_t *h_in_occ, const int32_t *h_cond,
                                             pcvl_slos_plan *out)
 {
-    return pcvl_slos_plan_create_masks(m, h_in_occ, 1, h_cond, nullptr, out);
+    return pcvl_slos_plan_create_masks(m, h_in_occ, 1, h_cond, nullptr, 0, out);
 }
 
 extern "C" int pcvl_slos_plan_create_masks(int m, const int32_t *h_in_occ, int nrows, const int32_t *h_conds,
-                                           const int32_t *h_at_least, pcvl_slos_plan *out)
+                                           const int32_t *h_at_least, int mask_n, pcvl_slos_plan *out)
 {
     PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
     PCVL_ARG_CHECK(h_in_occ != nullptr && h_conds != nullptr, "input state or mask is null");
@@ -841,6 +858,8 @@ extern "C" int pcvl_slos_plan_create_masks(int m, const int32_t *h_in_occ, int n
         nfree += free_mode ? 1 : 0;
     }
     PCVL_ARG_CHECK(n >= 1 && n <= 30, "masked plans take 1 to 30 photons");
+    PCVL_ARG_CHECK(mask_n == 0 || (mask_n >= n && mask_n <= 1024),
+                   "mask photon count must be 0 (the input's) or between the input's and 1024");
     PCVL_ARG_CHECK(nfree >= 2, "a masked plan needs at least two modes that no mask row constrains");
     auto *plan = new (std::nothrow) pcvl_slos_plan_s();
     if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
@@ -859,7 +878,7 @@ extern "C" int pcvl_slos_plan_create_masks(int m, const int32_t *h_in_occ, int n
         double pin = 1.0;
         for (int i = 0; i < m; i++) for (int v = 2; v <= h_in_occ[i]; v++) pin *= v;
         int rc = device_grid_cap(&p.grid_cap);
-        if (rc == OK) rc = blk_masked_layers_ex(m, n, nrows, h_conds, h_at_least, &p.mask);
+        if (rc == OK) rc = blk_masked_layers_ex(m, n, mask_n > n ? mask_n : n, nrows, h_conds, h_at_least, &p.mask);
         if (rc != OK) { delete plan; return rc; }
         // the herald modes' factorials go with each final sub-layer (its groups)
         p.inv_prod_in = 1.0 / pin;
@@ -1373,11 +1392,13 @@ extern "C" int64_t pcvl_fock_count(int m, int n)
     return c > (uint64_t)INT64_MAX ? -1 : (int64_t)c;
 }
 
-extern "C" int pcvl_fock_mask_select(int m, int n, const int32_t *h_cond, int nmask, const int32_t *h_at_least,
-                                     int64_t *d_out, int64_t *d_count, hipStream_t stream)
+extern "C" int pcvl_fock_mask_select(int m, int n, int mask_n, const int32_t *h_cond, int nmask,
+                                     const int32_t *h_at_least, int64_t *d_out, int64_t *d_count, hipStream_t stream)
 {
     PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 64]");
     PCVL_ARG_CHECK(n >= 0 && n <= 63, "photon count outside [0, 63]");
+    PCVL_ARG_CHECK(mask_n >= 0 && mask_n <= 1024, "mask photon count outside [0, 1024]");
+    const int spare = mask_n > n ? mask_n - n : 0;
     PCVL_ARG_CHECK(nmask >= 1 && nmask <= MASK_MAX, "between 1 and 16 masks");
     PCVL_ARG_CHECK(h_cond && d_out && d_count, "null buffer");
     const uint64_t total = binom_u64((int64_t)m + n - 1, n);
@@ -1387,6 +1408,10 @@ extern "C" int pcvl_fock_mask_select(int m, int n, const int32_t *h_cond, int nm
     build_tables(m, n, T, E);
     std::vector<int32_t> at((size_t)m, 0);
     if (h_at_least) for (int q = 0; q < m; q++) at[(size_t)q] = h_at_least[q] ? 1 : 0;
+    uint32_t absorbs = 0;   // masks with a free or at-least mode
+    for (int k = 0; k < nmask; k++)
+        for (int q = 0; q < m; q++)
+            if (h_cond[(size_t)k * m + q] < 0 || at[(size_t)q]) absorbs |= 1u << k;
     const int64_t nb = (int64_t)((total + MASK_THREADS - 1) / MASK_THREADS);
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t b_T = al(sizeof(uint64_t) * T.size());
@@ -1409,7 +1434,7 @@ extern "C" int pcvl_fock_mask_select(int m, int n, const int32_t *h_cond, int nm
     if (e == hipSuccess) e = hipMemcpyAsync(da, at.data(), sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(mask_count_kernel, dim3((unsigned)nb), dim3(MASK_THREADS), 0, stream, total, m, n, dT, w, dc,
-                           nmask, da, dfl, dcnt);
+                           nmask, da, spare, absorbs, dfl, dcnt);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {

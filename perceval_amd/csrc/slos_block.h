@@ -139,7 +139,8 @@ struct BlkMasked {
     std::vector<uint32_t> perm;                 // final layer in FSArray order (empty: already in order)
 };
 int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out);
-int blk_masked_layers_ex(int m, int n, int nrows, const int32_t *conds, const int32_t *at_least, BlkMasked *out);
+int blk_masked_layers_ex(int m, int n, int n_mask, int nrows, const int32_t *conds, const int32_t *at_least,
+                         BlkMasked *out);
 
 // Uploads a host-built layout (allocates d_mem).
 int blk_layout_upload(BlkLayout *lay, const std::vector<char> &image);

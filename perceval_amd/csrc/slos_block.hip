@@ -659,12 +659,13 @@ static size_t blk_lds_bytes(const BlkLayout &lay, int k)
 
 int blk_masked_layers(int m, int n, const int32_t *cond, BlkMasked *out)
 {
-    return blk_masked_layers_ex(m, n, 1, cond, nullptr, out);
+    return blk_masked_layers_ex(m, n, n, 1, cond, nullptr, out);
 }
 
-int blk_masked_layers_ex(int m, int n, int nrows, const int32_t *conds, const int32_t *at_least, BlkMasked *out)
+int blk_masked_layers_ex(int m, int n, int n_mask, int nrows, const int32_t *conds, const int32_t *at_least,
+                         BlkMasked *out)
 {
-    if (m < 1 || n < 0 || nrows < 1 || !conds) { set_error("masked SLOS: bad mask arguments"); return ERR_ARG; }
+    if (m < 1 || n < 0 || n_mask < n || nrows < 1 || !conds) { set_error("masked SLOS: bad mask arguments"); return ERR_ARG; }
     BlkMasked mk;
     std::vector<int32_t> H;
     for (int j = 0; j < m; j++) {
@@ -675,6 +676,9 @@ int blk_masked_layers_ex(int m, int n, int nrows, const int32_t *conds, const in
     }
     mk.m_free = (int)mk.rowmap.size();
     mk.nher = (int)H.size();
+    // a free mode takes any photons beyond a row's missing ones, which makes
+    // "missing <= photons to come" the exact test for a row still reachable
+    if (mk.m_free < 1) { set_error("masked SLOS: needs a mode that no mask row constrains"); return ERR_ARG; }
     for (int32_t j : H) mk.rowmap.push_back(j);
     const int nh = (int)H.size();
     // per row, over the herald modes: the count asked (-1 any) and whether it is a floor
@@ -765,13 +769,15 @@ int blk_masked_layers_ex(int m, int n, int nrows, const int32_t *conds, const in
     mk.subs.assign((size_t)n + 1, {});
     mk.counts.assign((size_t)n + 1, 0);
     // pos[t]: offset of configuration t's sub-layer in the layer (-1: not kept);
-    // layer k keeps a when |a| <= k and some row still fits in the n - k photons to come
+    // layer k keeps a when |a| <= k and some row still fits in the n_mask - k
+    // photons to come (n_mask > n: the output layer itself holds the states
+    // that photons from elsewhere can complete)
     std::vector<int64_t> pos_prev(NA, -1), pos(NA, -1);
     for (int k = 0; k <= n; k++) {
         uint64_t off = 0;
         for (size_t t = 0; t < NA; t++) {
             pos[t] = -1;
-            if (asum[t] > k || dmin[t] < 0 || dmin[t] > n - k) continue;
+            if (asum[t] > k || dmin[t] < 0 || dmin[t] > n_mask - k) continue;
             pos[t] = (int64_t)off;
             mk.nmax_free = std::max(mk.nmax_free, k - asum[t]);
             off += nstates(mk.m_free, k - asum[t]);

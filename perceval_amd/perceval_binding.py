@@ -81,7 +81,8 @@ def make_backends(backends_module, types=None) -> dict:
             cache = self._gpu_cache
             if key not in cache:
                 if self._masks_str is not None:
-                    cache[key] = _slos.fock_mask_select(istate.m, istate.n, self._masks_str, self._no_limit_modes)
+                    cache[key] = _slos.fock_mask_select(istate.m, istate.n, self._masks_str, self._no_limit_modes,
+                                                        self._mask_n)
                 else:
                     cache[key] = None
             return cache[key]
@@ -172,20 +173,21 @@ def make_backends(backends_module, types=None) -> dict:
             return complex(coefs[idx] * math.sqrt(output_state.prodnfact() / self._input_state.prodnfact()))
 
         def _mask_plan_args(self, istate):
-            """(condition rows, at-least modes) for a masked plan, or None when
-            the mask goes through selection from the full output layer (no
-            mode constrained, fewer than two modes free in every row, more
-            than 64 rows, a photon count outside 1..30 or another than the
-            mask's)."""
+            """(condition rows, at-least modes, mask photon count) for a
+            masked plan, or None when the mask goes through selection from the
+            full output layer (no mode constrained, fewer than two modes free
+            in every row, more than 64 rows, a photon count outside 1..30, or
+            a mask set for fewer photons than the input's)."""
             if self._masks_str is None or len(self._masks_str) > 64:
                 return None
             cond = _slos.mask_conditions(self._masks_str, istate.m)
             free = (cond < 0).all(axis=0)
             if free.sum() < 2 or free.all() or istate.n < 1 or istate.n > 30:
                 return None
-            if self._mask_n is not None and self._mask_n != istate.n:
+            mask_n = self._mask_n or istate.n
+            if mask_n < istate.n:
                 return None
-            return cond, tuple(int(q) for q in (self._no_limit_modes or ()))
+            return cond, tuple(int(q) for q in (self._no_limit_modes or ())), int(mask_n)
 
         def all_prob(self, input_state=None):
             if input_state is not None:
@@ -196,11 +198,11 @@ def make_backends(backends_module, types=None) -> dict:
                 # every layer restricted to the states that can still match
                 # some mask row (the masked FSArray of _slos.py:161-164), the
                 # matching outputs in FSArray order
-                cond, at_least = args
-                key = ("masked", istate, cond.tobytes(), at_least)
+                cond, at_least, mask_n = args
+                key = ("masked", istate, cond.tobytes(), at_least, mask_n)
                 if key not in self._results:
                     try:
-                        plan = _slos.SlosPlan(list(istate), mask_cond=cond, at_least=at_least)
+                        plan = _slos.SlosPlan(list(istate), mask_cond=cond, at_least=at_least, mask_n=mask_n)
                         self._results[key] = plan.run(self._umat, _slos.PROBS).cpu().numpy()
                     except _native.NativeError as e:
                         if "herald configurations" not in str(e) and "2^32" not in str(e):

@@ -22,12 +22,15 @@ PROBS = 1
 class SlosPlan:
     """mask_cond: optional conditions, one row per mask (-1 any count, c >= 0
     c photons; a 1-D row is one mask; see mask_conditions); at_least: the
-    modes whose digit is a floor (at least c photons) instead of a count.  A
-    masked plan computes only the layers' states that can still match some
-    row (the masked FSArray of _slos.py:161-164) and returns the matching
-    output states only, in FSArray order."""
+    modes whose digit is a floor (at least c photons) instead of a count;
+    mask_n: the photon count the masks were set for (None: the input's; else
+    at least the input's).  A masked plan computes only the layers' states
+    that can still match some row (the masked FSArray of _slos.py:161-164)
+    and returns the kept output states only, in FSArray order: those matching
+    a row, or with mask_n above the input's photon count, those that the
+    missing photons can still complete (FSMask.keeps)."""
 
-    def __init__(self, in_occ, mask_cond=None, at_least=None):
+    def __init__(self, in_occ, mask_cond=None, at_least=None, mask_n=None):
         lib = _native.load()
         self.in_occ = np.ascontiguousarray(np.asarray(list(in_occ), dtype=np.int32))
         self.m = int(self.in_occ.size)
@@ -36,8 +39,8 @@ class SlosPlan:
         self.at_least = None
         i32p = ctypes.POINTER(ctypes.c_int32)
         if mask_cond is None:
-            if at_least:
-                raise ValueError("at-least modes need mask conditions")
+            if at_least or mask_n:
+                raise ValueError("at-least modes and a mask photon count need mask conditions")
             _native.check(lib.pcvl_slos_plan_create(self.m, self.in_occ.ctypes.data_as(i32p), ctypes.byref(self._plan)),
                           "pcvl_slos_plan_create")
         else:
@@ -56,7 +59,7 @@ class SlosPlan:
             self.at_least = at
             _native.check(lib.pcvl_slos_plan_create_masks(self.m, self.in_occ.ctypes.data_as(i32p), cond.shape[0],
                                                           cond.ctypes.data_as(i32p), at.ctypes.data_as(i32p),
-                                                          ctypes.byref(self._plan)),
+                                                          int(mask_n or 0), ctypes.byref(self._plan)),
                           "pcvl_slos_plan_create_masks")
         self._lib = lib
         n, count, work = ctypes.c_int(), ctypes.c_int64(), ctypes.c_size_t()
@@ -245,9 +248,10 @@ def mask_char_count(ch: str) -> int:
     return c
 
 
-def fock_mask_select(m: int, n: int, masks, at_least_modes=None):
-    """Ranks (FSArray order) of the n-photon states matching the masks, as a
-    CUDA int64 tensor, selected on the GPU (pcvl_fock_mask_select)."""
+def fock_mask_select(m: int, n: int, masks, at_least_modes=None, mask_n=None):
+    """Ranks (FSArray order) of the n-photon states the masks keep (matching a
+    row; with mask_n > n, completable by the mask_n - n photons still to
+    come), as a CUDA int64 tensor, selected on the GPU (pcvl_fock_mask_select)."""
     import torch
     lib = _native.load()
     cond = mask_conditions(masks, m)
@@ -257,7 +261,8 @@ def fock_mask_select(m: int, n: int, masks, at_least_modes=None):
     total = fock_count(m, n)
     out = torch.empty(max(1, total), dtype=torch.int64, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
-    _native.check(lib.pcvl_fock_mask_select(m, n, cond.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cond.shape[0],
+    _native.check(lib.pcvl_fock_mask_select(m, n, int(mask_n or 0), cond.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            cond.shape[0],
                                             at.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _native.ptr(out),
                                             _native.ptr(cnt), _native.stream_handle()), "pcvl_fock_mask_select")
     return out[:int(cnt.item())]

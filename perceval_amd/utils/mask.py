@@ -49,3 +49,35 @@ class FSMask:
 
     def match(self, state, allow_partial: bool = False) -> bool:
         return any(self._match_one(c, state, allow_partial) for c in self.conditions)
+
+    def _missing(self, cond, state):
+        """Photons that state lacks for cond (None: an exact count exceeded)."""
+        d = 0
+        for i, c in enumerate(cond):
+            if c is None:
+                continue
+            v = state[i]
+            if i in self.at_least:
+                d += max(0, c - v)
+            elif v > c:
+                return None
+            else:
+                d += c - v
+        return d
+
+    def keeps(self, state) -> bool:
+        """Membership of the masked Fock space of state.n photons (the masked
+        FSArray a SLOS layer holds, perceval/backends/_slos.py:161-164): the
+        self.n - state.n photons still to come can complete some row (they
+        cover its missing photons, and a free or at-least mode takes the
+        rest).  At state.n >= self.n this is match(state)."""
+        spare = self.n - sum(state)
+        if spare <= 0:
+            return self.match(state)
+        for cond in self.conditions:
+            d = self._missing(cond, state)
+            if d is None or d > spare:
+                continue
+            if d == spare or any(c is None or i in self.at_least for i, c in enumerate(cond)):
+                return True
+        return False

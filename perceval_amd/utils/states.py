@@ -285,14 +285,15 @@ def fock_space(m: int, n: int) -> np.ndarray:
 
 def allstate_iterator(input_state, mask=None) -> Iterable:
     """All output states with the input's photon number(s), FSArray order
-    (perceval/utils/states.py:268-285)."""
+    (perceval/utils/states.py:268-285); with a mask, the states it keeps
+    (FSMask.keeps)."""
     m = input_state.m
     ns = input_state.n
     ns = [ns] if isinstance(ns, int) else list(ns)
     for n in ns:
         for row in fock_space(m, n):
             st = FockState(row)
-            if mask is None or mask.match(st):
+            if mask is None or mask.keeps(st):
                 yield st
 
 

@@ -310,20 +310,21 @@ static void emulate_masked(int m, int n, const std::vector<int32_t> &cond, int w
 // states some matching output extends), ordered by (constrained modes, free
 // modes) descending; the output order is the FSArray order of the final set.
 static void emulate_masks(int m, int n, const std::vector<std::vector<int32_t>> &rows,
-                          const std::vector<int32_t> &at_least, int w)
+                          const std::vector<int32_t> &at_least, int w, int n_mask = -1)
 {
+    if (n_mask < 0) n_mask = n;
     const int nrows = (int)rows.size();
     std::vector<int32_t> flat;
     for (const auto &r : rows) flat.insert(flat.end(), r.begin(), r.end());
     BlkMasked mk;
-    if (blk_masked_layers_ex(m, n, nrows, flat.data(), at_least.empty() ? nullptr : at_least.data(), &mk) != OK) {
+    if (blk_masked_layers_ex(m, n, n_mask, nrows, flat.data(), at_least.empty() ? nullptr : at_least.data(), &mk) != OK) {
         printf("masks failed %s\n", pcvl_last_error());
         exit(1);
     }
     char what[96];
-    snprintf(what, sizeof what, "masks m=%d n=%d rows=%d w=%d", m, n, nrows, w);
-    std::vector<std::vector<int>> fin;   // FSArray order
-    for (const auto &s : space(m, n)) {
+    snprintf(what, sizeof what, "masks m=%d n=%d n_mask=%d rows=%d w=%d", m, n, n_mask, nrows, w);
+    std::vector<std::vector<int>> full;   // the n_mask-photon states matching a row
+    for (const auto &s : space(m, n_mask)) {
         bool any = false;
         for (const auto &r : rows) {
             bool ok = true;
@@ -334,13 +335,12 @@ static void emulate_masks(int m, int n, const std::vector<std::vector<int32_t>> 
             }
             any = any || ok;
         }
-        if (any) fin.push_back(s);
+        if (any) full.push_back(s);
     }
-    CHECK(mk.empty == fin.empty(), "%s: empty flag", what);
-    if (mk.empty) return;
-    std::vector<std::set<std::vector<int>>> closure((size_t)n + 1);
-    closure[(size_t)n].insert(fin.begin(), fin.end());
-    for (int k = n; k >= 1; k--)
+    // every layer, the output layer included, is the closure of those under photon removal
+    std::vector<std::set<std::vector<int>>> closure((size_t)n_mask + 1);
+    closure[(size_t)n_mask].insert(full.begin(), full.end());
+    for (int k = n_mask; k >= 1; k--)
         for (const auto &t : closure[(size_t)k])
             for (int j = 0; j < m; j++)
                 if (t[(size_t)j] > 0) {
@@ -348,6 +348,11 @@ static void emulate_masks(int m, int n, const std::vector<std::vector<int32_t>> 
                     u[(size_t)j]--;
                     closure[(size_t)k - 1].insert(u);
                 }
+    std::vector<std::vector<int>> fin;   // the kept outputs, FSArray order
+    for (const auto &s : space(m, n))
+        if (closure[(size_t)n].count(s)) fin.push_back(s);
+    CHECK(mk.empty == fin.empty(), "%s: empty flag", what);
+    if (mk.empty) return;
     const std::vector<int32_t> perm = mk.rowmap;
     std::vector<std::vector<std::vector<int>>> layers;
     for (int k = 0; k <= n; k++) {
@@ -447,11 +452,26 @@ int main()
     emulate_masks(14, 6, {{1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 0}, {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 1}},
                   {}, 8);
     cases++;
+    // masks set for more photons than the input holds (a part of a separated
+    // input): the output keeps the states that the missing photons complete
+    for (const auto &mc : general) {
+        int nfree = 0;
+        for (int j = 0; j < mc.m; j++) {
+            bool fr = true;
+            for (const auto &r : mc.rows) fr = fr && r[(size_t)j] < 0;
+            nfree += fr;
+        }
+        for (int n = 1; n <= 4; n++)
+            for (int extra = 1; extra <= 3; extra++) {
+                emulate_masks(mc.m, n, mc.rows, mc.at, std::min(3, nfree - 1), n + extra);
+                cases++;
+            }
+    }
     {   // floors of 0 on 24 modes: C(34, 10) configurations, refused after the cap, not enumerated
         std::vector<int32_t> row(26, 0), at(26, 1);
         row[0] = row[1] = -1;
         BlkMasked mk;
-        CHECK(blk_masked_layers_ex(26, 10, 1, row.data(), at.data(), &mk) == ERR_RANGE, "configuration cap");
+        CHECK(blk_masked_layers_ex(26, 10, 10, 1, row.data(), at.data(), &mk) == ERR_RANGE, "configuration cap");
         cases++;
     }
     printf("%d cases, %d failures\n", cases, fails);

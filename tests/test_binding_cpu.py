@@ -38,14 +38,16 @@ def _forbid(name):
 
 class _FakePlan:
     """SlosPlan on the oracle: coefficients and probabilities in FSArray order
-    (with mask conditions: the states that match some condition row, a row
+    (with mask conditions: the states that some condition row keeps, a row
     asking exactly c_j photons of every mode whose condition is c_j >= 0, at
-    least c_j on the at-least modes, in FSArray order)."""
+    least c_j on the at-least modes, in FSArray order; with mask_n above the
+    photon count, the photons missing from a row at most mask_n - n)."""
 
-    def __init__(self, in_occ, mask_cond=None, at_least=None):
+    def __init__(self, in_occ, mask_cond=None, at_least=None, mask_n=None):
         self.in_occ = list(in_occ)
         self.mask_cond = None if mask_cond is None else np.atleast_2d(np.asarray(mask_cond))
         self.at_least = list(at_least or [])
+        self.spare = max(0, (mask_n or 0) - sum(self.in_occ))
         _FakePlan.masked_runs += self.mask_cond is not None
 
     masked_runs = 0
@@ -59,8 +61,9 @@ class _FakePlan:
             floor[self.at_least] = True
             keep = np.zeros(len(states), dtype=bool)
             for row in self.mask_cond:
-                ok = (row < 0) | np.where(floor, states >= row, states == row)
-                keep |= np.all(ok, axis=1)
+                exceeded = np.any((row >= 0) & ~floor & (states > row), axis=1)
+                missing = np.where(row >= 0, np.maximum(row - states, 0), 0).sum(axis=1)
+                keep |= ~exceeded & (missing <= self.spare)
             r = r[keep]
         return torch.from_numpy(r)
 
@@ -98,10 +101,10 @@ class _FakeSingle:
         return oracle.perm(np.asarray(M))
 
 
-def _fake_mask_select(m, n, masks, at_least=None):
-    fm = FSMask(m, n, masks, at_least)
+def _fake_mask_select(m, n, masks, at_least=None, mask_n=None):
+    fm = FSMask(m, mask_n or n, masks, at_least)
     states = oracle.fock_enumerate(m, n)
-    return torch.tensor([i for i, s in enumerate(states) if fm.match(list(s))], dtype=torch.int64)
+    return torch.tensor([i for i, s in enumerate(states) if fm.keeps(list(s))], dtype=torch.int64)
 
 
 @pytest.fixture
@@ -248,3 +251,25 @@ def test_bound_slos_runs_heralds_as_a_masked_plan(bound):
     before = _FakePlan.masked_runs
     test_bound_backends_reproduce_reference_values(bound, case, "SLOS")
     assert _FakePlan.masked_runs > before
+
+
+def test_bound_slos_mask_for_the_whole_separated_state(bound):
+    """set_mask(heralds, n) with n the photons of the whole separated input
+    (perceval/simulators/simulator.py:412-470) and a 2-photon part: the bound
+    SLOS returns, through a masked plan, the states the other two photons can
+    complete to the heralds 1, 1 on modes 4, 5 (at most one photon in each),
+    with the oracle's probabilities."""
+    from perceval_amd.components import Unitary
+    from perceval_amd.utils.workloads import haar_unitary
+    b = bound.BACKEND_LIST["SLOS"]()
+    U = haar_unitary(6, seed=6)
+    b.set_circuit(Unitary(U))
+    b.set_mask("    11", 4)
+    b.set_input_state(amd_states.BasicState([1, 1, 0, 0, 0, 0]))
+    before = _FakePlan.masked_runs
+    probs = np.asarray(b.all_prob())
+    assert _FakePlan.masked_runs > before
+    states = oracle.fock_enumerate(6, 2)
+    keep = (states[:, 4] <= 1) & (states[:, 5] <= 1)
+    assert [list(s) for s in b._get_iterator(b._input_state)] == states[keep].tolist()
+    assert np.max(np.abs(probs - oracle.slos_probs(U, [1, 1, 0, 0, 0, 0])[keep])) < 1e-14

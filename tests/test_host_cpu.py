@@ -57,6 +57,24 @@ def test_mask_semantics():
     assert not mask.match(BasicState("|0,0,1,1,1,1>"))
 
 
+@pytest.mark.parametrize("masks,at_least,m,n", [
+    (["  01 1"], [], 6, 4), (["1 2   ", "0 0  1"], [0, 2], 6, 5), (["   011", "   110"], [], 6, 4),
+    (["1111"], [], 4, 5), (["21  "], [1], 4, 5), (["0101"], [0, 2], 4, 3)])
+def test_mask_keeps_is_the_closure_of_the_matches(masks, at_least, m, n):
+    """FSMask.keeps on k-photon states (mask set for n photons) is the set of
+    states that some matching n-photon state contains: the matches
+    (tests/utils/test_mask.py:33-45), closed under photon removal, layer by
+    layer down to the vacuum.  Includes a mask no n-photon state matches
+    (4 exact counts summing to 4 at n = 5): nothing is kept at any k."""
+    from perceval_amd.utils.states import fock_space
+    fm = FSMask(m, n, masks, at_least)
+    layer = {tuple(r) for r in fock_space(m, n).tolist() if fm.match(list(r))}
+    for k in range(n, -1, -1):
+        got = {tuple(r) for r in fock_space(m, k).tolist() if fm.keeps(list(r))}
+        assert got == layer, (masks, k)
+        layer = {s[:j] + (s[j] - 1,) + s[j + 1:] for s in layer for j in range(m) if s[j] > 0}
+
+
 def test_mask_counts_above_nine():
     """Herald masks write a count c as chr(0x30 + c), up to 32 photons
     (perceval/simulators/simulator.py:650-662): ':' is 10, 'P' is 32."""
@@ -172,9 +190,13 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
 
     cond = (ctypes.c_int32 * 4)(-1, 0, 1, -1)
     out = ctypes.c_void_p(1)
-    assert "masks" in err(lib.pcvl_fock_mask_select(4, 2, cond, 0, None, out, out, None))
-    assert "mode count" in err(lib.pcvl_fock_mask_select(65, 2, cond, 1, None, out, out, None))
-    assert "photon count" in err(lib.pcvl_fock_mask_select(4, 64, cond, 1, None, out, out, None))
+    assert "masks" in err(lib.pcvl_fock_mask_select(4, 2, 0, cond, 0, None, out, out, None))
+    assert "mode count" in err(lib.pcvl_fock_mask_select(65, 2, 0, cond, 1, None, out, out, None))
+    assert "photon count" in err(lib.pcvl_fock_mask_select(4, 64, 0, cond, 1, None, out, out, None))
+    assert "mask photon count" in err(lib.pcvl_fock_mask_select(4, 2, -1, cond, 1, None, out, out, None))
+    occ = (ctypes.c_int32 * 4)(1, 1, 0, 0)
+    plan = ctypes.c_void_p()
+    assert "mask photon count" in err(lib.pcvl_slos_plan_create_masks(4, occ, 1, cond, None, 1, ctypes.byref(plan)))
     n = (ctypes.c_int32 * 1)(49)
     offs = (ctypes.c_int64 * 1)(0)
     assert "order" in err(lib.pcvl_gurvits_batch(out, n, offs, 1, 100, 1, 0, out, out, None))

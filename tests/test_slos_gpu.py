@@ -332,6 +332,89 @@ def test_fock_mask_select_matches_host_fsmask(gpu, m, n, masks, at_least):
     assert np.array_equal(got, ref)
 
 
+def _closure_keep(m, k, rows, at_least, mask_n):
+    """Flags over the k-photon states (FSArray order) of those contained in
+    some mask_n-photon state that matches a row: the matches, closed under
+    photon removal mask_n - k times."""
+    full = oracle.fock_enumerate(m, mask_n)
+    layer = {tuple(s) for s in full[_rows_keep(full, rows, at_least)].tolist()}
+    for _ in range(mask_n - k):
+        layer = {s[:j] + (s[j] - 1,) + s[j + 1:] for s in layer for j in range(m) if s[j] > 0}
+    return np.array([tuple(s) in layer for s in oracle.fock_enumerate(m, k).tolist()], dtype=bool)
+
+
+@pytest.mark.parametrize("m,k,masks,at_least,mask_n", [
+    (6, 2, ["  0  1"], [], 4),
+    (6, 3, ["1     ", "  2   "], [], 5),
+    (6, 2, ["   1 1"], [3], 5),
+    (5, 2, ["11111"], [], 5),                    # every mode exact: the surplus has nowhere to go
+    (5, 2, ["11111"], [], 6),                    # no 6-photon state matches: nothing kept
+    (8, 4, ["1 1 1 1 ", "  00  0 ", " 2      "], [0, 2], 6),
+])
+def test_fock_mask_select_parts_of_a_larger_state(gpu, m, k, masks, at_least, mask_n):
+    """Masks set for mask_n photons (set_mask's n, the photons of the whole
+    separated input), applied to a k-photon part: the kept states are those
+    that the other mask_n - k photons can complete to a match."""
+    from perceval_amd.slos import fock_mask_select, mask_conditions
+    ref = np.flatnonzero(_closure_keep(m, k, mask_conditions(masks, m), at_least, mask_n))
+    got = fock_mask_select(m, k, masks, at_least, mask_n).cpu().numpy()
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("inp,rows,at_least,mask_n", [
+    ([1, 1, 0, 1, 0, 0], [[-1, -1, -1, 0, 1, 1]], [], 5),
+    ([1, 0, 1, 1, 0, 0, 1], [[1, -1, -1, 2, -1, -1, -1], [0, -1, -1, 0, -1, -1, 1]], [0, 3], 6),
+    ([2, 0, 1, 0, 0, 0, 0, 1], [[0, 1, -1, -1, -1, -1, -1, 1], [2, 2, -1, -1, -1, -1, -1, 0]], [7], 7),
+    ([1] * 6 + [0] * 6, [[-1] * 10 + [1, 1]], [], 7),
+])
+def test_slos_masks_plan_parts_of_a_larger_state(gpu, inp, rows, at_least, mask_n):
+    """A masked plan for masks set for more photons than the input holds:
+    its outputs are the states the missing photons can complete, each within
+    1e-12 of the full plan's value, every block width."""
+    from perceval_amd.slos import COEFS, PROBS, SlosPlan
+    m = len(inp)
+    U = haar_unitary(m, seed=3 * m + mask_n)
+    keep = _closure_keep(m, sum(inp), np.asarray(rows), at_least, mask_n)
+    full = SlosPlan(inp)
+    pf = full.run(U, PROBS).cpu().numpy()[keep]
+    cf = full.run(U, COEFS).cpu().numpy()[keep]
+    plan = SlosPlan(inp, mask_cond=rows, at_least=at_least, mask_n=mask_n)
+    assert plan.count == int(keep.sum()) > 0
+    free = int((np.asarray(rows) < 0).all(axis=0).sum())
+    for w in range(1, min(free - 1, 16) + 1):
+        plan.set_block_width(w)
+        assert np.max(np.abs(plan.run(U, PROBS).cpu().numpy() - pf)) < ABS_TOL, w
+        assert np.max(np.abs(plan.run(U, COEFS).cpu().numpy() - cf)) < ABS_TOL, w
+    plan.close()
+    full.close()
+
+
+def test_slos_backend_mask_for_the_whole_separated_state(gpu):
+    """set_mask(heralds, n) with n the photons of the whole input and a part
+    holding fewer (the reference simulator's separated inputs,
+    perceval/simulators/simulator.py:412-470): all_prob returns the states the
+    other photons can complete, through a masked plan, and matches the
+    selection path on the full output layer."""
+    from perceval_amd.backends import SLOSBackend
+    from perceval_amd.components import Unitary
+    from perceval_amd.slos import mask_conditions
+    from perceval_amd.utils.states import FockState
+    U = haar_unitary(8, seed=81)
+    part = FockState([1, 0, 1, 0, 0, 0, 0, 0])
+    b = SLOSBackend()
+    b.set_circuit(Unitary(U))
+    b.set_input_state(part)
+    full = np.asarray(b.all_prob())
+    b.set_mask("      11", 4)
+    b.set_input_state(part)
+    got = np.asarray(b.all_prob())
+    assert b._plans_masked and all(p is not None for p in b._plans_masked.values())
+    keep = _closure_keep(8, 2, mask_conditions("      11", 8), [], 4)
+    assert got.shape == (int(keep.sum()),) and np.max(np.abs(got - full[keep])) < ABS_TOL
+    states = list(b._get_iterator(part))
+    assert [list(s) for s in states] == oracle.fock_enumerate(8, 2)[keep].tolist()
+
+
 def test_fock_mask_select_m24_n10_heralds(gpu):
     """Heralds: modes 20..23 hold exactly 0, 1, 0, 1 photons; the kept states
     are the 8-photon states of the other 20 modes, C(27, 8) of them."""
