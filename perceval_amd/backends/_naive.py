@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from ..permanent import PermanentPlan
+from ..permanent import BatchPermanents, output_submatrices
 from ..utils.states import BSDistribution, FockState, StateVector
 from ._abstract_backends import AStrongSimulationBackend
 
@@ -55,25 +55,15 @@ class NaiveBackend(AStrongSimulationBackend):
         return self._compute_permanent(M) / math.sqrt(p) if M.size > 1 else complex(M[0, 0])
 
     def _all_amplitudes(self) -> np.ndarray:
-        """Amplitudes of every state of the iterator, one batched GPU launch."""
-        import torch
+        """Amplitudes of every state of the iterator, one batched GPU launch
+        (the batch plan kept per size)."""
         states = self._get_iterator(self._input_state)
-        n = self._input_state.n
-        if n <= 1:
+        if self._input_state.n <= 1 or not states:
             return np.array([self.prob_amplitude(s) for s in states], dtype=complex)
-        cols = _photon_modes(self._input_state)
-        mats = np.empty((len(states), n, n), dtype=complex)
-        norm = np.empty(len(states))
-        pin = self._input_state.prodnfact()
-        for k, s in enumerate(states):
-            mats[k] = self._umat[np.ix_(_photon_modes(s), cols)]
-            norm[k] = math.sqrt(s.prodnfact() * pin)
-        plan = PermanentPlan(np.full(len(states), n, dtype=np.int32))
-        try:
-            out = plan(torch.from_numpy(mats.reshape(-1)).cuda()).cpu().numpy()
-        finally:
-            plan.close()
-        return out / norm
+        mats, norms = output_submatrices(self._umat, [tuple(s) for s in states], tuple(self._input_state))
+        if getattr(self, "_batch", None) is None:
+            self._batch = BatchPermanents()
+        return self._batch(mats) / norms
 
     def all_prob(self, input_state=None) -> list:
         if input_state is not None:

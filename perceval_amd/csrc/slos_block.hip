@@ -46,9 +46,31 @@ struct BlkTabOffsets {
     uint32_t te, cte, f32, fd, bytes;
 };
 
+// C(a, b) for a < 128 by Pascal's rule, saturated at UINT64_MAX, built once
+// (the host builders look binomials up millions of times).
+struct BlkBinomTable {
+    static constexpr int A = 128;
+    uint64_t c[A][A];
+    BlkBinomTable()
+    {
+        for (int a = 0; a < A; a++) {
+            c[a][0] = 1;
+            for (int b = 1; b < A; b++) {
+                if (b > a) { c[a][b] = 0; continue; }
+                const uint64_t x = c[a - 1][b - 1], y = (b <= a - 1) ? c[a - 1][b] : 0;
+                c[a][b] = (x > UINT64_MAX - y) ? UINT64_MAX : x + y;
+            }
+        }
+    }
+};
+
 static uint64_t blk_binom(int64_t a, int64_t b)
 {
     if (b < 0 || a < 0 || b > a) return 0;
+    if (a < BlkBinomTable::A) {
+        static const BlkBinomTable table;   // thread-safe one-time construction
+        return table.c[a][b];
+    }
     if (b > a - b) b = a - b;
     unsigned __int128 r = 1;
     for (int64_t i = 1; i <= b; i++) {
@@ -467,20 +489,25 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
         const size_t p0 = spf.size(), s0 = sslot.size();
         spf.resize(p0 + S);
         sslot.resize(s0 + S * (size_t)ns, 0xffffffffu);
-        std::vector<int> beta((size_t)w, 0);
+        std::vector<int> beta((size_t)w, 0), after((size_t)w + 1, 0);
         beta[0] = r;
         for (uint64_t i = 0; i < S; i++) {
             double pf = 1.0;
             int d = 0;
+            // beta has rank i; taking a photon from mode j lowers the photons
+            // after each mode q < j by one, which lowers the rank by
+            // sum_{q<j} C(after_q + w - q - 3, w - q - 2) (Pascal's rule on
+            // each term of the rank sum)
+            for (int q = w - 1; q >= 0; q--) after[(size_t)q] = after[(size_t)q + 1] + beta[(size_t)q];
+            uint64_t drop = 0;
             for (int j = 0; j < w; j++) {
                 for (int v = 2; v <= beta[(size_t)j]; v++) pf *= v;
                 if (beta[(size_t)j] > 0) {
-                    beta[(size_t)j]--;
-                    const uint64_t lp = local_rank(beta);
-                    beta[(size_t)j]++;
+                    const uint64_t lp = i - drop;
                     sslot[s0 + (size_t)d * S + i] = (uint32_t)(lp << 5) | (uint32_t)j;
                     d++;
                 }
+                if (j + 2 <= w) drop += blk_binom((int64_t)after[(size_t)j + 1] + w - j - 3, w - j - 2);
             }
             spf[p0 + i] = pf;
             // next state in descending lexicographic order

@@ -41,10 +41,6 @@ from . import slos as _slos
 from .utils._random import next_sampler_seed
 
 
-def _photon_modes(occ) -> list:
-    return [i for i, v in enumerate(occ) for _ in range(int(v))]
-
-
 def _rank(state) -> int:
     """FSArray rank of an occupation vector (descending lexicographic order):
     sum over q < m-1 of C(R_{q+1} + m-q-2, m-q-1), R_q = photons in modes >= q."""
@@ -240,6 +236,7 @@ def make_backends(backends_module, types=None) -> dict:
         def __init__(self, *args, **kwargs):
             self._gpu_cache = {}
             self._single = _permanent.SinglePermanents()
+            self._batch = _permanent.BatchPermanents()
             NaiveBase.__init__(self, *args, **kwargs)
 
         def _compute_permanent(self, M):
@@ -254,25 +251,12 @@ def make_backends(backends_module, types=None) -> dict:
             self._umat = np.asarray(self._umat, dtype=np.complex128)
 
         def _all_amplitudes(self):
-            import torch
             istate = self._input_state
             states = self._get_iterator(istate)
-            n = istate.n
-            if n <= 1 or not states:
+            if istate.n <= 1 or not states:
                 return np.array([self.prob_amplitude(s) for s in states], dtype=complex)
-            cols = _photon_modes(istate)
-            mats = np.empty((len(states), n, n), dtype=complex)
-            norm = np.empty(len(states))
-            pin = istate.prodnfact()
-            for k, s in enumerate(states):
-                mats[k] = self._umat[np.ix_(_photon_modes(s), cols)]
-                norm[k] = math.sqrt(s.prodnfact() * pin)
-            plan = _permanent.PermanentPlan(np.full(len(states), n, dtype=np.int32))
-            try:
-                out = plan(torch.from_numpy(mats.reshape(-1)).cuda()).cpu().numpy()
-            finally:
-                plan.close()
-            return out / norm
+            mats, norms = _permanent.output_submatrices(self._umat, [list(s) for s in states], list(istate))
+            return self._batch(mats) / norms
 
         def all_prob(self, input_state=None):
             if input_state is not None:

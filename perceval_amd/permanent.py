@@ -128,6 +128,47 @@ class SinglePermanents:
         self._by_order = {}
 
 
+def output_submatrices(U, occ, in_occ):
+    """Naive submatrices of many output states (_naive.py:51-68): mats[k] =
+    U[photon modes of occ[k], photon modes of in_occ], both ascending with
+    repetition, and norms[k] = sqrt(prod occ[k]! prod in_occ!).  occ is
+    (N, m) with n photons per row; mats is (N, n, n) complex128."""
+    occ = np.asarray(occ, dtype=np.int64)
+    in_occ = np.asarray(in_occ, dtype=np.int64)
+    N, m = occ.shape
+    n = int(in_occ.sum())
+    # every row holds n photons, so repeating the mode indices by the flat
+    # counts lists each state's photon modes in order, n per state
+    rows = np.repeat(np.tile(np.arange(m), N), occ.reshape(-1)).reshape(N, n)
+    cols = np.repeat(np.arange(in_occ.size), in_occ)
+    mats = np.ascontiguousarray(np.asarray(U, dtype=np.complex128)[rows[:, :, None], cols[None, None, :]])
+    fact = np.cumprod(np.concatenate([[1.0], np.arange(1, max(n, 1) + 1, dtype=float)]))
+    norms = np.sqrt(np.prod(fact[occ], axis=1) * np.prod(fact[in_occ]))
+    return mats, norms
+
+
+class BatchPermanents:
+    """Permanents of an (N, n, n) batch in one launch, the plan kept per
+    (N, n): repeated all-output calls of the Naive backend allocate no plan."""
+
+    def __init__(self):
+        self._plans = {}
+
+    def __call__(self, mats) -> np.ndarray:
+        import torch
+        mats = np.ascontiguousarray(np.asarray(mats, dtype=np.complex128))
+        N, n = mats.shape[0], mats.shape[1]
+        key = (N, n)
+        if key not in self._plans:
+            self._plans[key] = PermanentPlan(np.full(N, n, dtype=np.int32))
+        return self._plans[key](torch.from_numpy(mats.reshape(-1)).cuda()).cpu().numpy()
+
+    def close(self):
+        for plan in self._plans.values():
+            plan.close()
+        self._plans = {}
+
+
 _single = None
 
 

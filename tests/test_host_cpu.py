@@ -216,3 +216,21 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     assert "workspace" in err(lib.pcvl_mlp_wgrad(out, 3, out, 3, 100, 3, 3, out, out, out, 0, None))
     assert "probe" in err(lib.pcvl_probe_fp64_fma(0, 10, out, ctypes.byref(f), None))
     assert "1 <= n <= 28" in err(lib.pcvl_mis_permanents(out, 10, out, 29, out, 5, out, None))
+
+
+def test_output_submatrices_match_the_per_state_construction():
+    """The batched gather of the Naive submatrices (rows: output photons'
+    modes, columns: input photons' modes, ascending with repetition,
+    _naive.py:51-68) equals np.ix_ per state, norms sqrt(prod s! prod t!)."""
+    from perceval_amd.permanent import output_submatrices
+    from perceval_amd.utils.states import fock_space
+    rng = np.random.default_rng(5)
+    U = rng.normal(size=(5, 5)) + 1j * rng.normal(size=(5, 5))
+    inp = [2, 0, 1, 0, 1]
+    occ = fock_space(5, 4)
+    mats, norms = output_submatrices(U, occ, inp)
+    cols = [0, 0, 2, 4]
+    for k, s in enumerate(occ.tolist()):
+        rows = [i for i, v in enumerate(s) for _ in range(v)]
+        assert np.array_equal(mats[k], U[np.ix_(rows, cols)])
+        assert norms[k] == math.sqrt(np.prod([math.factorial(v) for v in s]) * 2)
