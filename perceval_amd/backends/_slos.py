@@ -43,6 +43,28 @@ class _Layer:
         self.count = math.comb(m + k - 1, k)
 
 
+class _LazyPlan:
+    """A single-input SlosPlan built on first use.  Under a mask all_prob
+    usually runs a masked plan, and the full plan's tables (tens of ms of
+    host work and tens of MB of device memory at m=24, n=10) are then never
+    built."""
+
+    def __init__(self, occ):
+        self._occ = list(occ)
+        self._plan = None
+
+    def _get(self):
+        if self._plan is None:
+            self._plan = SlosPlan(self._occ)
+        return self._plan
+
+    def run(self, *args, **kwargs):
+        return self._get().run(*args, **kwargs)
+
+    def __getattr__(self, name):
+        return getattr(self._get(), name)
+
+
 class SLOSBackend(AStrongSimulationBackend):
     def __init__(self, mask=None, use_symbolic=False):
         super().__init__()
@@ -102,7 +124,7 @@ class SLOSBackend(AStrongSimulationBackend):
             return False
         self._deploy(new)
         if len(new) == 1:
-            self._state_mapping[new[0]] = SlosPlan(list(new[0]))
+            self._state_mapping[new[0]] = SlosPlan(list(new[0])) if self._masks_str is None else _LazyPlan(new[0])
         else:
             tree = SlosTree([list(s) for s in new])
             for i, s in enumerate(new):

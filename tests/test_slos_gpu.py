@@ -409,6 +409,7 @@ def test_slos_backend_mask_for_the_whole_separated_state(gpu):
     b.set_input_state(part)
     got = np.asarray(b.all_prob())
     assert b._plans_masked and all(p is not None for p in b._plans_masked.values())
+    assert b._state_mapping[part]._plan is None        # the full plan was never built
     keep = _closure_keep(8, 2, mask_conditions("      11", 8), [], 4)
     assert got.shape == (int(keep.sum()),) and np.max(np.abs(got - full[keep])) < ABS_TOL
     states = list(b._get_iterator(part))
