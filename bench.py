@@ -10,7 +10,8 @@ configurations are reported beside it:
 
   mixed_n12_24  4096 permanents, n uniform in 12..24, one launch
   slos_m24_n10  SLOS all_prob, Haar 24x24, 10 photons, 92,561,040 outputs
-                (achieved algorithmic HBM GB/s)
+                (achieved algorithmic HBM GB/s); masked plans on the same input;
+                m = 32, n = 10 (1,121,099,408 outputs)
   slos_tree     37 inputs (an 8-photon state and its 1- and 2-photon-loss
                 variants, m = 20) through one shared SLOS tree vs one plan each
   gradient      adjoint phase gradients, Clements 12 modes (132 phases),
@@ -454,6 +455,23 @@ def bench_slos(args, res):
         masked[name] = {"ms": t.ms / 10, "outputs": mplan.count}
         mplan.close()
     res["slos_m24_n10_masked"] = masked
+    # a larger space: m=32, n=10, 1,121,099,408 outputs (9.0 GB) from a 273M-state parent layer
+    U32 = torch.from_numpy(haar_unitary(32, seed=32)).cuda()
+    big = SlosPlan([1] * 10 + [0] * 22)
+    bout = torch.empty(big.count, dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        big.run(U32, PROBS, bout)
+    with Timer() as t:
+        for _ in range(5):
+            big.run(U32, PROBS, bout)
+    big_ms = t.ms / 5
+    big_gb = slos_bytes(32, 10) / 1e9
+    res["slos_m32_n10"] = {"ms": big_ms, "outputs": big.count, "algorithmic_GB": big_gb,
+                           "GB_per_s": big_gb / (big_ms * 1e-3), "block_width": big.block_width}
+    log(f"[bench] SLOS m=32 n=10: {big.count} outputs in {big_ms:.2f} ms, {big_gb / (big_ms * 1e-3):.0f} GB/s algorithmic")
+    del bout
+    big.close()
+    torch.cuda.empty_cache()
     log("[bench] SLOS m=24 n=10 masked: " + ", ".join(f"{k} {v['outputs']} outputs {v['ms']:.3f} ms"
                                                        for k, v in masked.items()))
     gbytes = slos_bytes(m, n) / 1e9

@@ -221,8 +221,9 @@ int pcvl_slos_tree_destroy(pcvl_slos_tree plan);
  * Kernels not applicable to the plan give -1.  For comparisons. */
 int pcvl_slos_plan_set_kernel(pcvl_slos_plan plan, int kernel);
 
-/* Suffix width w of kernel 8 (0: the default, the widest w <= 12 below m
- * whose tables stay under 32 MiB; else 1 <= w <= min(m - 1, 16)).  Rebuilds
+/* Suffix width w of kernel 8 (0: the default, min(16, m - 1, max(12, m / 2))
+ * narrowed until its tables stay under 512 MiB; else 1 <= w <= min(m - 1,
+ * 16)).  Rebuilds
  * the plan's block tables (synchronises the device).  A tuning knob: the
  * results do not depend on w.  No counterpart in the reference. */
 int pcvl_slos_plan_set_block_width(pcvl_slos_plan plan, int w);

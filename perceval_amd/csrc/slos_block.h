@@ -97,24 +97,28 @@ struct BlkLayout {
     size_t bytes = 0;
 };
 
-// Suffix width used when the caller does not choose one (0): the widest
-// w <= 12 below m whose suffix tables stay under 32 MiB.
+// Suffix width used when the caller does not choose one (0): min(16, m - 1,
+// max(12, m / 2)), narrowed until the suffix tables stay under 512 MiB.
 int blk_default_width(int m, int nmax);
 
 // The layout of layers 1..n of the full space F(m, k), suffix width w
 // (1 <= w < m), with its device tables.  Every layer must hold fewer than
 // 2^32 states.
 int blk_layout_create(int m, int nmax, int w, BlkLayout *out);
+// Block records kept by default: 256 MiB of 128-byte records.
+constexpr size_t BLK_MAX_RECORDS = (256ull << 20) / 128;
 // The same without the device upload: fills `image` with the bytes that
 // blk_layout_create copies to the device (offsets in the layout).
-int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<char> *image);
+int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<char> *image,
+                          size_t max_records = BLK_MAX_RECORDS);
 // General form: m_free free modes holding at most nmax_free photons, the
 // sub-layers of each layer (subs[k], k = 1..subs.size()-1, subs[0] unused),
 // their state counts (counts[k], k = 0..), and the U row of each local mode
-// (rowmap: free modes, then herald modes).
+// (rowmap: free modes, then herald modes).  Block records take at most
+// max_records entries; the groups past that walk per lane.
 int blk_layout_build_host_ex(int m_free, int nmax_free, int w, const std::vector<std::vector<BlkSub>> &subs,
                              const std::vector<uint64_t> &counts, const std::vector<int32_t> &rowmap,
-                             BlkLayout *out, std::vector<char> *image);
+                             BlkLayout *out, std::vector<char> *image, size_t max_records = BLK_MAX_RECORDS);
 // The masked layers of an n-photon plan over m modes for a mask of `nrows`
 // condition rows (FSMask, perceval/backends/_abstract_backends.py:103-146):
 // conds[r * m + j] >= 0 asks for that many photons in mode j (at least that

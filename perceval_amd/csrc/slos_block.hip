@@ -417,8 +417,10 @@ static uint64_t suffix_table_words(int w, int nmax)
 
 int blk_default_width(int m, int nmax)
 {
-    int w = std::min(12, m - 1);
-    while (w > 1 && suffix_table_words(w, nmax) * 4 > (32u << 20)) w--;
+    // measured on MI355X at n = 10: m = 24 is fastest at w = 12 (1.46 ms
+    // against 1.51-1.77 at 13-16), m = 32 at w = 16 (21.5 ms against 23.2 at 12)
+    int w = std::min({16, m - 1, std::max(12, m / 2)});
+    while (w > 1 && suffix_table_words(w, nmax) * 4 > (512ull << 20)) w--;
     return std::max(w, 1);
 }
 
@@ -438,8 +440,9 @@ static uint64_t local_rank(const std::vector<int> &beta)
 
 int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vector<BlkSub>> &subs,
                              const std::vector<uint64_t> &counts, const std::vector<int32_t> &rowmap,
-                             BlkLayout *out, std::vector<char> *image)
+                             BlkLayout *out, std::vector<char> *image, size_t max_records)
 {
+    static_assert(sizeof(BlkRec) == 128, "BLK_MAX_RECORDS counts 128-byte records");
     if (m < 2 || nmax < 0 || nmax > 30) { set_error("block SLOS layout: needs 2 or more free modes and at most 30 photons"); return ERR_ARG; }
     if (w < 1 || w >= m || w > BLK_WMAX) { set_error("block SLOS layout: suffix width outside [1, min(m-1, 16)]"); return ERR_ARG; }
     if (subs.empty() || counts.size() < subs.size()) { set_error("block SLOS layout: inconsistent layers"); return ERR_ARG; }
@@ -586,21 +589,25 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
                 std::memcpy(&g.hf_lo, &sub.hfact, sizeof(double));   // hf_lo, hf_hi
                 g.nb = (uint32_t)nstates(h, sub.kf - r);
                 g.S = (uint32_t)nstates(w, r);
+                // a block's record depends on (K, r) only: sub-layers share
+                // them.  Records take at most max_records entries (256 MiB
+                // by default); groups past that walk per lane (regime B).
+                bool rec = false;
                 if (g.S >= BLK_A_MIN && use_records) {
-                    g.nch = (g.S + 63) / 64;
-                    g.nitems = g.nb * g.nch;
-                    // a block's record depends on (K, r) only: sub-layers share them
                     const auto key = std::make_pair(sub.kf - r, r);
                     auto rit = rec_of.find(key);
-                    if (rit == rec_of.end()) {
+                    if (rit == rec_of.end() && recs.size() + g.nb <= max_records) {
                         rit = rec_of.emplace(key, (uint32_t)recs.size()).first;
                         add_records(sub.kf - r, r);
                     }
-                    g.rec0 = rit->second;
-                    if (recs.size() > (256ull << 20) / sizeof(BlkRec)) {
-                        set_error("block SLOS layout: block records above 256 MiB (use a wider suffix)");
-                        return ERR_RANGE;
+                    if (rit != rec_of.end()) {
+                        rec = true;
+                        g.rec0 = rit->second;
                     }
+                }
+                if (rec) {
+                    g.nch = (g.S + 63) / 64;
+                    g.nitems = g.nb * g.nch;
                 } else {
                     g.nch = 0;
                     g.nitems = (uint32_t)(((uint64_t)g.nb * g.S + 63) / 64);
@@ -658,7 +665,7 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
     return OK;
 }
 
-int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<char> *image)
+int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<char> *image, size_t max_records)
 {
     if (!blk_layout_ok(m, nmax)) { set_error("block SLOS layout: layers must hold fewer than 2^32 states"); return ERR_ARG; }
     std::vector<std::vector<BlkSub>> subs((size_t)nmax + 1);
@@ -673,7 +680,7 @@ int blk_layout_build_host(int m, int nmax, int w, BlkLayout *out, std::vector<ch
     }
     std::vector<int32_t> rowmap((size_t)m);
     for (int i = 0; i < m; i++) rowmap[(size_t)i] = i;
-    return blk_layout_build_host_ex(m, nmax, w, subs, counts, rowmap, out, image);
+    return blk_layout_build_host_ex(m, nmax, w, subs, counts, rowmap, out, image, max_records);
 }
 
 // LDS of layer k's launch: the U column, then the per-thread lists of waves

@@ -241,17 +241,21 @@ static void replay(const BlkLayout &lay, const std::vector<char> &img, const std
     }
 }
 
-static void emulate(int m, int nmax, int w)
+static void emulate(int m, int nmax, int w, size_t max_records = BLK_MAX_RECORDS)
 {
     BlkLayout lay;
     std::vector<char> img;
-    if (blk_layout_build_host(m, nmax, w, &lay, &img) != OK) { printf("build failed %s\n", pcvl_last_error()); exit(1); }
+    if (blk_layout_build_host(m, nmax, w, &lay, &img, max_records) != OK) {
+        printf("build failed %s\n", pcvl_last_error());
+        exit(1);
+    }
     std::vector<std::vector<std::vector<int>>> layers;
     for (int k = 0; k <= nmax; k++) layers.push_back(space(m, k));
     std::vector<int32_t> perm((size_t)m);
     for (int j = 0; j < m; j++) perm[(size_t)j] = j;
-    char what[64];
-    snprintf(what, sizeof what, "m=%d n=%d w=%d", m, nmax, w);
+    char what[96];
+    snprintf(what, sizeof what, "m=%d n=%d w=%d records<=%zu", m, nmax, w, max_records);
+    CHECK(lay.nrec <= max_records, "%s: %lu records", what, (unsigned long)lay.nrec);
     replay(lay, img, layers, perm, what, 16);
     replay(lay, img, layers, perm, what, 2048);
 }
@@ -409,6 +413,11 @@ int main()
     emulate(20, 6, 12); cases++;
     emulate(3, 14, 2); cases++;
     emulate(8, 13, 7); cases++;
+    // a record budget that runs out part way: later groups walk per lane
+    for (size_t cap : {0, 40, 300, 2000}) {
+        emulate(12, 6, 4, cap); cases++;
+        emulate(10, 7, 3, cap); cases++;
+    }
     // masks: heralds anywhere among the modes, several counts, empty masks
     const std::vector<std::pair<int, std::vector<int32_t>>> masks = {
         {6, {-1, -1, -1, -1, 1, 1}}, {6, {1, -1, 0, -1, -1, -1}}, {7, {-1, 2, -1, -1, 0, -1, 1}},

@@ -228,6 +228,35 @@ def test_slos_m24_n10_full_size(gpu, kernel):
     plan.close()
 
 
+def test_slos_m32_n10_a_billion_outputs(gpu):
+    """m=32, n=10: C(41, 10) = 1,121,099,408 probabilities (9.0 GB) from a
+    273M-state parent layer (two 4.4 GB ping-pong buffers), the default
+    kernel: the probabilities sum to 1 and 64 of them, the first and last
+    included, equal |perm|^2 / prod s! from the oracle within 1e-12."""
+    import torch
+    from perceval_amd.slos import PROBS, SlosPlan
+    U = haar_unitary(32, seed=32)
+    inp = [1] * 10 + [0] * 22
+    plan = SlosPlan(inp)
+    assert plan.count == math.comb(41, 10)
+    _say(f"plan: {plan.count} outputs, workspace {plan.work_bytes / 2**30:.1f} GiB")
+    p = plan.run(U, PROBS)
+    torch.cuda.synchronize()
+    _say("all_prob done")
+    assert abs(float(p.sum().item()) - 1.0) < 1e-10
+    assert float(p.min().item()) >= 0.0
+    rng = np.random.RandomState(2)
+    idx = np.concatenate([[0, p.numel() - 1], rng.randint(0, p.numel(), 62)])
+    got = p[torch.from_numpy(idx).cuda()].cpu().numpy()
+    for i, g in zip(idx, got):
+        s = oracle.fock_unrank(int(i), 32, 10)
+        ref = abs(oracle.perm(oracle.submatrix(U, inp, s))) ** 2 / oracle.prodnfact(s)
+        assert abs(g - ref) < ABS_TOL, (i, s, g, ref)
+    del p
+    plan.close()
+    torch.cuda.empty_cache()
+
+
 def test_slos_vacuum_and_one_photon(gpu):
     from perceval_amd.slos import COEFS, PROBS, SlosPlan
     U = haar_unitary(5, seed=3)
