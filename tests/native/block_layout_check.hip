@@ -476,6 +476,31 @@ int main()
                 cases++;
             }
     }
+    {   // random masks: 2..4 free modes among 5..8, 1..3 rows, floors, counts 0..3, a mask
+        // photon count up to 2 above the input's (fixed seed)
+        uint64_t st = 0x9e3779b97f4a7c15ull;
+        auto rnd = [&](int lo, int hi) {   // uniform in [lo, hi]
+            st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+            return lo + (int)(st % (uint64_t)(hi - lo + 1));
+        };
+        for (int trial = 0; trial < 150; trial++) {
+            const int m = rnd(5, 8), nfree = rnd(2, 4), nrows = rnd(1, 3);
+            std::vector<int32_t> free_mode((size_t)m, 0), at((size_t)m, 0);
+            for (int f = 0; f < nfree; f++) {
+                int j = rnd(0, m - 1);
+                while (free_mode[(size_t)j]) j = (j + 1) % m;
+                free_mode[(size_t)j] = 1;
+            }
+            std::vector<std::vector<int32_t>> rows((size_t)nrows, std::vector<int32_t>((size_t)m, -1));
+            for (auto &r : rows)
+                for (int j = 0; j < m; j++)
+                    if (!free_mode[(size_t)j]) r[(size_t)j] = rnd(0, 4) == 0 ? -1 : rnd(0, 3);
+            for (int j = 0; j < m; j++) at[(size_t)j] = !free_mode[(size_t)j] && rnd(0, 3) == 0;
+            const int n = rnd(1, 5);
+            emulate_masks(m, n, rows, at, rnd(1, nfree - 1), n + rnd(0, 2));
+            cases++;
+        }
+    }
     {   // floors of 0 on 24 modes: C(34, 10) configurations, refused after the cap, not enumerated
         std::vector<int32_t> row(26, 0), at(26, 1);
         row[0] = row[1] = -1;
