@@ -236,7 +236,7 @@ int pcvl_slos_plan_block_width(pcvl_slos_plan plan);
  * Clifford 2017, count samples drawn in parallel.  Sample s uses the
  * counter-based random stream (seed, first_sample + s), so a call is fully
  * determined by its arguments.  d_out: count rows of m int32 occupation
- * counts.  At most 24 photons and 256 modes.
+ * counts.  At most 30 photons and 256 modes.
  * ------------------------------------------------------------------------- */
 
 /* Device workspace needed by pcvl_cc_samples for n photons and count samples. */

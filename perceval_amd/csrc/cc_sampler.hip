@@ -29,14 +29,18 @@
 
 namespace pcvl {
 
-constexpr int CC_MAX_N = 24;
+constexpr int CC_MAX_N = 30;
 constexpr int CC_MAX_M = 256;   // multiple of 64
 
+// Per-step shared state, sized for step K: the LDS a step takes (and with it
+// the occupancy the compiler plans its registers for) does not grow with the
+// largest photon count the library accepts.
+template <int K>
 struct CCShared {
-    cd B[(CC_MAX_N - 1) * CC_MAX_N];     // B[i*k + l] = 2 * U[r_i, cols[perm[l]]]
-    cd sub[CC_MAX_N];
-    int pc[CC_MAX_N];                    // cols[perm[l]]: input mode of permuted column l
-    int r[CC_MAX_N];
+    cd B[(K > 1 ? K - 1 : 1) * K];       // B[i*k + l] = 2 * U[r_i, cols[perm[l]]]
+    cd sub[K];
+    int pc[K];                           // cols[perm[l]]: input mode of permuted column l
+    int r[K];
 };
 
 // Partial sums of the k sub-permanents over this lane pair's chunk of
@@ -144,7 +148,7 @@ __device__ __forceinline__ double pair_sum_d(double v)
 }
 
 template <int K>
-__device__ __forceinline__ void subperms(CCShared &sh, int lane)
+__device__ __forceinline__ void subperms(CCShared<K> &sh, int lane)
 {
     constexpr int h = K / 2;
     constexpr int hmax = K - h;
@@ -222,7 +226,7 @@ __global__ void __launch_bounds__(64) cc_step_kernel(const cd *__restrict__ U, i
                                                      uint64_t seed, uint64_t first, const int32_t *__restrict__ pc_all,
                                                      int32_t *__restrict__ r_all, const int64_t *__restrict__ gidx)
 {
-    __shared__ CCShared sh;
+    __shared__ CCShared<K> sh;
     const int lane = threadIdx.x;
     constexpr int CH = CC_MAX_M / 64;
     for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
@@ -329,7 +333,7 @@ using namespace pcvl;
 extern "C" int pcvl_cc_workspace_bytes(int n, int64_t count, size_t *bytes)
 {
     PCVL_ARG_CHECK(bytes != nullptr, "bytes is null");
-    PCVL_ARG_CHECK(n >= 0 && n <= CC_MAX_N && count >= 0, "n outside [0, 24] or count < 0");
+    PCVL_ARG_CHECK(n >= 0 && n <= CC_MAX_N && count >= 0, "n outside [0, 30] or count < 0");
     *bytes = sizeof(int32_t) * ((size_t)2 * (size_t)count * (size_t)std::max(n, 1) + (size_t)CC_MAX_N);
     return OK;
 }
@@ -346,7 +350,7 @@ extern "C" int pcvl_cc_samples(const double *d_U, int m, const int32_t *h_in_occ
         PCVL_ARG_CHECK(h_in_occ[i] >= 0, "negative occupation");
         for (int v = 0; v < h_in_occ[i]; v++) cols.push_back(i);
     }
-    PCVL_ARG_CHECK((int)cols.size() <= CC_MAX_N, "more than 24 photons");
+    PCVL_ARG_CHECK((int)cols.size() <= CC_MAX_N, "more than 30 photons");
     if (count == 0) return OK;
     PCVL_ARG_CHECK(d_U && d_out, "null device buffer");
     const int n = (int)cols.size();
@@ -401,7 +405,7 @@ extern "C" int pcvl_cc_samples_multi(const double *d_U, int m, const int32_t *h_
             PCVL_ARG_CHECK(v >= 0, "negative occupation");
             for (int t = 0; t < v; t++) cols[(size_t)i].push_back(q);
         }
-        PCVL_ARG_CHECK(cols[(size_t)i].size() <= (size_t)CC_MAX_N, "more than 24 photons");
+        PCVL_ARG_CHECK(cols[(size_t)i].size() <= (size_t)CC_MAX_N, "more than 30 photons");
         row0[(size_t)i + 1] = row0[(size_t)i] + h_counts[i];
     }
     const int64_t total = row0[(size_t)ninputs];

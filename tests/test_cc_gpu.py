@@ -44,6 +44,23 @@ def test_same_seed_same_samples_m60_n20(gpu):
     assert np.mean(np.all(g == c, axis=1)) >= 0.97
 
 
+@pytest.mark.parametrize("n,count", [(26, 8), (30, 2)])
+def test_same_seed_same_samples_up_to_30_photons(gpu, n, count):
+    """The largest photon counts the sampler takes (steps up to 30, 2^28
+    Gray-code terms in the last): the oracle's samples, seed for seed."""
+    from perceval_amd.sampling import cc_samples
+    m = n + 10
+    U = haar_unitary(m, seed=m)
+    inp = [1] * n + [0] * 10
+    _say(f"GPU {count} samples, n={n}")
+    g = cc_samples(U, inp, count, seed=7)
+    _say(f"oracle {count} samples, n={n}")
+    c = oracle.cc_samples(U, inp, count, seed=7)
+    _say("done")
+    assert (g.sum(axis=1) == n).all()
+    assert np.array_equal(g, c)
+
+
 @pytest.mark.parametrize("inp", [[1, 1, 0, 1, 0], [2, 1, 0, 0]])
 def test_distribution_chi_square(gpu, inp):
     from perceval_amd.sampling import cc_samples

@@ -201,6 +201,11 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     offs = (ctypes.c_int64 * 1)(0)
     assert "order" in err(lib.pcvl_gurvits_batch(out, n, offs, 1, 100, 1, 0, out, out, None))
     assert "iterations" in err(lib.pcvl_gurvits_batch(out, n, offs, 1, 0, 1, 0, out, out, None))
+    nbytes = ctypes.c_size_t()
+    assert "n outside" in err(lib.pcvl_cc_workspace_bytes(31, 10, ctypes.byref(nbytes)))
+    assert lib.pcvl_cc_workspace_bytes(30, 10, ctypes.byref(nbytes)) == 0 and nbytes.value > 0
+    many = (ctypes.c_int32 * 2)(16, 15)
+    assert "more than 30 photons" in err(lib.pcvl_cc_samples(out, 2, many, 1, 1, 0, out, out, 1 << 20, None))
     inputs = (ctypes.c_int32 * 3)(1, 1, 0)
     counts = (ctypes.c_int64 * 1)(-1)
     assert "negative" in err(lib.pcvl_cc_samples_multi(out, 3, inputs, counts, 1, 1, 0, out, None))
