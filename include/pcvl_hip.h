@@ -14,7 +14,13 @@
  *     _umat[ok, ik] in perceval/backends/_naive.py:65;
  *   - d_ pointers are device memory owned by the caller, h_ pointers are host
  *     memory owned by the caller; work is enqueued on `stream` (0 = default)
- *     unless the function says it synchronises.
+ *     unless the function says it synchronises;
+ *   - a plan is immutable after creation except through its own setters, so
+ *     several threads may execute one plan at once on different streams with
+ *     different output and workspace buffers (the permanent plans' queue
+ *     cursor and partial sums are plan state: one execution of a permanent
+ *     plan at a time); creating, setting and destroying a plan are not
+ *     concurrent with its use.
  */
 #ifndef PCVL_HIP_H
 #define PCVL_HIP_H
