@@ -50,14 +50,17 @@ struct CCShared {
 // column has column sum 1 and never flips).  Per term each lane forms the
 // prefix products of its half and the half total T; the partners swap T, and
 // prod_{l' != l} c_l' = prefix(l) * suffix(l) * T_partner comes out of one
-// backward pass.  acc[j] is this lane's partial for column c0 + j.
-template <int K>
-__device__ __forceinline__ void subperm_pair(const cd *__restrict__ B, int lane, cd *acc)
+// backward pass.  acc[j] is this lane's partial for column c0 + j.  With
+// PLOG2 > 0 the 2^{rows-1} terms are cut into 2^PLOG2 parts and the wave
+// takes part `part` only (terms part * 2^{rows-1-PLOG2} onwards).
+template <int K, int PLOG2 = 0>
+__device__ __forceinline__ void subperm_pair(const cd *__restrict__ B, int lane, cd *acc, uint32_t part = 0)
 {
     constexpr int rows = K - 1;
     constexpr int h = K / 2;
     constexpr int hmax = K - h;
-    constexpr int total_log2 = rows - 1;                  // 2^{rows-1} terms
+    constexpr int total_log2 = rows - 1 - PLOG2;          // 2^{rows-1-PLOG2} terms in the part
+    static_assert(total_log2 >= 0, "more parts than Gray-code terms");
     constexpr int c = total_log2 > 5 ? total_log2 - 5 : 0; // 32 pairs
     constexpr int active_pairs = total_log2 > 5 ? 32 : (1 << total_log2);
     const int half = lane & 1;
@@ -68,7 +71,7 @@ __device__ __forceinline__ void subperm_pair(const cd *__restrict__ B, int lane,
     for (int j = 0; j < hmax; j++) acc[j] = {0.0, 0.0};
     if (pair >= active_pairs) return;
     const uint32_t steps = 1u << c;
-    const uint32_t base = (uint32_t)pair << c;
+    const uint32_t base = (part << total_log2) + ((uint32_t)pair << c);
     const uint32_t g0 = base ^ (base >> 1);
 
     // For odd K the even lane's last column is padding: it reads a valid
@@ -219,6 +222,55 @@ __global__ void cc_finish_multi_kernel(const int32_t *__restrict__ r_all, const 
     out[gidx[s] * m + i] = v;
 }
 
+// Draw of photon K of sample s from the sub-permanents in sh.sub: the weight
+// of every candidate row i (lane i of each 64-row chunk), then an inclusive
+// scan over rows in order picks r_k.
+template <int K>
+__device__ __forceinline__ void cc_draw(const CCShared<K> &sh, const cd *__restrict__ U, int m, int n, int64_t s,
+                                        uint64_t seed, uint64_t stream, int lane, int32_t *__restrict__ r_all)
+{
+    constexpr int CH = CC_MAX_M / 64;
+    double wv[CH];
+    double total = 0.0;
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+        const int i = c * 64 + lane;
+        double w = 0.0;
+        if (c * 64 < m && i < m) {
+            cd a = {0.0, 0.0};
+#pragma unroll
+            for (int l = 0; l < K; l++) cfma_acc(a, U[(size_t)i * m + sh.pc[l]], sh.sub[l]);
+            w = a.r * a.r + a.i * a.i;
+        }
+        wv[c] = w;
+        total += wave_sum_d(w);
+    }
+    const double target = uniform01(seed, stream, (uint64_t)(n - 1) + (uint64_t)(K - 1)) * total;
+    int pick = m - 1;
+    double base = 0.0;
+    bool found = false;
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+        if (!found && c * 64 < m) {
+            const int i = c * 64 + lane;
+            double incl = wv[c];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const double v = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += v;
+            }
+            const bool hit = (i < m) && (target < base + incl);
+            const unsigned long long mask = __ballot(hit);
+            if (mask) {
+                pick = c * 64 + (int)__builtin_ctzll(mask);
+                found = true;
+            }
+            base += __shfl(incl, 63, 64);
+        }
+    }
+    if (lane == 0) r_all[s * n + (K - 1)] = pick;
+}
+
 // Step k (= K) of every sample: one wavefront per sample, persistent.
 // r[s*n + i] holds the output mode of photon i (i < k-1 already drawn).
 template <int K>
@@ -228,7 +280,6 @@ __global__ void __launch_bounds__(64) cc_step_kernel(const cd *__restrict__ U, i
 {
     __shared__ CCShared<K> sh;
     const int lane = threadIdx.x;
-    constexpr int CH = CC_MAX_M / 64;
     for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
         const uint64_t stream = first + (uint64_t)(gidx ? gidx[s] : s);
         if (lane < K) sh.pc[lane] = pc_all[s * n + lane];
@@ -247,47 +298,88 @@ __global__ void __launch_bounds__(64) cc_step_kernel(const cd *__restrict__ U, i
             subperms<K>(sh, lane);
         }
         __syncthreads();
-        // weight of every candidate row i (lane i of each 64-row chunk), then
-        // an inclusive scan over rows in order draws r_k
-        double wv[CH];
-        double total = 0.0;
-#pragma unroll
-        for (int c = 0; c < CH; c++) {
-            const int i = c * 64 + lane;
-            double w = 0.0;
-            if (c * 64 < m && i < m) {
-                cd a = {0.0, 0.0};
-#pragma unroll
-                for (int l = 0; l < K; l++) cfma_acc(a, U[(size_t)i * m + sh.pc[l]], sh.sub[l]);
-                w = a.r * a.r + a.i * a.i;
-            }
-            wv[c] = w;
-            total += wave_sum_d(w);
+        cc_draw<K>(sh, U, m, n, s, seed, stream, lane, r_all);
+        __syncthreads();
+    }
+}
+
+// Steps above CC_SPLIT: the 2^{K-2} Gray-code terms of a sample are cut into
+// 2^{K - CC_SPLIT} parts of 2^{CC_SPLIT - 2} terms, one wave per (sample,
+// part), so that even one sample fills the device.  cc_part_kernel writes
+// each part's pair-summed partial sub-permanents; cc_split_draw_kernel sums a
+// sample's parts in part order, scales, and draws.  The cut depends on K
+// only, so a sample's draws do not depend on how many samples a call holds.
+constexpr int CC_SPLIT = 20;
+constexpr size_t CC_PART_BYTES = size_t(256) << 20;   // partial sums of one chunk of samples
+
+constexpr int cc_parts_log2(int k) { return k > CC_SPLIT ? k - CC_SPLIT : 0; }
+
+// Samples of one chunk [s0, s0 + nchunk): item t = (sample s0 + t / P, part t % P).
+template <int K>
+__global__ void __launch_bounds__(64) cc_part_kernel(const cd *__restrict__ U, int m, int n, int64_t s0,
+                                                     int64_t nchunk, const int32_t *__restrict__ pc_all,
+                                                     const int32_t *__restrict__ r_all, cd *__restrict__ partials)
+{
+    constexpr int PLOG2 = cc_parts_log2(K);
+    constexpr int64_t P = int64_t(1) << PLOG2;
+    constexpr int h = K / 2;
+    constexpr int hmax = K - h;
+    __shared__ CCShared<K> sh;
+    const int lane = threadIdx.x;
+    for (int64_t t = blockIdx.x; t < nchunk * P; t += gridDim.x) {
+        const int64_t sl = t >> PLOG2;
+        const uint32_t part = (uint32_t)(t & (P - 1));
+        const int64_t s = s0 + sl;
+        if (lane < K) sh.pc[lane] = pc_all[s * n + lane];
+        if (lane < K - 1) sh.r[lane] = r_all[s * n + lane];
+        __syncthreads();
+        for (int e = lane; e < (K - 1) * K; e += 64) {
+            const int i = e / K, l = e % K;
+            const cd a = U[(size_t)sh.r[i] * m + sh.pc[l]];
+            sh.B[e] = {2.0 * a.r, 2.0 * a.i};
         }
-        const double target = uniform01(seed, stream, (uint64_t)(n - 1) + (uint64_t)(K - 1)) * total;
-        int pick = m - 1;
-        double base = 0.0;
-        bool found = false;
+        __syncthreads();
+        cd acc[hmax];
+        subperm_pair<K, PLOG2>(sh.B, lane, acc, part);
+        cd *dst = partials + t * K;
 #pragma unroll
-        for (int c = 0; c < CH; c++) {
-            if (!found && c * 64 < m) {
-                const int i = c * 64 + lane;
-                double incl = wv[c];
-#pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    const double v = __shfl_up(incl, off, 64);
-                    if (lane >= off) incl += v;
-                }
-                const bool hit = (i < m) && (target < base + incl);
-                const unsigned long long mask = __ballot(hit);
-                if (mask) {
-                    pick = c * 64 + (int)__builtin_ctzll(mask);
-                    found = true;
-                }
-                base += __shfl(incl, 63, 64);
-            }
+        for (int j = 0; j < hmax; j++) {
+            const double re = pair_sum_d(acc[j].r), im = pair_sum_d(acc[j].i);
+            if (lane == 0 && j < h) dst[j] = {re, im};
+            if (lane == 1) dst[h + j] = {re, im};
         }
-        if (lane == 0) r_all[s * n + (K - 1)] = pick;
+        __syncthreads();
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(64) cc_split_draw_kernel(const cd *__restrict__ U, int m, int n, int64_t s0,
+                                                           int64_t nchunk, uint64_t seed, uint64_t first,
+                                                           const int32_t *__restrict__ pc_all,
+                                                           int32_t *__restrict__ r_all,
+                                                           const int64_t *__restrict__ gidx,
+                                                           const cd *__restrict__ partials)
+{
+    constexpr int64_t P = int64_t(1) << cc_parts_log2(K);
+    __shared__ CCShared<K> sh;
+    const int lane = threadIdx.x;
+    const double scale = ldexp(1.0, -(K - 2));
+    for (int64_t sl = blockIdx.x; sl < nchunk; sl += gridDim.x) {
+        const int64_t s = s0 + sl;
+        const uint64_t stream = first + (uint64_t)(gidx ? gidx[s] : s);
+        if (lane < K) {
+            sh.pc[lane] = pc_all[s * n + lane];
+            cd v = {0.0, 0.0};
+            const cd *src = partials + sl * P * K + lane;
+            for (int64_t p = 0; p < P; p++) {   // in part order
+                const cd x = src[p * K];
+                v.r += x.r;
+                v.i += x.i;
+            }
+            sh.sub[lane] = {v.r * scale, v.i * scale};
+        }
+        __syncthreads();
+        cc_draw<K>(sh, U, m, n, s, seed, stream, lane, r_all);
         __syncthreads();
     }
 }
@@ -305,23 +397,62 @@ __global__ void cc_finish_kernel(const int32_t *__restrict__ r_all, int m, int n
     out[e] = v;
 }
 
+// Samples per chunk of the split steps, and the partial sums a chunk needs.
+static int64_t cc_chunk(int n)
+{
+    const int64_t per_sample = (int64_t(1) << cc_parts_log2(n)) * n * (int64_t)sizeof(cd);
+    return std::max<int64_t>(1, (int64_t)CC_PART_BYTES / per_sample);
+}
+
+static size_t cc_partial_bytes(int n, int64_t count)
+{
+    if (n <= CC_SPLIT) return 0;
+    return (size_t)std::min(count, cc_chunk(n)) * ((size_t)1 << cc_parts_log2(n)) * (size_t)n * sizeof(cd);
+}
+
 template <int K>
 static int launch_step(int k, const cd *U, int m, int n, int64_t count, uint64_t seed, uint64_t first,
-                       const int32_t *pc, int32_t *r, int cus, hipStream_t st, const int64_t *gidx = nullptr)
+                       const int32_t *pc, int32_t *r, cd *partials, int cus, hipStream_t st,
+                       const int64_t *gidx = nullptr)
 {
     if constexpr (K > CC_MAX_N) {
         set_error("internal: photon step out of range");
         return ERR_ARG;
     } else {
-        if (k != K) return launch_step<K + 1>(k, U, m, n, count, seed, first, pc, r, cus, st, gidx);
-        int per_cu = 0;
-        PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&cc_step_kernel<K>), 64, 0));
-        if (per_cu < 1) per_cu = 1;
-        const int grid = (int)std::min<int64_t>(count, (int64_t)cus * per_cu);
-        hipLaunchKernelGGL((cc_step_kernel<K>), dim3(grid), dim3(64), 0, st, U, m, n, count, seed, first, pc, r,
-                           gidx);
-        PCVL_HIP_CHECK(hipGetLastError());
+        if (k != K) return launch_step<K + 1>(k, U, m, n, count, seed, first, pc, r, partials, cus, st, gidx);
+        if constexpr (K <= CC_SPLIT) {
+            int per_cu = 0;
+            PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void *>(&cc_step_kernel<K>), 64, 0));
+            if (per_cu < 1) per_cu = 1;
+            const int grid = (int)std::min<int64_t>(count, (int64_t)cus * per_cu);
+            hipLaunchKernelGGL((cc_step_kernel<K>), dim3(grid), dim3(64), 0, st, U, m, n, count, seed, first, pc, r,
+                               gidx);
+            PCVL_HIP_CHECK(hipGetLastError());
+        } else {
+            constexpr int64_t P = int64_t(1) << cc_parts_log2(K);
+            int per_cu_p = 0, per_cu_d = 0;
+            PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu_p, reinterpret_cast<const void *>(&cc_part_kernel<K>), 64, 0));
+            PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu_d, reinterpret_cast<const void *>(&cc_split_draw_kernel<K>), 64, 0));
+            per_cu_p = std::max(per_cu_p, 1);
+            per_cu_d = std::max(per_cu_d, 1);
+            // chunks sized for the photon count n of the call (the largest
+            // partial buffer of its steps), so the workspace holds any step's
+            const int64_t chunk = cc_chunk(n);
+            for (int64_t s0 = 0; s0 < count; s0 += chunk) {
+                const int64_t nchunk = std::min(chunk, count - s0);
+                const int gp = (int)std::min<int64_t>(nchunk * P, (int64_t)cus * per_cu_p);
+                hipLaunchKernelGGL((cc_part_kernel<K>), dim3(gp), dim3(64), 0, st, U, m, n, s0, nchunk, pc, r,
+                                   partials);
+                PCVL_HIP_CHECK(hipGetLastError());
+                const int gd = (int)std::min<int64_t>(nchunk, (int64_t)cus * per_cu_d);
+                hipLaunchKernelGGL((cc_split_draw_kernel<K>), dim3(gd), dim3(64), 0, st, U, m, n, s0, nchunk, seed,
+                                   first, pc, r, gidx, partials);
+                PCVL_HIP_CHECK(hipGetLastError());
+            }
+        }
         return OK;
     }
 }
@@ -334,7 +465,11 @@ extern "C" int pcvl_cc_workspace_bytes(int n, int64_t count, size_t *bytes)
 {
     PCVL_ARG_CHECK(bytes != nullptr, "bytes is null");
     PCVL_ARG_CHECK(n >= 0 && n <= CC_MAX_N && count >= 0, "n outside [0, 30] or count < 0");
-    *bytes = sizeof(int32_t) * ((size_t)2 * (size_t)count * (size_t)std::max(n, 1) + (size_t)CC_MAX_N);
+    // pc (count*n) | r (count*n) | cols (CC_MAX_N), then from a 256-byte
+    // boundary the partial sums of the split steps (more than 20 photons)
+    const size_t head = sizeof(int32_t) * ((size_t)2 * (size_t)count * (size_t)std::max(n, 1) + (size_t)CC_MAX_N);
+    const size_t part = cc_partial_bytes(n, count);
+    *bytes = part ? ((head + 255) & ~size_t(255)) + part : head;
     return OK;
 }
 
@@ -364,10 +499,12 @@ extern "C" int pcvl_cc_samples(const double *d_U, int m, const int32_t *h_in_occ
         PCVL_HIP_CHECK(hipGetLastError());
         return OK;
     }
-    // workspace: pc (count*n) | r (count*n) | cols (CC_MAX_N)
+    // workspace: pc (count*n) | r (count*n) | cols (CC_MAX_N) | partial sums
     int32_t *pc = static_cast<int32_t *>(d_work);
     int32_t *r = pc + (size_t)count * n;
     int32_t *d_cols = r + (size_t)count * n;
+    const size_t head = sizeof(int32_t) * ((size_t)2 * (size_t)count * (size_t)n + (size_t)CC_MAX_N);
+    cd *partials = reinterpret_cast<cd *>(static_cast<char *>(d_work) + ((head + 255) & ~size_t(255)));
     PCVL_HIP_CHECK(hipMemcpyAsync(d_cols, cols.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream));
     const cd *U = reinterpret_cast<const cd *>(d_U);
     int dev = 0, cus = 0;
@@ -377,7 +514,7 @@ extern "C" int pcvl_cc_samples(const double *d_U, int m, const int32_t *h_in_occ
                        seed, first_sample, pc);
     PCVL_HIP_CHECK(hipGetLastError());
     for (int k = 1; k <= n; k++) {
-        int rc = launch_step<1>(k, U, m, n, count, seed, first_sample, pc, r, cus, stream);
+        int rc = launch_step<1>(k, U, m, n, count, seed, first_sample, pc, r, partials, cus, stream);
         if (rc != OK) return rc;
     }
     const int64_t tot = count * m;
@@ -424,20 +561,23 @@ extern "C" int pcvl_cc_samples_multi(const double *d_U, int m, const int32_t *h_
     }
     int nmax = 1;
     for (int n : ns) nmax = std::max(nmax, n);
-    // device block: pc | r (gmax * nmax each) | inp | gidx (gmax) | cols (ninputs * nmax)
+    // device block: pc | r (gmax * nmax each) | inp | gidx (gmax) | cols (ninputs * nmax) | partial sums
     const size_t b_pr = sizeof(int32_t) * (size_t)gmax * nmax;
     const size_t b_inp = sizeof(int32_t) * (size_t)gmax;
     const size_t b_gidx = sizeof(int64_t) * (size_t)gmax;
     const size_t b_cols = sizeof(int32_t) * (size_t)ninputs * nmax;
+    size_t b_part = 0;
+    for (int n : ns) b_part = std::max(b_part, cc_partial_bytes(n, gmax));
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     char *blk = nullptr;
-    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), 2 * al(b_pr) + al(b_inp) + al(b_gidx) + al(b_cols),
-                                  stream));
+    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk),
+                                  2 * al(b_pr) + al(b_inp) + al(b_gidx) + al(b_cols) + al(b_part), stream));
     int32_t *pc = reinterpret_cast<int32_t *>(blk);
     int32_t *r = reinterpret_cast<int32_t *>(blk + al(b_pr));
     int32_t *d_inp = reinterpret_cast<int32_t *>(blk + 2 * al(b_pr));
     int64_t *d_gidx = reinterpret_cast<int64_t *>(blk + 2 * al(b_pr) + al(b_inp));
     int32_t *d_cols = reinterpret_cast<int32_t *>(blk + 2 * al(b_pr) + al(b_inp) + al(b_gidx));
+    cd *partials = reinterpret_cast<cd *>(blk + 2 * al(b_pr) + al(b_inp) + al(b_gidx) + al(b_cols));
     const cd *U = reinterpret_cast<const cd *>(d_U);
     int dev = 0, cus = 0;
     int rc = OK;
@@ -465,7 +605,7 @@ extern "C" int pcvl_cc_samples_multi(const double *d_U, int m, const int32_t *h_
             e = hipGetLastError();
             if (e != hipSuccess) { set_error(std::string("cc_init_multi: ") + hipGetErrorString(e)); rc = ERR_HIP; break; }
             for (int k = 1; k <= n && rc == OK; k++)
-                rc = launch_step<1>(k, U, m, n, count, seed, first_sample, pc, r, cus, stream, d_gidx);
+                rc = launch_step<1>(k, U, m, n, count, seed, first_sample, pc, r, partials, cus, stream, d_gidx);
             if (rc != OK) break;
         }
         const int64_t tot = count * m;

@@ -44,10 +44,12 @@ def test_same_seed_same_samples_m60_n20(gpu):
     assert np.mean(np.all(g == c, axis=1)) >= 0.97
 
 
-@pytest.mark.parametrize("n,count", [(26, 8), (30, 2)])
-def test_same_seed_same_samples_up_to_30_photons(gpu, n, count):
-    """The largest photon counts the sampler takes (steps up to 30, 2^28
-    Gray-code terms in the last): the oracle's samples, seed for seed."""
+@pytest.mark.parametrize("n,count", [(26, 8), (28, 1)])
+def test_same_seed_same_samples_up_to_28_photons(gpu, n, count):
+    """Steps above 20 photons run split over many waves per sample (2^{k-20}
+    parts of the 2^{k-2} Gray-code terms): the oracle's samples, seed for
+    seed, at 26 and 28 photons (one sample at 28: a single sample fills the
+    device)."""
     from perceval_amd.sampling import cc_samples
     m = n + 10
     U = haar_unitary(m, seed=m)
@@ -59,6 +61,23 @@ def test_same_seed_same_samples_up_to_30_photons(gpu, n, count):
     _say("done")
     assert (g.sum(axis=1) == n).all()
     assert np.array_equal(g, c)
+
+
+@pytest.mark.parametrize("n,count", [(22, 6), (30, 2)])
+def test_split_steps_do_not_depend_on_the_call(gpu, n, count):
+    """A sample's draws depend on (seed, sample index) only: one call of
+    `count` samples equals `count` calls of one sample each (first_sample
+    set), with the split steps cut the same way in both, up to 30 photons."""
+    from perceval_amd.sampling import cc_samples
+    m = n + 8
+    U = haar_unitary(m, seed=3 * m)
+    inp = [1] * n + [0] * 8
+    _say(f"n={n}: one call of {count}")
+    together = cc_samples(U, inp, count, seed=11)
+    _say(f"n={n}: {count} calls of one")
+    apart = np.concatenate([cc_samples(U, inp, 1, seed=11, first_sample=i) for i in range(count)])
+    assert (together.sum(axis=1) == n).all()
+    assert np.array_equal(together, apart)
 
 
 @pytest.mark.parametrize("inp", [[1, 1, 0, 1, 0], [2, 1, 0, 0]])
