@@ -45,9 +45,10 @@ int pcvl_abi_version(void);
  * the generic all_prob/prob_distribution loops
  * (perceval/backends/_abstract_backends.py:173-208).
  *
- * Batch layout: matrix b has order h_n[b] (0..28) and occupies h_n[b]^2
+ * Batch layout: matrix b has order h_n[b] (0..32) and occupies h_n[b]^2
  * complex elements, row-major, starting at complex element h_offsets[b] of
- * d_mats.  d_out receives one complex permanent per matrix.
+ * d_mats.  d_out receives one complex permanent per matrix.  Orders 9..28
+ * share one persistent launch; orders 29..32 take a launch of their own.
  * ------------------------------------------------------------------------- */
 typedef struct pcvl_perm_plan_s *pcvl_perm_plan;
 
@@ -124,7 +125,8 @@ int pcvl_fock_unrank(const int64_t *d_rank, int64_t count, int m, int n, int32_t
  * mask_n <= n a state is kept when it matches a row; with mask_n > n, when
  * the mask_n - n photons still to come can complete a row (no exact count
  * exceeded, the photons missing from exact counts and floors at most
- * mask_n - n), as the reference's simulator needs for the separated parts of
+ * mask_n - n, and exactly mask_n - n for a row with no free or at-least mode
+ * to take the rest), as the reference's simulator needs for the separated parts of
  * a noisy input (perceval/simulators/simulator.py:412-470).  d_out must hold
  * C(m+n-1, n) entries; *d_count (device) receives the number of states kept.
  * At most 16 masks and 2^34 states.  Synchronises `stream`. */

@@ -30,7 +30,11 @@
 namespace pcvl {
 
 constexpr int PERM_WAVE_MIN = 9;
-constexpr int PERM_WAVE_MAX = 28;
+constexpr int PERM_WAVE_MAX = 28;   // the mixed-order launch
+// Orders 29..32 run in a launch of their own (their registers would lower the
+// occupancy of the mixed launch), from their own part of the queue.
+constexpr int PERM_BIG_MIN = PERM_WAVE_MAX + 1;
+constexpr int PERM_BIG_MAX = 32;
 constexpr int PERM_SMALL_MAX = 8;
 
 struct PermItem {
@@ -380,7 +384,7 @@ __global__ void perm_finalize_kernel(const int64_t *__restrict__ idx, const int3
                                      unsigned long long *cursor)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) *cursor = 0ull;
+    if (k == 0) { cursor[0] = 0ull; cursor[1] = 0ull; }
     if (k >= count) return;
     cd s = {0.0, 0.0};
     for (int q = 0; q < nslot[k]; q++) s = cadd(s, partials[slot0[k] + q]);
@@ -393,11 +397,13 @@ __global__ void perm_finalize_kernel(const int64_t *__restrict__ idx, const int3
 struct PermPlanImpl {
     int64_t count = 0;
     int64_t nitems = 0, nwave_mats = 0, nsmall = 0;
+    int64_t nbig = 0;          // items of orders 29..32, after the nitems others in the queue
+    int grid_big = 0;
     int single_n = 0;          // order shared by every wave item, or 0 if mixed
     int variant = 0;           // 0 auto, 1..4 (single-order plans): row 1 in VGPRs / LDS, separate / fused flips
     int grid = 0;
     PermItem *d_items = nullptr;
-    unsigned long long *d_cursor = nullptr;
+    unsigned long long *d_cursor = nullptr;   // [0]: the queue of orders <= 28, [1]: orders 29..32
     cd *d_partials = nullptr;
     int64_t *d_wave_idx = nullptr;
     int32_t *d_wave_slot0 = nullptr, *d_wave_nslot = nullptr, *d_wave_n = nullptr;
@@ -417,12 +423,19 @@ static int pick_chunk_log2(int n, int max_c)
 }
 
 template <int NLO, int NHI, int R1>
-static int launch_wave(PermPlanImpl *p, const double *mats, hipStream_t st)
+static int launch_wave(const PermPlanImpl *p, const double *mats, hipStream_t st, const PermItem *items,
+                       int64_t nitems, unsigned long long *cursor, int grid)
 {
-    hipLaunchKernelGGL((perm_wave_kernel<NLO, NHI, R1>), dim3(p->grid), dim3(64), 0, st, mats, p->d_items,
-                       p->nitems, p->d_cursor, p->d_partials);
+    hipLaunchKernelGGL((perm_wave_kernel<NLO, NHI, R1>), dim3(grid), dim3(64), 0, st, mats, items, nitems, cursor,
+                       p->d_partials);
     PCVL_HIP_CHECK(hipGetLastError());
     return OK;
+}
+
+template <int NLO, int NHI, int R1>
+static int launch_wave(const PermPlanImpl *p, const double *mats, hipStream_t st)
+{
+    return launch_wave<NLO, NHI, R1>(p, mats, st, p->d_items, p->nitems, p->d_cursor, p->grid);
 }
 
 template <int N>
@@ -507,9 +520,10 @@ static int perm_plan_build(const int32_t *h_n, const int64_t *h_offsets, int64_t
     std::vector<int32_t> wave_slot0, wave_nslot, wave_n, small_n;
     std::vector<int> wave_c;
     int nmin = 1 << 30, nmax = -1;
+    bool any_big = false;
     for (int64_t b = 0; b < count; b++) {
         int n = h_n[b];
-        PCVL_ARG_CHECK(n >= 0 && n <= PERM_WAVE_MAX, "matrix order outside [0, 28]");
+        PCVL_ARG_CHECK(n >= 0 && n <= PERM_BIG_MAX, "matrix order outside [0, 32]");
         PCVL_ARG_CHECK(h_offsets[b] >= 0, "negative matrix offset");
         if (n <= PERM_SMALL_MAX) {
             small_idx.push_back(b);
@@ -520,29 +534,41 @@ static int perm_plan_build(const int32_t *h_n, const int64_t *h_offsets, int64_t
         wave_idx.push_back(b);
         wave_n.push_back(n);
         wave_c.push_back(pick_chunk_log2(n, max_chunk_log2));
+        if (n >= PERM_BIG_MIN) {
+            any_big = true;
+            continue;
+        }
         nmin = std::min(nmin, n);
         nmax = std::max(nmax, n);
     }
-    const int single_n = (nmin == nmax) ? nmax : 0;
+    // orders <= 28: one launch, specialised when they share one order
+    const int single_n = (nmax >= 0 && nmin == nmax) ? nmax : 0;
     const int kvariant = single_n ? variant : 0;
-    int rw = 0;
-    if (!wave_idx.empty()) {
+    int rw = 0, rwb = 0;
+    if (nmax >= 0) {
         const void *kp = single_n ? single_kernel_ptr<PERM_WAVE_MIN>(single_n, kvariant)
                                   : reinterpret_cast<const void *>(&perm_wave_kernel<PERM_WAVE_MIN, PERM_WAVE_MAX, 0>);
         int rc = resident_waves(kp, &rw);
         if (rc != OK) return rc;
     }
-    // Tail refinement: the matrices whose items form the last round of the
-    // queue (the last `rw` items in cost order) use chunks 4x shorter, so the
-    // waves that finish last wait for at most a quarter item.
-    {
-        std::vector<size_t> order(wave_idx.size());
-        for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    if (any_big) {
+        int rc = resident_waves(reinterpret_cast<const void *>(&perm_wave_kernel<PERM_BIG_MIN, PERM_BIG_MAX, 0>), &rwb);
+        if (rc != OK) return rc;
+    }
+    // Tail refinement, per launch: the matrices whose items form the last
+    // round of its queue (the last `resident waves` items in cost order) use
+    // chunks 4x shorter, so the waves that finish last wait for at most a
+    // quarter item.
+    for (int big = 0; big < 2; big++) {
+        std::vector<size_t> order;
+        for (size_t i = 0; i < wave_idx.size(); i++)
+            if ((wave_n[i] >= PERM_BIG_MIN) == (big == 1)) order.push_back(i);
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
             return ((int64_t)wave_n[a] << wave_c[a]) > ((int64_t)wave_n[b] << wave_c[b]);
         });
+        const int resident = big ? rwb : rw;
         int64_t tail = 0;
-        for (size_t r = order.size(); r-- > 0 && tail < rw;) {
+        for (size_t r = order.size(); r-- > 0 && tail < resident;) {
             const size_t i = order[r];
             tail += (int64_t)((1ull << (wave_n[i] - 1 - wave_c[i])) / 64);
             wave_c[i] = std::max(2, wave_c[i] - 2);
@@ -557,24 +583,33 @@ static int perm_plan_build(const int32_t *h_n, const int64_t *h_offsets, int64_t
         return ERR_RANGE;
     }
     items.reserve((size_t)total_slots);
+    std::vector<PermItem> big_items;
     int32_t slots = 0;
     for (size_t i = 0; i < wave_idx.size(); i++) {
         const int n = wave_n[i], c = wave_c[i];
         const int64_t waves = (int64_t)((1ull << (n - 1 - c)) / 64);
         wave_slot0.push_back(slots);
         wave_nslot.push_back((int32_t)waves);
+        std::vector<PermItem> &dst = n >= PERM_BIG_MIN ? big_items : items;
         for (int64_t w = 0; w < waves; w++)
-            items.push_back(PermItem{h_offsets[wave_idx[i]], (uint32_t)(w * 64), slots + (int32_t)w, n, c});
+            dst.push_back(PermItem{h_offsets[wave_idx[i]], (uint32_t)(w * 64), slots + (int32_t)w, n, c});
         slots += (int32_t)waves;
     }
     // Largest items first: the atomic queue then gives a greedy
-    // longest-processing-time schedule over the resident waves.
-    std::stable_sort(items.begin(), items.end(), [](const PermItem &a, const PermItem &b) {
+    // longest-processing-time schedule over the resident waves.  The queue
+    // holds the items of orders <= 28, then those of orders 29..32.
+    auto by_cost = [](const PermItem &a, const PermItem &b) {
         return ((int64_t)a.n << a.c) > ((int64_t)b.n << b.c);
-    });
+    };
+    std::stable_sort(items.begin(), items.end(), by_cost);
+    std::stable_sort(big_items.begin(), big_items.end(), by_cost);
+    const int64_t nregular = (int64_t)items.size();
+    items.insert(items.end(), big_items.begin(), big_items.end());
 
     p.count = count;
-    p.nitems = (int64_t)items.size();
+    p.nitems = nregular;
+    p.nbig = (int64_t)big_items.size();
+    p.grid_big = p.nbig > 0 ? (int)std::min<int64_t>(p.nbig, (int64_t)rwb) : 0;
     p.nwave_mats = (int64_t)wave_idx.size();
     p.nsmall = (int64_t)small_idx.size();
     p.single_n = single_n;
@@ -584,7 +619,7 @@ static int perm_plan_build(const int32_t *h_n, const int64_t *h_offsets, int64_t
     auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
     L.o_items = 0;
     L.o_cursor = align(L.o_items + sizeof(PermItem) * items.size());
-    L.o_part = align(L.o_cursor + sizeof(unsigned long long));
+    L.o_part = align(L.o_cursor + 2 * sizeof(unsigned long long));
     L.o_widx = align(L.o_part + sizeof(cd) * (size_t)slots);
     L.o_ws0 = align(L.o_widx + sizeof(int64_t) * wave_idx.size());
     L.o_wns = align(L.o_ws0 + sizeof(int32_t) * wave_idx.size());
@@ -687,7 +722,12 @@ extern "C" int pcvl_perm_plan_execute(pcvl_perm_plan plan, const double *d_mats,
         PCVL_HIP_CHECK(hipGetLastError());
     }
     if (p.nwave_mats > 0) {
-        int rc = p.single_n ? launch_single<PERM_WAVE_MIN>(&p, d_mats, stream)
+        int rc = OK;
+        if (p.nbig > 0)   // orders 29..32 first: the longest items
+            rc = launch_wave<PERM_BIG_MIN, PERM_BIG_MAX, 0>(&p, d_mats, stream, p.d_items + p.nitems, p.nbig,
+                                                            p.d_cursor + 1, p.grid_big);
+        if (rc == OK && p.nitems > 0)
+            rc = p.single_n ? launch_single<PERM_WAVE_MIN>(&p, d_mats, stream)
                             : launch_wave<PERM_WAVE_MIN, PERM_WAVE_MAX, 0>(&p, d_mats, stream);
         if (rc != OK) return rc;
         int blocks = (int)((p.nwave_mats + 255) / 256);
@@ -715,7 +755,7 @@ extern "C" int pcvl_perm_plan_destroy(pcvl_perm_plan plan)
 extern "C" int pcvl_perm_plan_info(pcvl_perm_plan plan, int64_t *nitems, int *grid, int *single_n)
 {
     PCVL_ARG_CHECK(plan != nullptr, "plan is null");
-    if (nitems) *nitems = plan->impl.nitems;
+    if (nitems) *nitems = plan->impl.nitems + plan->impl.nbig;
     if (grid) *grid = plan->impl.grid;
     if (single_n) *single_n = plan->impl.single_n;
     return OK;

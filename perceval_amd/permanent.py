@@ -2,8 +2,8 @@
 
 Replaces ``xq.permanent_cx`` (perceval/backends/_naive.py:70-71).  A batch is a
 packed complex128 device tensor holding row-major matrices back to back; the
-orders may differ from matrix to matrix (0..28) and are handled in one
-persistent launch.  ``PermanentPlan`` keeps the launch plan for a batch shape so
+orders may differ from matrix to matrix (0..32); orders 9..28 are handled in
+one persistent launch, orders 29..32 in a second one.  ``PermanentPlan`` keeps the launch plan for a batch shape so
 repeated batches (sampler acceptance, Naive all_prob over a fixed Fock space)
 pay the host-side scheduling once.
 """
@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _native
 
-MAX_ORDER = 28
+MAX_ORDER = 32
 
 
 class PermanentPlan:

@@ -108,6 +108,39 @@ def test_permanent_orders_25_to_28_vs_oracle(gpu):
         assert rel < REL_TOL, (M.shape[0], g, ref)
 
 
+def test_permanent_orders_29_to_32(gpu):
+    """Orders 29..32 (their own launch beside the one of orders <= 28): the
+    all-ones matrix J/2 (perm = n!/2^n, heavy cancellation) at every order,
+    a block-diagonal order-30 matrix (perm = product of the blocks'), one
+    Haar submatrix of order 29 against the threaded long-double oracle, and
+    a mixed batch holding orders 12, 20, 29 and 32 in one plan."""
+    from perceval_amd.permanent import permanents
+    U = haar_unitary(64, seed=2932)
+    rng = np.random.RandomState(29)
+    Js = [np.ones((n, n), dtype=complex) / 2 for n in range(29, 33)]
+    A = U[:20, 20:40]
+    B = haar_unitary(10, seed=10)
+    D = np.zeros((30, 30), dtype=complex)
+    D[:20, :20] = A
+    D[20:, 20:] = B
+    M29 = U[np.ix_(rng.choice(64, 29, replace=False), rng.choice(64, 29, replace=False))]
+    M12, M20 = U[30:42, 40:52], U[40:60, 2:22]
+    _say("GPU permanents of orders 29..32")
+    got = permanents(Js + [D, M29])
+    for n, g in zip(range(29, 33), got[:4]):
+        ref = math.factorial(n) / 2 ** n
+        assert abs(g - ref) / ref < REL_TOL, (n, g, ref)
+    pd = oracle.perm(A) * oracle.perm(B)
+    assert abs(got[4] - pd) / abs(pd) < REL_TOL
+    _say(f"oracle order 29 on {_threads()} threads")
+    ref29 = oracle.perm_large(M29, threads=_threads())
+    assert abs(got[5] - ref29) / abs(ref29) < REL_TOL, (got[5], ref29)
+    mixed = permanents([M12, Js[0], M20, Js[3]])
+    refs = [oracle.perm(M12), math.factorial(29) / 2 ** 29, oracle.perm(M20), math.factorial(32) / 2 ** 32]
+    for g, r in zip(mixed, refs):
+        assert abs(g - r) / abs(r) < REL_TOL, (g, r)
+
+
 def test_permanent_order28_closed_forms(gpu):
     """Order 28 against closed forms: the all-ones matrix (perm = 28!, the
     sum with the most cancellation: sum |terms| / |perm| = 1.26e4) and a

@@ -129,7 +129,7 @@ def test_identities_order20(gpu):
 def test_rejects_order_above_max(gpu):
     from perceval_amd.permanent import PermanentPlan
     with pytest.raises(ValueError):
-        PermanentPlan([29])
+        PermanentPlan([33])
 
 
 def test_fp64_probe_counts_and_values(gpu):
