@@ -620,6 +620,37 @@ def bench_gradient(args, res):
                                "sample": "2 parameter sets, central differences over 132 phases with oracle "
                                          "permanents, 1 thread"}
         log(f"[bench] gradient CPU (FD): {len(ph) / dt:.2f} sets/s")
+    # the objective of benchmark/benchmark_QML-DE-solver.py on a phase mesh:
+    # 6 photons in 6 modes, every one of the 462 outputs weighted by a
+    # standardised lambda (x 200), Clements mesh of 6 modes (30 phases)
+    prog6 = MeshProgram(clements_mesh(6))
+    rng6 = np.random.RandomState(0)
+    t6 = fock_space(6, 6)
+    lam = rng6.rand(len(t6))
+    lam = 200 * (lam - lam.mean()) / lam.std()
+    ph6 = torch.from_numpy(rng6.uniform(0, 2 * math.pi, size=(256, prog6.nparams))).cuda()
+    mesh_adjoint(prog6, ph6, [1] * 6, t6, weights=lam)
+    with Timer() as t:
+        for _ in range(reps):
+            mesh_adjoint(prog6, ph6, [1] * 6, t6, weights=lam)
+    ms6 = t.ms / reps
+    res["gradient_qml_m6_n6"] = {"ms_per_batch": ms6, "sets_per_s": 256 / (ms6 * 1e-3), "targets": len(t6),
+                                 "plans": (len(t6) + 63) // 64}
+    log(f"[bench] gradient of sum_s lambda_s P(s) over all {len(t6)} outputs, m=n=6: {ms6:.3f} ms per 256 sets")
+    if not args.no_cpu:
+        import oracle
+        x0 = ph6[0].cpu().numpy()
+        t0 = time.perf_counter()
+        for k in range(prog6.nparams):
+            a = x0.copy(); a[k] += 1e-6
+            c = x0.copy(); c[k] -= 1e-6
+            _ = (np.dot(lam, oracle.slos_probs(prog6.unitary(a), [1] * 6)) -
+                 np.dot(lam, oracle.slos_probs(prog6.unitary(c), [1] * 6))) / 2e-6
+        dt = time.perf_counter() - t0
+        res["gradient_qml_cpu"] = {"value": 1 / dt, "unit": "sets/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
+                                   "sample": "1 parameter set, central differences over 30 phases with the oracle's "
+                                             "SLOS probabilities (the reference benchmark's numerical gradient), 1 thread"}
+        log(f"[bench] gradient QML CPU (FD over SLOS): {1 / dt:.1f} sets/s")
 
 
 def _tvd_curve(occ, exact, grid):

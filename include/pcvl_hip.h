@@ -300,6 +300,13 @@ int pcvl_mesh_plan_create(const int32_t *h_kind, const int32_t *h_mode, const in
                           const int32_t *h_targets, int ntargets, pcvl_mesh_plan *out);
 int pcvl_mesh_plan_execute(pcvl_mesh_plan plan, const double *d_phases, int nsets, double *d_loss,
                            double *d_grad, double *d_probs, hipStream_t stream);
+/* Weights w_s of the plan's targets (ntargets doubles, finite; NULL: all 1):
+ * execute then returns loss = sum_s w_s P(s) and its gradient, the objective
+ * of a diagonal observable (e.g. sum over every output state of lambda_s
+ * P(s), as benchmark/benchmark_QML-DE-solver.py:81-84 evaluates); probs stay
+ * the unweighted P(s).  Copies synchronously; not concurrent with an
+ * execution of the plan. */
+int pcvl_mesh_plan_set_weights(pcvl_mesh_plan plan, const double *h_weights);
 int pcvl_mesh_plan_destroy(pcvl_mesh_plan plan);
 
 /* ---------------------------------------------------------------------------

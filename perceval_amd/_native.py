@@ -109,6 +109,7 @@ SIGNATURES = {
                                       P(c_i32), c_int, ctypes.POINTER(c_void_p)]),
     "pcvl_mesh_plan_execute": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pcvl_mesh_plan_destroy": (c_int, [c_void_p]),
+    "pcvl_mesh_plan_set_weights": (c_int, [c_void_p, P(ctypes.c_double)]),
     "pcvl_cc_samples": (c_int, [c_void_p, c_int, P(c_i32), c_i64, c_u64, c_u64, c_void_p, c_void_p, c_size_t,
                                 c_void_p]),
     "pcvl_cc_samples_multi": (c_int, [c_void_p, c_int, P(c_i32), P(c_i64), c_int, c_u64, c_u64, c_void_p, c_void_p]),

@@ -141,7 +141,7 @@ struct AdjIdx {
 // amplitudes, cofactors (targets x n x n), target norms, a reduction buffer of
 // ADJ_THREADS doubles, the index tables.
 struct AdjLds {
-    size_t ops, zph, U, Z, amp, cof, tn, red, idx, bytes;
+    size_t ops, zph, U, Z, amp, cof, tn, tw, red, idx, bytes;
 };
 
 __host__ __device__ inline size_t adj_al16(size_t b) { return (b + 15) & ~(size_t)15; }
@@ -157,6 +157,7 @@ __host__ __device__ inline AdjLds adj_lds(int nops, int nparams, int m, int n, i
     L.amp = o; o = adj_al16(o + sizeof(cd) * (size_t)ntargets);
     L.cof = o; o = adj_al16(o + sizeof(cd) * (size_t)ntargets * n * n);
     L.tn = o; o = adj_al16(o + sizeof(double) * (size_t)ntargets);
+    L.tw = o; o = adj_al16(o + sizeof(double) * (size_t)ntargets);
     L.red = o; o = adj_al16(o + sizeof(double) * ADJ_THREADS);
     L.idx = o; o = adj_al16(o + sizeof(int32_t) * (size_t)nidx);
     L.bytes = o;
@@ -199,8 +200,9 @@ __device__ __forceinline__ double group_sum(double v, int glog)
 // quantity, (F Z)_qq, is a reduction within the slot.
 __global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
     const MeshOp *__restrict__ ops_g, int nops, int m, const double *__restrict__ phases, int nparams, int nsets,
-    int n, const int32_t *__restrict__ idx_g, AdjIdx X, const double *__restrict__ tn_g, int ntargets,
-    double *__restrict__ loss_out, double *__restrict__ grad_out, double *__restrict__ probs_out, int stage_ops)
+    int n, const int32_t *__restrict__ idx_g, AdjIdx X, const double *__restrict__ tn_g,
+    const double *__restrict__ tw_g, int ntargets, double *__restrict__ loss_out, double *__restrict__ grad_out,
+    double *__restrict__ probs_out, int stage_ops)
 {
     extern __shared__ uint64_t s_adj[];
     const AdjLds L = adj_lds(stage_ops ? nops : 0, nparams, m, n, ntargets, X.total);
@@ -213,6 +215,7 @@ __global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
     cd *amp = reinterpret_cast<cd *>(base + L.amp);
     cd *cof = reinterpret_cast<cd *>(base + L.cof);
     double *tn = reinterpret_cast<double *>(base + L.tn);
+    double *tw = reinterpret_cast<double *>(base + L.tw);
     double *red = reinterpret_cast<double *>(base + L.red);
     int32_t *idx = reinterpret_cast<int32_t *>(base + L.idx);
     const int32_t *lop = idx + X.lop, *lstart = idx + X.lstart, *rstart = idx + X.rstart, *rlist = idx + X.rlist;
@@ -223,7 +226,10 @@ __global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
         for (int e = tid; e < nops * (int)(sizeof(MeshOp) / 8); e += ADJ_THREADS)
             reinterpret_cast<uint64_t *>(ops_l)[e] = reinterpret_cast<const uint64_t *>(ops_g)[e];
     for (int e = tid; e < X.total; e += ADJ_THREADS) idx[e] = idx_g[e];
-    for (int e = tid; e < ntargets; e += ADJ_THREADS) tn[e] = tn_g[e];
+    for (int e = tid; e < ntargets; e += ADJ_THREADS) {
+        tn[e] = tn_g[e];
+        tw[e] = tw_g[e];
+    }
     __syncthreads();
     const bool w0 = tid < 64;
     const int glog = X.glog, slots = 64 >> glog;
@@ -268,7 +274,7 @@ __global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
             const cd a = amp[tid];
             const double pr = a.r * a.r + a.i * a.i;
             if (probs_out) probs_out[(size_t)set * ntargets + tid] = pr;
-            red[tid] = pr;
+            red[tid] = tw[tid] * pr;
         }
         __syncthreads();
         if (tid == 0) {
@@ -277,7 +283,7 @@ __global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
             loss_out[set] = l;
         }
         // G^T into Z: Z[j][i] = G[i][j] = sum over (s, r) with rows_s[r] = i (increasing s, r) of
-        // conj(A_s) sum over c with in[c] = j of cof[s][r][c]
+        // w_s conj(A_s) sum over c with in[c] = j of cof[s][r][c]
         for (int e = tid; e < m * m; e += ADJ_THREADS) {
             const int j = e / m, i = e - (e / m) * m;
             cd g = {0.0, 0.0};
@@ -285,7 +291,9 @@ __global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
             if (c0 < c1)
                 for (int u = rstart[i]; u < rstart[i + 1]; u++) {
                     const int sr = rlist[u];
-                    const cd ca = cconj(amp[sr / n]);
+                    const cd a = amp[sr / n];
+                    const double ws = tw[sr / n];
+                    const cd ca = {ws * a.r, -(ws * a.i)};
                     for (int v = c0; v < c1; v++) cfma_acc(g, ca, cof[(size_t)sr * n + clist[v]]);
                 }
             Z[j * m + i] = g;
@@ -356,6 +364,7 @@ struct pcvl_mesh_plan_s {
     const MeshOp *d_ops = nullptr;
     const int32_t *d_idx = nullptr;
     const double *d_tn = nullptr;
+    double *d_tw = nullptr;           // per-target weights of the loss (ones unless set)
     int grid_cap = 0;
 };
 
@@ -472,8 +481,8 @@ extern "C" int pcvl_mesh_plan_create(const int32_t *h_kind, const int32_t *h_mod
     plan->lds = L.bytes;
     // one device block: program, index tables, norms
     const size_t b_ops = adj_al16(sizeof(MeshOp) * std::max(1, nops));
-    const size_t b_idx = adj_al16(sizeof(int32_t) * idx.size()), b_n = sizeof(double) * ntargets;
-    hipError_t e = hipMalloc((void **)&plan->d_blk, b_ops + b_idx + b_n);
+    const size_t b_idx = adj_al16(sizeof(int32_t) * idx.size()), b_n = adj_al16(sizeof(double) * ntargets);
+    hipError_t e = hipMalloc((void **)&plan->d_blk, b_ops + b_idx + 2 * b_n);
     if (e != hipSuccess) {
         delete plan;
         set_error(std::string("hipMalloc(mesh plan): ") + hipGetErrorString(e));
@@ -482,9 +491,13 @@ extern "C" int pcvl_mesh_plan_create(const int32_t *h_kind, const int32_t *h_mod
     plan->d_ops = reinterpret_cast<const MeshOp *>(plan->d_blk);
     plan->d_idx = reinterpret_cast<const int32_t *>(plan->d_blk + b_ops);
     plan->d_tn = reinterpret_cast<const double *>(plan->d_blk + b_ops + b_idx);
+    plan->d_tw = reinterpret_cast<double *>(plan->d_blk + b_ops + b_idx + b_n);
+    const std::vector<double> ones((size_t)ntargets, 1.0);
     if (nops) e = hipMemcpy(plan->d_blk, ops.data(), sizeof(MeshOp) * nops, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(plan->d_blk + b_ops, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(plan->d_blk + b_ops + b_idx, tnorm.data(), b_n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(plan->d_blk + b_ops + b_idx, tnorm.data(), sizeof(double) * ntargets,
+                                       hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(plan->d_tw, ones.data(), sizeof(double) * ntargets, hipMemcpyHostToDevice);
     int dev = 0, cus = 0;
     if (e == hipSuccess) e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -509,9 +522,22 @@ extern "C" int pcvl_mesh_plan_execute(pcvl_mesh_plan plan, const double *d_phase
     const pcvl_mesh_plan_s &P = *plan;
     const int grid = std::min(nsets, P.grid_cap);
     hipLaunchKernelGGL(mesh_adjoint_kernel, dim3(grid), dim3(ADJ_THREADS), P.lds, stream, P.d_ops, P.nops, P.m,
-                       d_phases, P.nparams, nsets, P.n, P.d_idx, P.X, P.d_tn, P.ntargets, d_loss, d_grad, d_probs,
-                       P.stage_ops ? 1 : 0);
+                       d_phases, P.nparams, nsets, P.n, P.d_idx, P.X, P.d_tn, P.d_tw, P.ntargets, d_loss, d_grad,
+                       d_probs, P.stage_ops ? 1 : 0);
     PCVL_HIP_CHECK(hipGetLastError());
+    return OK;
+}
+
+extern "C" int pcvl_mesh_plan_set_weights(pcvl_mesh_plan plan, const double *h_weights)
+{
+    PCVL_ARG_CHECK(plan != nullptr, "plan is null");
+    std::vector<double> w((size_t)plan->ntargets, 1.0);
+    if (h_weights)
+        for (int s = 0; s < plan->ntargets; s++) {
+            PCVL_ARG_CHECK(std::isfinite(h_weights[s]), "target weights must be finite");
+            w[(size_t)s] = h_weights[s];
+        }
+    PCVL_HIP_CHECK(hipMemcpy(plan->d_tw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
     return OK;
 }
 

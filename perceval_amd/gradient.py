@@ -111,6 +111,20 @@ class MeshPlan:
             program.const.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), program.nops, program.m,
             program.nparams, self.in_occ.ctypes.data_as(i32), self.targets.ctypes.data_as(i32), self.ntargets,
             ctypes.byref(self._plan)), "pcvl_mesh_plan_create")
+        self.weights = None
+
+    def set_weights(self, weights=None):
+        """Per-target weights w_s of the loss sum_s w_s P(s) (None: all 1)."""
+        if weights is None:
+            self.weights = None
+            _native.check(self._lib.pcvl_mesh_plan_set_weights(self._plan, None), "pcvl_mesh_plan_set_weights")
+            return
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).reshape(-1))
+        if w.size != self.ntargets:
+            raise ValueError(f"expected {self.ntargets} target weights, got {w.size}")
+        _native.check(self._lib.pcvl_mesh_plan_set_weights(self._plan, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                      "pcvl_mesh_plan_set_weights")
+        self.weights = w
 
     def __call__(self, phases, want_probs: bool = False, stream=None):
         import torch
@@ -141,36 +155,62 @@ class MeshPlan:
             pass
 
 
-def mesh_adjoint(program: MeshProgram, phases, in_occ, targets, want_probs: bool = False):
+MAX_PLAN_TARGETS = 64   # targets of one kernel plan; more are split over several plans
+
+
+def mesh_adjoint(program: MeshProgram, phases, in_occ, targets, want_probs: bool = False, weights=None):
     """Run the kernel.  phases: (B, P) float64 CUDA tensor or numpy.
-    Returns (loss (B,), grad (B, P)[, probs (B, T)]) as CUDA tensors.  The
-    program keeps one MeshPlan per (input state, targets)."""
+    Returns (loss (B,), grad (B, P)[, probs (B, T)]) as CUDA tensors, with
+    loss = sum_s w_s P(s) over the targets (weights None: all w_s = 1), e.g.
+    the expectation of a diagonal observable over every output state.  More
+    than 64 targets are split over several plans, whose losses and gradients
+    add (the gradient is linear in the targets).  The program keeps one
+    MeshPlan per (input state, targets)."""
+    import torch
     in_occ = tuple(int(v) for v in in_occ)
     targets = np.ascontiguousarray(np.asarray([list(t) for t in targets], dtype=np.int32))
-    key = (in_occ, targets.shape, targets.tobytes())
+    w = None if weights is None else np.asarray(weights, dtype=np.float64).reshape(-1)
+    if w is not None and w.size != len(targets):
+        raise ValueError(f"expected {len(targets)} target weights, got {w.size}")
     plans = program.__dict__.setdefault("_plans", {})
-    plan = plans.get(key)
-    if plan is None:
-        plan = plans[key] = MeshPlan(program, in_occ, targets)
-    return plan(phases, want_probs)
+    loss = grad = None
+    probs = []
+    for t0 in range(0, max(1, len(targets)), MAX_PLAN_TARGETS):
+        part = targets[t0:t0 + MAX_PLAN_TARGETS]
+        key = (in_occ, part.shape, part.tobytes())
+        plan = plans.get(key)
+        if plan is None:
+            plan = plans[key] = MeshPlan(program, in_occ, part)
+        wp = None if w is None else w[t0:t0 + MAX_PLAN_TARGETS]
+        if not (wp is None and plan.weights is None) and not (wp is not None and plan.weights is not None
+                                                              and np.array_equal(wp, plan.weights)):
+            plan.set_weights(wp)
+        res = plan(phases, want_probs)
+        loss = res[0] if loss is None else loss + res[0]
+        grad = res[1] if grad is None else grad + res[1]
+        if want_probs:
+            probs.append(res[2])
+    if want_probs:
+        return loss, grad, torch.cat(probs, dim=1)
+    return loss, grad
 
 
 def _autograd_function():
     import torch
 
     class TargetProbability(torch.autograd.Function):
-        """loss[b] = sum_{s in targets} P_b(s); backward from the adjoint kernel."""
+        """loss[b] = sum_{s in targets} w_s P_b(s); backward from the adjoint kernel."""
 
         @staticmethod
-        def forward(ctx, phases, program, in_occ, targets):
-            loss, grad = mesh_adjoint(program, phases.detach(), in_occ, targets)
+        def forward(ctx, phases, program, in_occ, targets, weights):
+            loss, grad = mesh_adjoint(program, phases.detach(), in_occ, targets, weights=weights)
             ctx.save_for_backward(grad)
             return loss
 
         @staticmethod
         def backward(ctx, grad_loss):
             (grad,) = ctx.saved_tensors
-            return grad_loss[:, None] * grad, None, None, None
+            return grad_loss[:, None] * grad, None, None, None, None
 
     return TargetProbability
 
@@ -178,9 +218,10 @@ def _autograd_function():
 _TP = None
 
 
-def target_probability(phases, program: MeshProgram, in_occ, targets):
-    """Differentiable summed target probability for a batch of phase vectors."""
+def target_probability(phases, program: MeshProgram, in_occ, targets, weights=None):
+    """Differentiable sum_s w_s P(s) over the targets (weights None: all 1)
+    for a batch of phase vectors."""
     global _TP
     if _TP is None:
         _TP = _autograd_function()
-    return _TP.apply(phases, program, in_occ, targets)
+    return _TP.apply(phases, program, in_occ, targets, weights)

@@ -156,3 +156,48 @@ def test_one_shot_entry_matches_the_plan(gpu):
             _native.ptr(grad), _native.ptr(probs), _native.stream_handle()), "pcvl_mesh_adjoint")
         assert torch.equal(loss, loss_p) and torch.equal(grad, grad_p) and torch.equal(probs, probs_p)
     plan.close()
+
+
+def test_weighted_observable_over_every_output(gpu):
+    """The objective of benchmark/benchmark_QML-DE-solver.py:61-84 on a phase
+    mesh: 6 photons in 6 modes, every one of the 462 output states weighted by
+    a seeded lambda (standardised and scaled by 200 as there), loss =
+    sum_s lambda_s P(s).  The 462 targets run as 8 plans of at most 64; the
+    loss matches the oracle's SLOS probabilities, the gradient central
+    differences of that loss (h = 1e-6), and the unweighted loss is the total
+    probability 1."""
+    import torch
+    from perceval_amd.gradient import MeshProgram, mesh_adjoint, target_probability
+    m = n = 6
+    prog = MeshProgram(clements_mesh(m))
+    rng = np.random.RandomState(0)
+    targets = oracle.fock_enumerate(m, n)
+    assert len(targets) == 462
+    lam = rng.rand(len(targets))
+    lam = 200 * (lam - lam.mean()) / lam.std()
+    phases = rng.uniform(0, 2 * math.pi, size=(4, prog.nparams))
+    inp = [1] * n
+    loss, grad, probs = mesh_adjoint(prog, phases, inp, targets, want_probs=True, weights=lam)
+    loss, grad, probs = loss.cpu().numpy(), grad.cpu().numpy(), probs.cpu().numpy()
+
+    def f(ph):
+        return float(np.dot(lam, oracle.slos_probs(prog.unitary(ph), inp)))
+
+    for b in range(4):
+        ref = oracle.slos_probs(prog.unitary(phases[b]), inp)
+        assert np.max(np.abs(probs[b] - ref)) < 1e-13
+        assert abs(loss[b] - f(phases[b])) < 1e-10
+    h = 1e-6
+    for b in (0, 3):
+        fd = np.empty(prog.nparams)
+        for k in range(prog.nparams):
+            pp = phases[b].copy(); pp[k] += h
+            pm = phases[b].copy(); pm[k] -= h
+            fd[k] = (f(pp) - f(pm)) / (2 * h)
+        assert np.max(np.abs(grad[b] - fd)) < 1e-6 * max(1.0, np.max(np.abs(fd))), np.max(np.abs(grad[b] - fd))
+    ones, gones = mesh_adjoint(prog, phases, inp, targets)
+    assert np.max(np.abs(ones.cpu().numpy() - 1.0)) < 1e-12
+    assert np.max(np.abs(gones.cpu().numpy())) < 1e-11          # the total probability does not move
+    x = torch.from_numpy(phases).cuda().requires_grad_(True)
+    target_probability(x, prog, inp, targets, lam).sum().backward()
+    assert np.max(np.abs(x.grad.cpu().numpy() - grad)) < 1e-12
