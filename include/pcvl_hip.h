@@ -152,7 +152,8 @@ int pcvl_gather_f64(const double *d_src, const int64_t *d_idx, int64_t count, do
 
 typedef struct pcvl_slos_plan_s *pcvl_slos_plan;
 
-/* m in [1, 64], at most 40 photons, every layer below 2^40 states. */
+/* m in [1, 256], at most 40 photons, every layer below 2^40 states (the rank
+ * tables of the per-state kernel, 16 m (n + 1) bytes, within 160 KiB). */
 int pcvl_slos_plan_create(int m, const int32_t *h_in_occ, pcvl_slos_plan *out);
 
 /* Photon number, output count, workspace bytes needed by execute, and the

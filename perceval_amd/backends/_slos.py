@@ -194,7 +194,7 @@ class SLOSBackend(AStrongSimulationBackend):
             try:
                 self._plans_masked[key] = SlosPlan(list(istate), mask_cond=cond, at_least=at_least, mask_n=mask_n)
             except NativeError as e:
-                if "herald configurations" not in str(e) and "2^32" not in str(e):
+                if "failed (-4)" not in str(e):   # out of range for a masked plan: select instead
                     raise
                 self._plans_masked[key] = None   # too many constrained configurations: select instead
         return self._plans_masked[key]

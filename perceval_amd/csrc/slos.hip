@@ -31,7 +31,7 @@
 
 namespace pcvl {
 
-constexpr int SLOS_MAX_MODES = 64;
+constexpr int SLOS_MAX_MODES = 256;
 constexpr int SLOS_MAX_PHOTONS = 40;
 constexpr int SLOS_THREADS = 256;
 
@@ -622,9 +622,11 @@ __global__ void __launch_bounds__(MASK_THREADS) mask_count_kernel(uint64_t total
                                                                   uint8_t *__restrict__ flags,
                                                                   int32_t *__restrict__ counts)
 {
-    __shared__ uint64_t sT[SLOS_MAX_MODES * 64];
-    __shared__ int32_t scond[MASK_MAX * SLOS_MAX_MODES];
-    __shared__ int32_t sat[SLOS_MAX_MODES];
+    // rank table (m x w), conditions (nmask x m) and floors (m), sized per launch
+    extern __shared__ uint64_t s_mask[];
+    uint64_t *sT = s_mask;
+    int32_t *scond = reinterpret_cast<int32_t *>(sT + (size_t)m * w);
+    int32_t *sat = scond + (size_t)nmask * m;
     for (int e = threadIdx.x; e < m * w; e += MASK_THREADS) sT[e] = gT[e];
     for (int e = threadIdx.x; e < nmask * m; e += MASK_THREADS) scond[e] = gcond[e];
     for (int e = threadIdx.x; e < m; e += MASK_THREADS) sat[e] = gat[e];
@@ -774,7 +776,7 @@ extern "C" int pcvl_slos_plan_create(int m, const int32_t *h_in_occ, pcvl_slos_p
 {
     PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
     PCVL_ARG_CHECK(h_in_occ != nullptr, "input state is null");
-    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 64]");
+    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 256]");
     *out = nullptr;
     int n = 0;
     for (int i = 0; i < m; i++) {
@@ -826,8 +828,17 @@ extern "C" int pcvl_slos_plan_create(int m, const int32_t *h_in_occ, pcvl_slos_p
         return ERR_HIP;
     }
     int rc = device_grid_cap(&p.grid_cap);
-    if (rc == OK && blk_layout_ok(m, n)) rc = blk_layout_create(m, n, blk_default_width(m, n), &p.blk);
-    if (rc != OK) { (void)hipFree(p.d_T); delete plan; return rc; }
+    // the block kernel where its layout fits (a layout out of range leaves the others)
+    if (rc == OK && blk_layout_ok(m, n)) {
+        rc = blk_layout_create(m, n, blk_default_width(m, n), &p.blk);
+        if (rc == ERR_RANGE) rc = OK;
+    }
+    // the per-state kernel, which every other falls back to, stages its rank tables in LDS
+    if (rc == OK && !blk_ok(p) && 2 * sizeof(uint64_t) * (size_t)m * (size_t)p.w > 160 * 1024) {
+        set_error("SLOS: modes and photons too many for any layer kernel's tables");
+        rc = ERR_RANGE;
+    }
+    if (rc != OK) { blk_layout_destroy(&p.blk); (void)hipFree(p.d_T); delete plan; return rc; }
     *out = plan;
     return OK;
 }
@@ -843,7 +854,7 @@ extern "C" int pcvl_slos_plan_create_masks(int m, const int32_t *h_in_occ, int n
 {
     PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
     PCVL_ARG_CHECK(h_in_occ != nullptr && h_conds != nullptr, "input state or mask is null");
-    PCVL_ARG_CHECK(m >= 2 && m <= SLOS_MAX_MODES, "mode count outside [2, 64]");
+    PCVL_ARG_CHECK(m >= 2 && m <= SLOS_MAX_MODES, "mode count outside [2, 256]");
     PCVL_ARG_CHECK(nrows >= 1 && nrows <= 64, "mask row count outside [1, 64]");
     *out = nullptr;
     int n = 0, nfree = 0;
@@ -1172,7 +1183,7 @@ extern "C" int pcvl_slos_tree_create(int m, const int32_t *h_inputs, int count, 
 {
     PCVL_ARG_CHECK(out != nullptr, "out plan pointer is null");
     PCVL_ARG_CHECK(count >= 1 && h_inputs != nullptr, "no input states");
-    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 64]");
+    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 256]");
     *out = nullptr;
     auto *plan = new (std::nothrow) pcvl_slos_tree_s();
     if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
@@ -1237,8 +1248,15 @@ extern "C" int pcvl_slos_tree_create(int m, const int32_t *h_inputs, int count, 
         return ERR_HIP;
     }
     int rc = device_grid_cap(&t.grid_cap);
-    if (rc == OK && blk_layout_ok(m, t.nmax)) rc = blk_layout_create(m, t.nmax, blk_default_width(m, t.nmax), &t.blk);
-    if (rc != OK) { (void)hipFree(t.d_T); delete plan; return rc; }
+    if (rc == OK && blk_layout_ok(m, t.nmax)) {
+        rc = blk_layout_create(m, t.nmax, blk_default_width(m, t.nmax), &t.blk);
+        if (rc == ERR_RANGE) rc = OK;   // the tree's per-state kernel then runs every layer
+    }
+    if (rc == OK && !t.blk.d_mem && 2 * sizeof(uint64_t) * (size_t)m * (size_t)t.w > 160 * 1024) {
+        set_error("SLOS tree: modes and photons too many for the per-state kernel's tables");
+        rc = ERR_RANGE;
+    }
+    if (rc != OK) { blk_layout_destroy(&t.blk); (void)hipFree(t.d_T); delete plan; return rc; }
     *out = plan;
     return OK;
 }
@@ -1347,7 +1365,7 @@ static int make_rank_tables(int m, int nmax, RankTables *rt)
 
 extern "C" int pcvl_fock_rank(const int32_t *d_occ, int64_t count, int m, int nmax, int64_t *d_out, hipStream_t stream)
 {
-    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 64]");
+    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 256]");
     PCVL_ARG_CHECK(nmax >= 0 && nmax <= 255, "nmax outside [0, 255]");
     PCVL_ARG_CHECK(count >= 0, "count < 0");
     if (count == 0) return OK;
@@ -1366,7 +1384,7 @@ extern "C" int pcvl_fock_rank(const int32_t *d_occ, int64_t count, int m, int nm
 
 extern "C" int pcvl_fock_unrank(const int64_t *d_rank, int64_t count, int m, int n, int32_t *d_occ, hipStream_t stream)
 {
-    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 64]");
+    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 256]");
     PCVL_ARG_CHECK(n >= 0 && n <= 255, "photon count outside [0, 255]");
     PCVL_ARG_CHECK(count >= 0, "count < 0");
     if (count == 0) return OK;
@@ -1395,11 +1413,13 @@ extern "C" int64_t pcvl_fock_count(int m, int n)
 extern "C" int pcvl_fock_mask_select(int m, int n, int mask_n, const int32_t *h_cond, int nmask,
                                      const int32_t *h_at_least, int64_t *d_out, int64_t *d_count, hipStream_t stream)
 {
-    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 64]");
+    PCVL_ARG_CHECK(m >= 1 && m <= SLOS_MAX_MODES, "mode count outside [1, 256]");
     PCVL_ARG_CHECK(n >= 0 && n <= 63, "photon count outside [0, 63]");
     PCVL_ARG_CHECK(mask_n >= 0 && mask_n <= 1024, "mask photon count outside [0, 1024]");
     const int spare = mask_n > n ? mask_n - n : 0;
     PCVL_ARG_CHECK(nmask >= 1 && nmask <= MASK_MAX, "between 1 and 16 masks");
+    const size_t mask_lds = sizeof(uint64_t) * (size_t)m * (size_t)(n + 1) + sizeof(int32_t) * (size_t)(nmask + 1) * m;
+    PCVL_ARG_CHECK(mask_lds <= 64 * 1024, "mode and photon counts too large for the selection kernel's tables");
     PCVL_ARG_CHECK(h_cond && d_out && d_count, "null buffer");
     const uint64_t total = binom_u64((int64_t)m + n - 1, n);
     PCVL_ARG_CHECK(total != UINT64_MAX && total < (1ull << 34), "Fock space too large (2^34 states or more)");
@@ -1433,8 +1453,8 @@ extern "C" int pcvl_fock_mask_select(int m, int n, int mask_n, const int32_t *h_
     if (e == hipSuccess) e = hipMemcpyAsync(dc, h_cond, sizeof(int32_t) * (size_t)nmask * m, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(da, at.data(), sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(mask_count_kernel, dim3((unsigned)nb), dim3(MASK_THREADS), 0, stream, total, m, n, dT, w, dc,
-                           nmask, da, spare, absorbs, dfl, dcnt);
+        hipLaunchKernelGGL(mask_count_kernel, dim3((unsigned)nb), dim3(MASK_THREADS), mask_lds, stream, total, m, n, dT,
+                           w, dc, nmask, da, spare, absorbs, dfl, dcnt);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {

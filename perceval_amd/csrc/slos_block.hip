@@ -438,6 +438,8 @@ static uint64_t local_rank(const std::vector<int> &beta)
     return rk;
 }
 
+static size_t blk_lds_bytes(const BlkLayout &lay, int k);
+
 int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vector<BlkSub>> &subs,
                              const std::vector<uint64_t> &counts, const std::vector<int32_t> &rowmap,
                              BlkLayout *out, std::vector<char> *image, size_t max_records)
@@ -646,6 +648,10 @@ int blk_layout_build_host_ex(int m, int nmax, int w, const std::vector<std::vect
     lay.nrec = recs.size();
     lay.off_rec = off; off += al(sizeof(BlkRec) * std::max<size_t>(recs.size(), 1));
     lay.bytes = off;
+    if (blk_lds_bytes(lay, lay.nlayers) > 160 * 1024) {
+        set_error("block SLOS layout: the walk tables and lists outgrow the LDS (too many modes and photons)");
+        return ERR_RANGE;
+    }
     std::vector<char> &host = *image;
     host.assign(off, 0);
     std::memcpy(host.data() + lay.off_th, th.data(), sizeof(uint32_t) * th.size());

@@ -52,6 +52,20 @@ def _rank(state) -> int:
     return r
 
 
+class _LazyPlan:
+    """A single-input SLOS plan built on first use: under a mask all_prob
+    usually runs a masked plan, and the full plan is then never built."""
+
+    def __init__(self, occ):
+        self._occ = list(occ)
+        self._plan = None
+
+    def run(self, *args, **kwargs):
+        if self._plan is None:
+            self._plan = _slos.SlosPlan(self._occ)
+        return self._plan.run(*args, **kwargs)
+
+
 def _default_types():
     import perceval.utils as pu
     return _types.SimpleNamespace(FockState=pu.FockState, BSDistribution=pu.BSDistribution,
@@ -135,7 +149,8 @@ def make_backends(backends_module, types=None) -> dict:
                 for k in range(len(self._fsms), s.n + 1):
                     self._fsms.append(("layer", m, k))
             if len(new) == 1:
-                self._plans[new[0]] = _slos.SlosPlan(list(new[0]))
+                self._plans[new[0]] = (_slos.SlosPlan(list(new[0])) if self._masks_str is None
+                                       else _LazyPlan(new[0]))
             else:
                 tree = _slos.SlosTree([list(s) for s in new])
                 for i, s in enumerate(new):
@@ -201,7 +216,7 @@ def make_backends(backends_module, types=None) -> dict:
                         plan = _slos.SlosPlan(list(istate), mask_cond=cond, at_least=at_least, mask_n=mask_n)
                         self._results[key] = plan.run(self._umat, _slos.PROBS).cpu().numpy()
                     except _native.NativeError as e:
-                        if "herald configurations" not in str(e) and "2^32" not in str(e):
+                        if "failed (-4)" not in str(e):   # out of range for a masked plan: select instead
                             raise
                         self._results[key] = None   # too many constrained configurations: select instead
                 if self._results[key] is not None:

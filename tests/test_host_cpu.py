@@ -191,7 +191,7 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     cond = (ctypes.c_int32 * 4)(-1, 0, 1, -1)
     out = ctypes.c_void_p(1)
     assert "masks" in err(lib.pcvl_fock_mask_select(4, 2, 0, cond, 0, None, out, out, None))
-    assert "mode count" in err(lib.pcvl_fock_mask_select(65, 2, 0, cond, 1, None, out, out, None))
+    assert "mode count" in err(lib.pcvl_fock_mask_select(257, 2, 0, cond, 1, None, out, out, None))
     assert "photon count" in err(lib.pcvl_fock_mask_select(4, 64, 0, cond, 1, None, out, out, None))
     assert "mask photon count" in err(lib.pcvl_fock_mask_select(4, 2, -1, cond, 1, None, out, out, None))
     occ = (ctypes.c_int32 * 4)(1, 1, 0, 0)

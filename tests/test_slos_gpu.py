@@ -257,6 +257,65 @@ def test_slos_m32_n10_a_billion_outputs(gpu):
     torch.cuda.empty_cache()
 
 
+def _perm3(M):
+    """Permanent of a 3 x 3 matrix, the six products written out."""
+    return (M[0, 0] * (M[1, 1] * M[2, 2] + M[1, 2] * M[2, 1]) + M[0, 1] * (M[1, 0] * M[2, 2] + M[1, 2] * M[2, 0])
+            + M[0, 2] * (M[1, 0] * M[2, 1] + M[1, 1] * M[2, 0]))
+
+
+def test_slos_wide_circuits(gpu):
+    """Circuits wider than 64 modes with few photons: m=128, n=3 (357,760
+    outputs) against the oracle's full distribution; m=100, n=2 against the
+    two-photon closed form, every output; m=256, n=3 (2,829,056 outputs)
+    summing to 1 with 64 outputs against 3 x 3 permanents written out; a
+    herald mask at m=100 (masked plan and selection)."""
+    import torch
+    from perceval_amd.slos import PROBS, SlosPlan, fock_mask_select, fock_unrank
+    U = haar_unitary(128, seed=128)
+    inp = [1, 0, 1, 1] + [0] * 124
+    p = SlosPlan(inp).run(U, PROBS).cpu().numpy()
+    assert p.size == math.comb(130, 3)
+    assert np.max(np.abs(p - oracle.slos_probs(U, inp))) < ABS_TOL
+
+    U = haar_unitary(100, seed=100)
+    inp = [0] * 100
+    inp[7] = inp[42] = 1
+    p = SlosPlan(inp).run(U, PROBS).cpu().numpy()
+    states = oracle.fock_enumerate(100, 2)
+    ref = np.empty(len(states))
+    for k, st in enumerate(states):
+        modes = np.repeat(np.arange(100), st)
+        a, b = modes
+        amp = U[a, 7] * U[b, 42] + U[a, 42] * U[b, 7]
+        ref[k] = abs(amp) ** 2 / (2 if a == b else 1)
+    assert np.max(np.abs(p - ref)) < ABS_TOL
+    # heralds: mode 99 empty, mode 98 one photon
+    cond = [-1] * 98 + [1, 0]
+    masked = SlosPlan(inp, mask_cond=cond).run(U, PROBS).cpu().numpy()
+    keep = (states[:, 98] == 1) & (states[:, 99] == 0)
+    assert np.max(np.abs(masked - ref[keep])) < ABS_TOL
+    idx = fock_mask_select(100, 2, [" " * 98 + "10"]).cpu().numpy()
+    assert np.array_equal(idx, np.flatnonzero(keep))
+
+    U = haar_unitary(256, seed=256)
+    inp = [0] * 256
+    inp[0] = inp[100] = inp[255] = 1
+    plan = SlosPlan(inp)
+    pt = plan.run(U, PROBS)
+    assert pt.numel() == math.comb(258, 3)
+    assert abs(float(pt.sum().item()) - 1.0) < 1e-10
+    rng = np.random.RandomState(5)
+    ranks = np.concatenate([[0, pt.numel() - 1], rng.randint(0, pt.numel(), 62)])
+    got = pt[torch.from_numpy(ranks).cuda()].cpu().numpy()
+    occ = fock_unrank(ranks, 256, 3)
+    for g, st in zip(got, occ):
+        rows = np.repeat(np.arange(256), st)
+        M = U[np.ix_(rows, [0, 100, 255])]
+        ref = abs(_perm3(M)) ** 2 / np.prod([math.factorial(v) for v in st])
+        assert abs(g - ref) < ABS_TOL
+    plan.close()
+
+
 def test_slos_vacuum_and_one_photon(gpu):
     from perceval_amd.slos import COEFS, PROBS, SlosPlan
     U = haar_unitary(5, seed=3)
