@@ -27,6 +27,7 @@
 namespace pcvl {
 
 constexpr int GURVITS_MAX_N = 48;
+constexpr int GURVITS_WAVES = 4;   // waves per workgroup of the matrix-core kernel
 
 struct GurvitsItem {
     int64_t off;   // complex offset of the matrix
@@ -132,15 +133,18 @@ __global__ void __launch_bounds__(64) gurvits_kernel(const cd *__restrict__ mats
 typedef double gv_d4 __attribute__((ext_vector_type(4)));
 
 template <int NP>
-__global__ void __launch_bounds__(64) gurvits_mfma_kernel(const cd *__restrict__ mats,
+__global__ void __launch_bounds__(64 * GURVITS_WAVES, NP <= 32 ? 3 : 1) gurvits_mfma_kernel(const cd *__restrict__ mats,
                                                           const GurvitsItem *__restrict__ items, int64_t count,
                                                           int parts, int64_t iters, int64_t per, uint64_t seed,
                                                           uint64_t stream0, double *__restrict__ partial)
 {
     constexpr int RB = NP / 16, KB = NP / 4;
-    constexpr bool AREG = NP <= 32;
+    // A is read from LDS at each product: the registers it would take cost a
+    // wave per SIMD, and the GURVITS_WAVES waves of a workgroup share one copy
+    constexpr bool AREG = false;
     __shared__ cd sA[NP * NP];
-    const int lane = threadIdx.x;
+    __shared__ double sred[GURVITS_WAVES][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, quad = lane >> 4;
     const int64_t nwork = count * parts;
     for (int64_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
@@ -148,7 +152,7 @@ __global__ void __launch_bounds__(64) gurvits_mfma_kernel(const cd *__restrict__
         const int64_t q = wi % parts;
         const int n = __builtin_amdgcn_readfirstlane(items[b].n);
         const int64_t off = items[b].off;
-        for (int e = lane; e < NP * NP; e += 64) {
+        for (int e = threadIdx.x; e < NP * NP; e += 64 * GURVITS_WAVES) {
             const int i = e / NP, j = e % NP;
             sA[e] = (i < n && j < n) ? mats[off + (int64_t)i * n + j] : cd{0.0, 0.0};
         }
@@ -169,7 +173,7 @@ __global__ void __launch_bounds__(64) gurvits_mfma_kernel(const cd *__restrict__
         const int64_t s_hi = (s_lo + per < iters) ? s_lo + per : iters;
         cd sum = {0.0, 0.0};
         double sq = 0.0;
-        for (int64_t s0 = s_lo; s0 < s_hi; s0 += 16) {
+        for (int64_t s0 = s_lo + 16 * wave; s0 < s_hi; s0 += 16 * GURVITS_WAVES) {
             const int64_t s = s0 + col;
             const bool valid = s < s_hi;
             const uint64_t bits = valid ? rand64(seed, stream0 + (uint64_t)b, (uint64_t)s) : 0ull;
@@ -222,9 +226,21 @@ __global__ void __launch_bounds__(64) gurvits_mfma_kernel(const cd *__restrict__
             sq += __shfl_xor(sq, o, 64);
         }
         if (lane == 0) {
-            partial[3 * wi] = sum.r;
-            partial[3 * wi + 1] = sum.i;
-            partial[3 * wi + 2] = sq;
+            sred[wave][0] = sum.r;
+            sred[wave][1] = sum.i;
+            sred[wave][2] = sq;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {   // the waves' sums in wave order
+            double r = 0.0, im = 0.0, q2 = 0.0;
+            for (int w = 0; w < GURVITS_WAVES; w++) {
+                r += sred[w][0];
+                im += sred[w][1];
+                q2 += sred[w][2];
+            }
+            partial[3 * wi] = r;
+            partial[3 * wi + 1] = im;
+            partial[3 * wi + 2] = q2;
         }
         __syncthreads();
     }
@@ -260,12 +276,13 @@ static int launch_gurvits(const cd *mats, const GurvitsItem *items, int64_t coun
     constexpr bool MF = NP >= 16 && NP % 16 == 0;
     const void *kp = MF ? reinterpret_cast<const void *>(&gurvits_mfma_kernel<MF ? NP : 16>)
                         : reinterpret_cast<const void *>(&gurvits_kernel<NP>);
-    PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, 64, 0));
+    const int threads = MF ? 64 * GURVITS_WAVES : 64;
+    PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, threads, 0));
     if (per_cu < 1) per_cu = 1;
     const int grid = (int)std::min<int64_t>(count * parts, (int64_t)cus * per_cu);
     if constexpr (MF)
-        hipLaunchKernelGGL((gurvits_mfma_kernel<NP>), dim3(grid), dim3(64), 0, st, mats, items, count, parts, iters,
-                           per, seed, stream0, partial);
+        hipLaunchKernelGGL((gurvits_mfma_kernel<NP>), dim3(grid), dim3(threads), 0, st, mats, items, count, parts,
+                           iters, per, seed, stream0, partial);
     else
         hipLaunchKernelGGL((gurvits_kernel<NP>), dim3(grid), dim3(64), 0, st, mats, items, count, parts, iters, per,
                            seed, stream0, partial);
