@@ -296,6 +296,13 @@ def test_slos_wide_circuits(gpu):
     assert np.max(np.abs(masked - ref[keep])) < ABS_TOL
     idx = fock_mask_select(100, 2, [" " * 98 + "10"]).cpu().numpy()
     assert np.array_equal(idx, np.flatnonzero(keep))
+    # a tree over two inputs of the wide circuit equals its single plans
+    from perceval_amd.slos import SlosTree
+    other = [0] * 100
+    other[7] = other[60] = other[99] = 1
+    outs = SlosTree([inp, other]).run(U, PROBS)
+    assert np.max(np.abs(outs[0].cpu().numpy() - p)) < 1e-13
+    assert np.max(np.abs(outs[1].cpu().numpy() - SlosPlan(other).run(U, PROBS).cpu().numpy())) < 1e-13
 
     U = haar_unitary(256, seed=256)
     inp = [0] * 256
