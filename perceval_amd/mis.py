@@ -222,9 +222,13 @@ def seq_to_occ(seq, m: int):
 
 class LearnedMIS:
     """Independence Metropolis chain with a ProposalNet proposal and FP64
-    permanent targets, for a fixed unitary and single-photon input."""
+    permanent targets, for a fixed unitary and single-photon input.  chunk:
+    proposals scored per round of launches (the chain's draws do not depend
+    on it; larger rounds leave fewer tails where the device drains: 10^5
+    samples at m=60, n=20 take 75.7 / 73.6 / 72.1 ms at 16,384 / 32,768 /
+    65,536)."""
 
-    def __init__(self, U, in_occ, net: ProposalNet, chunk: int = 16384, multiset: bool = True):
+    def __init__(self, U, in_occ, net: ProposalNet, chunk: int = 65536, multiset: bool = True):
         import torch
         self.in_occ = list(in_occ)
         self.m = len(self.in_occ)
