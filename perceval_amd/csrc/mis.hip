@@ -220,8 +220,8 @@ __device__ __forceinline__ void misp_pin(cd *h, const cd &part)
 // k_g equal rows (outer mixed-radix digits t_g in 0..k_g, weight
 // C(k_g, t_g)(-1)^t_g, column-sum offset (k_g - 2 t_g)/2 times the row).  Its
 // O * 2^nu terms are cut into units of 2^c Gray steps (units = O 2^(nu-c),
-// c <= 10 and, where possible, units >= 64), and the units into at most 64
-// slices.  Waves pull slices from one atomic cursor, so proposals of very
+// c <= 10, chosen per proposal so that the wave passes fill their lanes),
+// and the units into at most 64 slices.  Waves pull slices from one atomic cursor, so proposals of very
 // different cost (a collision-free proposal has up to 4000 times the terms
 // of a bunched one) balance over the device; each slice writes one partial,
 // and a finalize pass sums a proposal's partials in slice order
@@ -266,8 +266,26 @@ __global__ void mis_plan_kernel(const int32_t *__restrict__ seq, int64_t B, int 
         R[8 + MISP_MAX_N + ng++] = gcnt[g];
         O *= (uint64_t)(gcnt[g] + 1);
     }
-    int c = nu < MISP_MAX_C ? nu : MISP_MAX_C;
-    while (c > 0 && (O << (nu - c)) < 64) c--;
+    // Unit length 2^c: a wave runs its slice's units 64 at a time, so a unit
+    // count that is not a multiple of 64 idles lanes in the last pass, while
+    // short units repeat the column-sum set-up more often.  Take the c with
+    // the least estimated wave time, in per-lane FP64 operations: passes x
+    // (2^c Gray steps of 6n+2 operations + the set-up) + about 1000 per slice
+    // (record and rows into LDS, search, reduction).
+    const int cmax = nu < MISP_MAX_C ? nu : MISP_MAX_C;
+    const double step_ops = 6.0 * n + 2.0;
+    const double setup_ops = 2.0 * n * (2 + nu + ng) + 30.0 * ng;
+    int c = cmax;
+    double best = -1.0;
+    for (int cc = cmax; cc >= 0 && cc >= cmax - 5; cc--) {
+        const uint64_t un = O << (nu - cc);
+        const uint64_t sl_c = (un + 63) / 64 < (uint64_t)MISP_MAX_SLICES ? (un + 63) / 64 : (uint64_t)MISP_MAX_SLICES;
+        const uint64_t per = (un + sl_c - 1) / sl_c;
+        const uint64_t full = un / per, rest = un - full * per;
+        const uint64_t passes = full * ((per + 63) / 64) + (rest + 63) / 64;
+        const double est = (double)passes * ((double)(1ull << cc) * step_ops + setup_ops) + 1000.0 * (double)sl_c;
+        if (best < 0.0 || est < best) { best = est; c = cc; }
+    }
     const uint64_t units = O << (nu - c);
     const uint64_t sl = (units + 63) / 64;
     const int slices = (int)(sl < (uint64_t)MISP_MAX_SLICES ? sl : (uint64_t)MISP_MAX_SLICES);
