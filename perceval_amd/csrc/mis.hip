@@ -54,48 +54,139 @@ __global__ void mis_logp_kernel(const cd *__restrict__ perms, const int32_t *__r
     logp[b] = (a2 > 0.0 ? log(a2) : -INFINITY) - lf - log_prod_in;
 }
 
-// Accept/reject scan over B proposals, one wave.  state[t] = index of the
-// proposal the chain holds after step t, or -1 for the initial state.
-// Uniform of step t: stream (seed, first + t), counter 0.  Returns the number
-// of accepted moves in *accepted.
-__global__ void __launch_bounds__(64) mis_chain_kernel(const double *__restrict__ logw, int64_t B, double logw0,
-                                                       uint64_t seed, uint64_t first, int64_t *__restrict__ state,
-                                                       int64_t *__restrict__ accepted)
+// v of lane `lane` (wave-uniform) as a scalar: two v_readlane into SGPRs,
+// no LDS round trip as a shuffle by an arbitrary lane would take.
+__device__ __forceinline__ uint64_t mis_readlane_u64(uint64_t v, int lane)
 {
-    // One wave takes 64 steps at a time.  The log weights and log uniforms of
-    // the 64 steps are loaded and computed in parallel; the sequential part
-    // is one ballot per accepted move: every lane after the last acceptance
-    // tests its own step against the current state, and the first lane that
-    // accepts becomes the new state.  This is the step-by-step rule exactly:
-    // take step t iff lw_t - cur >= 0 or log u_t < lw_t - cur.
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double mis_readlane(double v, int lane)
+{
+    return __longlong_as_double((long long)mis_readlane_u64((uint64_t)__double_as_longlong(v), lane));
+}
+
+// ---- accept/reject scan ----------------------------------------------------
+//
+// The chain takes step t iff lw_t - cur >= 0 or log u_t < lw_t - cur, where
+// cur is the log weight of the state it holds, and after taking step t it
+// holds cur = lw_t.  Cut the steps into groups of 64.  Which steps of a group
+// the chain takes after holding step s of that group depends on s alone:
+// next(s) = the first later step of the group taken from cur = lw_s, and the
+// chain then takes next(s), next(next(s)), ... to the group's end.  So the
+// whole set, path(s), is computed for every step in parallel, and the
+// sequential scan does one ballot per group: the first step taken from the
+// state held on entry, a, and then path(a).  The decisions are the same
+// floating-point comparisons as a step-by-step scan, so the states are too.
+
+// One wave per group: lu[t] = log u_t (uniform of step t: stream
+// (seed, first + t), counter 0) and path[t] (bit u: step 64 g + u).
+__global__ void __launch_bounds__(64) mis_chain_path_kernel(const double *__restrict__ logw, int64_t B, uint64_t seed,
+                                                            uint64_t first, double *__restrict__ lu_out,
+                                                            uint64_t *__restrict__ path_out)
+{
+    const int64_t g = blockIdx.x;
+    const int s = threadIdx.x;
+    const int64_t t = g * 64 + s;
+    const bool valid = t < B;
+    const double lw = valid ? logw[t] : 0.0;
+    const double lu = valid ? log(uniform01(seed, first + (uint64_t)t, 0)) : 0.0;
+    const int nv = (int)(B - g * 64 < 64 ? B - g * 64 : 64);   // steps of this group
+    int nxt = 64;                                              // none
+    for (int u = 1; u < nv; u++) {
+        const double lwu = mis_readlane(lw, u), luu = mis_readlane(lu, u);
+        const double d = lwu - lw;
+        if (nxt == 64 && u > s && (d >= 0.0 || luu < d)) nxt = u;
+    }
+    // pointer doubling: after r rounds pm = {s, next(s), ..., next^(2^r - 1)(s)}
+    // and j = next^(2^r)(s); 6 rounds cover the 64 steps of a group.
+    uint64_t pm = 1ull << s;
+    int j = nxt;
+    for (int r = 0; r < 6; r++) {
+        const int src = j < 64 ? j : s;
+        const uint32_t plo = (uint32_t)__shfl((int)(uint32_t)pm, src, 64);
+        const uint32_t phi = (uint32_t)__shfl((int)(uint32_t)(pm >> 32), src, 64);
+        const int jj = __shfl(j, src, 64);
+        if (j < 64) {
+            pm |= ((uint64_t)phi << 32) | plo;
+            j = jj;
+        }
+    }
+    if (valid) {
+        lu_out[t] = lu;
+        path_out[t] = pm;
+    }
+}
+
+// The sequential scan, one workgroup.  Wave 0 walks the groups; the other
+// waves stage the next block of log weights, log uniforms and paths in LDS
+// (double buffered) meanwhile.  state[t] = index of the proposal the chain
+// holds after step t, or -1 for the initial state; *accepted = moves taken.
+constexpr int MIS_CHAIN_WAVES = 8;
+constexpr int MIS_CHAIN_BLOCK = 1024;   // steps per LDS buffer (a multiple of 64)
+
+__device__ __forceinline__ void mis_chain_stage(const double *__restrict__ logw, const double *__restrict__ lu_g,
+                                                const uint64_t *__restrict__ path_g, int64_t B, int64_t blk,
+                                                double *lw, double *lu, uint64_t *pm, int id, int nth)
+{
+    for (int i = id; i < MIS_CHAIN_BLOCK; i += nth) {
+        const int64_t t = blk * MIS_CHAIN_BLOCK + i;
+        const bool valid = t < B;
+        lw[i] = valid ? logw[t] : 0.0;
+        lu[i] = valid ? lu_g[t] : 0.0;
+        pm[i] = valid ? path_g[t] : 0ull;
+    }
+}
+
+__global__ void __launch_bounds__(64 * MIS_CHAIN_WAVES) mis_chain_kernel(const double *__restrict__ logw,
+                                                                         const double *__restrict__ lu_g,
+                                                                         const uint64_t *__restrict__ path_g, int64_t B,
+                                                                         double logw0, int64_t *__restrict__ state,
+                                                                         int64_t *__restrict__ accepted)
+{
+    __shared__ double s_lw[2][MIS_CHAIN_BLOCK], s_lu[2][MIS_CHAIN_BLOCK];
+    __shared__ uint64_t s_pm[2][MIS_CHAIN_BLOCK];
     if (blockIdx.x != 0) return;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t nblk = (B + MIS_CHAIN_BLOCK - 1) / MIS_CHAIN_BLOCK;
+    mis_chain_stage(logw, lu_g, path_g, B, 0, s_lw[0], s_lu[0], s_pm[0], tid, 64 * MIS_CHAIN_WAVES);
+    __syncthreads();
     double cur = logw0;
     int64_t idx = -1, acc = 0;
-    for (int64_t t0 = 0; t0 < B; t0 += 64) {
-        const int64_t t = t0 + lane;
-        const bool valid = t < B;
-        const double lw = valid ? logw[t] : 0.0;
-        const double lu = valid ? log(uniform01(seed, first + (uint64_t)t, 0)) : 0.0;
-        uint64_t taken = 0;      // steps of this group that accept
-        int after = -1;          // lanes <= after are decided
-        for (int guard = 0; guard < 64; guard++) {
-            const double d = lw - cur;
-            const bool take = valid && lane > after && (d >= 0.0 || lu < d);
-            const uint64_t mask = __ballot(take);
-            if (mask == 0) break;
-            const int f = __builtin_ctzll(mask);
-            taken |= 1ull << f;
-            cur = __shfl(lw, f, 64);
-            after = f;
+    for (int64_t k = 0; k < nblk; k++) {
+        const int buf = (int)(k & 1);
+        if (wave != 0) {
+            if (k + 1 < nblk)
+                mis_chain_stage(logw, lu_g, path_g, B, k + 1, s_lw[buf ^ 1], s_lu[buf ^ 1], s_pm[buf ^ 1], tid - 64,
+                                64 * (MIS_CHAIN_WAVES - 1));
+        } else {
+            const int64_t b0 = k * MIS_CHAIN_BLOCK;
+            const int bn = (int)(B - b0 < MIS_CHAIN_BLOCK ? B - b0 : MIS_CHAIN_BLOCK);
+            for (int g0 = 0; g0 < bn; g0 += 64) {
+                const int i = g0 + lane;
+                const bool valid = i < bn;
+                const double lw = s_lw[buf][i], lu = s_lu[buf][i];
+                const uint64_t pm = s_pm[buf][i];
+                const double d = lw - cur;
+                const uint64_t first_taken = __ballot(valid && (d >= 0.0 || lu < d));
+                uint64_t taken = 0;   // steps of this group that the chain takes
+                if (first_taken) {
+                    taken = mis_readlane_u64(pm, __builtin_ctzll(first_taken));
+                    cur = mis_readlane(lw, 63 - __builtin_clzll(taken));
+                }
+                const int64_t t0 = b0 + g0;
+                const uint64_t upto = taken & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+                const int64_t mine = upto ? t0 + (63 - __builtin_clzll(upto)) : idx;
+                if (valid) state[t0 + lane] = mine;
+                if (taken) idx = t0 + (63 - __builtin_clzll(taken));
+                acc += __builtin_popcountll(taken);
+            }
         }
-        const uint64_t upto = taken & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-        const int64_t mine = upto ? t0 + (63 - __builtin_clzll(upto)) : idx;
-        if (valid) state[t] = mine;
-        if (taken) idx = t0 + (63 - __builtin_clzll(taken));
-        acc += __builtin_popcountll(taken);
+        __syncthreads();   // block k's buffer is free, block k+1's is staged
     }
-    if (lane == 0) *accepted = acc;
+    if (tid == 0) *accepted = acc;
 }
 
 // Occupation counts of sorted sequences (B x n) -> (B x m).
@@ -605,10 +696,28 @@ extern "C" int pcvl_mis_logp(const double *d_perms, const int32_t *d_seq, int n,
 extern "C" int pcvl_mis_chain(const double *d_logw, int64_t B, double logw0, uint64_t seed, uint64_t first,
                               int64_t *d_state, int64_t *d_accepted, hipStream_t stream)
 {
-    PCVL_ARG_CHECK(B >= 0, "bad size");
+    PCVL_ARG_CHECK(B >= 0 && B <= ((int64_t)1 << 36), "bad size (0 <= B <= 2^36)");
     if (B == 0) return OK;
-    hipLaunchKernelGGL(mis_chain_kernel, dim3(1), dim3(64), 0, stream, d_logw, B, logw0, seed, first, d_state,
-                       d_accepted);
-    PCVL_HIP_CHECK(hipGetLastError());
-    return OK;
+    PCVL_ARG_CHECK(d_logw && d_state && d_accepted, "null device buffer");
+    // stream-ordered workspace: log uniforms | paths
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t b_lu = al(sizeof(double) * (size_t)B);
+    const size_t b_pm = al(sizeof(uint64_t) * (size_t)B);
+    char *blk = nullptr;
+    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), b_lu + b_pm, stream));
+    double *lu = reinterpret_cast<double *>(blk);
+    uint64_t *pm = reinterpret_cast<uint64_t *>(blk + b_lu);
+    int rc = OK;
+    hipLaunchKernelGGL(mis_chain_path_kernel, dim3((unsigned)((B + 63) / 64)), dim3(64), 0, stream, d_logw, B, seed,
+                       first, lu, pm);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mis_chain_kernel, dim3(1), dim3(64 * MIS_CHAIN_WAVES), 0, stream, d_logw, lu, pm, B, logw0,
+                           d_state, d_accepted);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) { set_error(std::string("mis chain: ") + hipGetErrorString(e)); rc = ERR_HIP; }
+    hipError_t fe = hipFreeAsync(blk, stream);
+    if (rc == OK && fe != hipSuccess) { set_error(std::string("hipFreeAsync(chain): ") + hipGetErrorString(fe)); rc = ERR_HIP; }
+    return rc;
 }

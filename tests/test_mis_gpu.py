@@ -233,24 +233,60 @@ def test_multiset_permanents_mixed_costs_deterministic(gpu):
     mis.close()
 
 
-def test_chain_scan_vs_host_restatement(gpu):
+@pytest.mark.parametrize("B", [1, 37, 64, 5000])
+def test_chain_scan_vs_host_restatement(gpu, B):
     import torch
     from perceval_amd import _native
     lib = _native.load()
     rng = np.random.RandomState(3)
-    logw = rng.randn(5000) * 2
+    logw = rng.randn(B) * 2
     d = torch.from_numpy(logw).cuda()
-    state = torch.empty(5000, dtype=torch.int64, device="cuda")
+    state = torch.empty(B, dtype=torch.int64, device="cuda")
     acc = torch.zeros(1, dtype=torch.int64, device="cuda")
-    _native.check(lib.pcvl_mis_chain(_native.ptr(d), 5000, 0.3, 77, 10, _native.ptr(state), _native.ptr(acc),
+    _native.check(lib.pcvl_mis_chain(_native.ptr(d), B, 0.3, 77, 10, _native.ptr(state), _native.ptr(acc),
                                      _native.stream_handle()), "chain")
     cur, idx, ref, nacc = 0.3, -1, [], 0
-    for t in range(5000):
+    for t in range(B):
         dlt = logw[t] - cur
         if dlt >= 0 or math.log(oracle.uniform(77, 10 + t, 0)) < dlt:
             cur, idx, nacc = logw[t], t, nacc + 1
         ref.append(idx)
     assert np.array_equal(state.cpu().numpy(), np.array(ref)) and acc.item() == nacc
+
+
+@pytest.mark.parametrize("first", [0, 1000])
+def test_run_vs_host_replay(gpu, first):
+    """A whole run: the chain over the proposals it returns, started from
+    x0's own weight, replayed on the host with the oracle's uniforms."""
+    import torch
+    from perceval_amd.mis import LearnedMIS, ProposalNet
+    m, n = 8, 3
+    U = haar_unitary(m, seed=12)
+    inp = [1, 1, 1] + [0] * (m - 3)
+    net = ProposalNet(m, n, hidden=32, seed=5)
+    net.train(torch.from_numpy(_seqs_from_cc(U, inp, 4000, seed=6)).cuda(), epochs=1, batch=1024)
+    mis = LearnedMIS(U, inp, net, chunk=1024)
+    x0 = torch.from_numpy(_seqs_from_cc(U, inp, 1, seed=9)[0]).cuda()
+    count, seed = 3000, 21
+    res = mis.run(count, seed=seed, x0_seq=x0, first=first)
+    props = res["proposals"].cpu().numpy()
+    logw = res["logw"].cpu().numpy()
+    # the proposals' weights against a separate scoring (their log q comes
+    # from the sampler's own fp32 logits, equal to net.log_q up to rounding)
+    lp = mis.log_p(res["proposals"]).cpu().numpy()
+    lq = net.log_q(res["proposals"]).cpu().numpy()
+    assert np.allclose(logw, lp - lq, rtol=0, atol=1e-3)
+    x0s = x0.reshape(1, n).to(torch.int32)
+    cur = float((mis.log_p(x0s) - net.log_q(x0s))[0].item())
+    held, nacc, ref = x0.cpu().numpy(), 0, []
+    for t in range(count):
+        dlt = logw[t] - cur
+        if dlt >= 0 or math.log(oracle.uniform(seed ^ 0x5DEECE66D, first + t, 0)) < dlt:
+            cur, held, nacc = logw[t], props[t], nacc + 1
+        ref.append(held)
+    assert np.array_equal(res["seq"].cpu().numpy(), np.array(ref))
+    assert res["acceptance"] == nacc / count
+    mis.close()
 
 
 @pytest.mark.parametrize("trained", [False, True])
