@@ -400,7 +400,7 @@ int pcvl_mis_logp(const double *d_perms, const int32_t *d_seq, int n, int64_t B,
  * after step t (-1: the initial state); *d_accepted = accepted moves.  Step t
  * is taken iff lw_t - cur >= 0 or log u_t < lw_t - cur, u_t the uniform of
  * stream (seed, first + t).  B <= 2^36; 16 B bytes of workspace come from the
- * stream-ordered allocator. */
+ * library's stream-ordered pool. */
 int pcvl_mis_chain(const double *d_logw, int64_t B, double logw0, uint64_t seed, uint64_t first,
                    int64_t *d_state, int64_t *d_accepted, hipStream_t stream);
 

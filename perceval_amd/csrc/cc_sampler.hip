@@ -570,7 +570,7 @@ extern "C" int pcvl_cc_samples_multi(const double *d_U, int m, const int32_t *h_
     for (int n : ns) b_part = std::max(b_part, cc_partial_bytes(n, gmax));
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     char *blk = nullptr;
-    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk),
+    PCVL_HIP_CHECK(ws_alloc(reinterpret_cast<void **>(&blk),
                                   2 * al(b_pr) + al(b_inp) + al(b_gidx) + al(b_cols) + al(b_part), stream));
     int32_t *pc = reinterpret_cast<int32_t *>(blk);
     int32_t *r = reinterpret_cast<int32_t *>(blk + al(b_pr));

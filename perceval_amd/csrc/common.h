@@ -20,6 +20,11 @@ enum Status : int {
     ERR_RANGE = -4,    // result does not fit (e.g. Fock space above 2^62)
 };
 
+// Stream-ordered device workspace from the library's own pool on the current
+// device (freed with hipFreeAsync).  Freed memory, up to 512 MiB, stays
+// mapped in the pool for the next call.
+hipError_t ws_alloc(void **p, size_t bytes, hipStream_t stream);
+
 #define PCVL_HIP_CHECK(expr)                                                     \
     do {                                                                         \
         hipError_t _e = (expr);                                                  \

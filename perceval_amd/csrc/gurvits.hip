@@ -318,7 +318,7 @@ extern "C" int pcvl_gurvits_batch(const double *d_mats, const int32_t *h_n, cons
     const size_t b_items = al(sizeof(GurvitsItem) * items.size());
     const size_t b_part = al(sizeof(double) * 3 * (size_t)count * parts);
     char *blk = nullptr;
-    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), b_items + b_part, stream));
+    PCVL_HIP_CHECK(ws_alloc(reinterpret_cast<void **>(&blk), b_items + b_part, stream));
     GurvitsItem *d_items = reinterpret_cast<GurvitsItem *>(blk);
     double *d_part = reinterpret_cast<double *>(blk + b_items);
     hipError_t e = hipMemcpyAsync(d_items, items.data(), sizeof(GurvitsItem) * items.size(), hipMemcpyHostToDevice,

@@ -610,7 +610,7 @@ extern "C" int pcvl_mis_permanents(const double *d_U, int m, const int32_t *d_co
     const size_t b_cur = al(sizeof(unsigned long long));
     const size_t b_part = al(sizeof(cd) * (size_t)B * MISP_MAX_SLICES);
     char *blk = nullptr;
-    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), b_rec + b_ns + b_s0 + b_cur + b_part, stream));
+    PCVL_HIP_CHECK(ws_alloc(reinterpret_cast<void **>(&blk), b_rec + b_ns + b_s0 + b_cur + b_part, stream));
     int32_t *rec = reinterpret_cast<int32_t *>(blk);
     int32_t *ns = reinterpret_cast<int32_t *>(blk + b_rec);
     int64_t *s0 = reinterpret_cast<int64_t *>(blk + b_rec + b_ns);
@@ -704,7 +704,7 @@ extern "C" int pcvl_mis_chain(const double *d_logw, int64_t B, double logw0, uin
     const size_t b_lu = al(sizeof(double) * (size_t)B);
     const size_t b_pm = al(sizeof(uint64_t) * (size_t)B);
     char *blk = nullptr;
-    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), b_lu + b_pm, stream));
+    PCVL_HIP_CHECK(ws_alloc(reinterpret_cast<void **>(&blk), b_lu + b_pm, stream));
     double *lu = reinterpret_cast<double *>(blk);
     uint64_t *pm = reinterpret_cast<uint64_t *>(blk + b_lu);
     int rc = OK;

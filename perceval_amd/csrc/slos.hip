@@ -1441,7 +1441,7 @@ extern "C" int pcvl_fock_mask_select(int m, int n, int mask_n, const int32_t *h_
     const size_t b_off = al(sizeof(int64_t) * ((size_t)nb + 1));
     const size_t b_fl = al((size_t)total);
     char *blk = nullptr;
-    PCVL_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&blk), b_T + b_c + b_a + b_cnt + b_off + b_fl, stream));
+    PCVL_HIP_CHECK(ws_alloc(reinterpret_cast<void **>(&blk), b_T + b_c + b_a + b_cnt + b_off + b_fl, stream));
     uint64_t *dT = reinterpret_cast<uint64_t *>(blk);
     int32_t *dc = reinterpret_cast<int32_t *>(blk + b_T);
     int32_t *da = reinterpret_cast<int32_t *>(blk + b_T + b_c);
