@@ -384,10 +384,13 @@ __global__ void __launch_bounds__(FS_THREADS) sample_fused_kernel(
         }
 #pragma unroll
         for (int o = 1; o < 8; o <<= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        double ev[8];   // this thread's exp(z_j - mx), reused by the draw below
         double part = 0.0;
+#pragma unroll
         for (int u = 0; u < 8; u++) {
             const int j = 8 * sub + u;
-            if (j >= lo && j < m) part += exp((double)(z[j] - mx));
+            ev[u] = (j >= lo && j < m) ? exp((double)(z[j] - mx)) : 0.0;
+            if (j >= lo && j < m) part += ev[u];
         }
         double se = part;
 #pragma unroll
@@ -402,10 +405,11 @@ __global__ void __launch_bounds__(FS_THREADS) sample_fused_kernel(
         int cand = -1;
         if (target >= base || sub == 0) {
             double cum = base;
-            for (int u = 0; u < 8 && cand < 0; u++) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
                 const int j = 8 * sub + u;
-                if (j >= lo && j < m) {
-                    cum += exp((double)(z[j] - mx));
+                if (cand < 0 && j >= lo && j < m) {
+                    cum += ev[u];
                     if (target < cum) cand = j;
                 }
             }
