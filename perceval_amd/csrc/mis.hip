@@ -330,32 +330,61 @@ __global__ void mis_plan_kernel(const int32_t *__restrict__ seq, int64_t B, int 
     if (b == 0) *cursor = 0ull;
     if (b >= B) return;
     int32_t *R = rec + b * MISP_REC;
-    int gmode[MISP_MAX_N], gcnt[MISP_MAX_N];
-    int G = 0;
-    for (int r = 0; r < n; r++) {
-        int j = seq[b * n + r];
-        j = (j >= 0 && j < m) ? j : 0;
-        if (G > 0 && gmode[G - 1] == j) gcnt[G - 1]++;
-        else { gmode[G] = j; gcnt[G] = 1; G++; }
+    const int32_t *sq = seq + b * n;
+    // The sorted sequence's runs (of modes clamped to [0, m)) are its distinct
+    // modes with their counts; they are walked in place, with no per-thread
+    // arrays.  The fixed row is the first run of count 1 if there is one, else
+    // one copy of the first run.  Products of (count + 1) stay below 2^28
+    // (at most 2^(n-1) terms), so 32-bit counts suffice.
+    auto mode_at = [&](int r) {
+        const int j = sq[r];
+        return (j >= 0 && j < m) ? j : 0;
+    };
+    auto run_end = [&](int r, int j) {
+        int e = r + 1;
+        while (e < n && mode_at(e) == j) e++;
+        return e;
+    };
+    int fixed_mode = -1;
+    for (int r = 0; r < n;) {
+        const int j = mode_at(r), e = run_end(r, j);
+        if (e - r == 1) { fixed_mode = j; break; }
+        r = e;
     }
-    int fixed = -1;
-    for (int g = 0; g < G; g++) if (gcnt[g] == 1) { fixed = g; break; }
     int nrow = 0, nu = 0, ng = 0;
-    if (fixed >= 0) {
-        R[8 + nrow++] = gmode[fixed];
-        for (int g = 0; g < G; g++)
-            if (g != fixed && gcnt[g] == 1) { R[8 + nrow++] = gmode[g]; nu++; }
+    uint32_t O = 1;
+    if (fixed_mode >= 0) {
+        R[8 + nrow++] = fixed_mode;
+        bool skipped = false;   // the fixed row's own run
+        for (int r = 0; r < n;) {
+            const int j = mode_at(r), e = run_end(r, j);
+            if (e - r == 1) {
+                if (skipped) { R[8 + nrow++] = j; nu++; }
+                skipped = true;
+            }
+            r = e;
+        }
+        for (int r = 0; r < n;) {
+            const int j = mode_at(r), e = run_end(r, j);
+            if (e - r >= 2) {
+                R[8 + nrow++] = j;
+                R[8 + MISP_MAX_N + ng++] = e - r;
+                O *= (uint32_t)(e - r + 1);
+            }
+            r = e;
+        }
     } else {
-        R[8 + nrow++] = gmode[0];
-        gcnt[0] -= 1;   // one copy of group 0 is the fixed row
-    }
-    uint64_t O = 1;
-    for (int g = 0; g < G; g++) {
-        if (g == fixed || gcnt[g] < 1) continue;
-        if (fixed >= 0 && gcnt[g] == 1) continue;   // already a unique row
-        R[8 + nrow++] = gmode[g];
-        R[8 + MISP_MAX_N + ng++] = gcnt[g];
-        O *= (uint64_t)(gcnt[g] + 1);
+        for (int r = 0; r < n;) {
+            const int j = mode_at(r), e = run_end(r, j);
+            const int k = r == 0 ? e - r - 1 : e - r;   // one copy of the first run is the fixed row
+            if (r == 0) R[8 + nrow++] = j;
+            if (k >= 1) {
+                R[8 + nrow++] = j;
+                R[8 + MISP_MAX_N + ng++] = k;
+                O *= (uint32_t)(k + 1);
+            }
+            r = e;
+        }
     }
     // Unit length 2^c: a wave runs its slice's units 64 at a time, so a unit
     // count that is not a multiple of 64 idles lanes in the last pass, while
@@ -369,43 +398,63 @@ __global__ void mis_plan_kernel(const int32_t *__restrict__ seq, int64_t B, int 
     int c = cmax;
     double best = -1.0;
     for (int cc = cmax; cc >= 0 && cc >= cmax - 5; cc--) {
-        const uint64_t un = O << (nu - cc);
-        const uint64_t sl_c = (un + 63) / 64 < (uint64_t)MISP_MAX_SLICES ? (un + 63) / 64 : (uint64_t)MISP_MAX_SLICES;
-        const uint64_t per = (un + sl_c - 1) / sl_c;
-        const uint64_t full = un / per, rest = un - full * per;
-        const uint64_t passes = full * ((per + 63) / 64) + (rest + 63) / 64;
-        const double est = (double)passes * ((double)(1ull << cc) * step_ops + setup_ops) + 1000.0 * (double)sl_c;
+        const uint32_t un = O << (nu - cc);
+        const uint32_t sl_c = (un + 63) / 64 < (uint32_t)MISP_MAX_SLICES ? (un + 63) / 64 : (uint32_t)MISP_MAX_SLICES;
+        const uint32_t per = (un + sl_c - 1) / sl_c;
+        const uint32_t full = un / per, rest = un - full * per;
+        const uint32_t passes = full * ((per + 63) / 64) + (rest + 63) / 64;
+        const double est = (double)passes * ((double)(1u << cc) * step_ops + setup_ops) + 1000.0 * (double)sl_c;
         if (best < 0.0 || est < best) { best = est; c = cc; }
     }
-    const uint64_t units = O << (nu - c);
-    const uint64_t sl = (units + 63) / 64;
-    const int slices = (int)(sl < (uint64_t)MISP_MAX_SLICES ? sl : (uint64_t)MISP_MAX_SLICES);
+    const uint32_t units = O << (nu - c);
+    const uint32_t sl = (units + 63) / 64;
+    const int slices = (int)(sl < (uint32_t)MISP_MAX_SLICES ? sl : (uint32_t)MISP_MAX_SLICES);
     R[0] = nu; R[1] = ng; R[2] = c; R[3] = slices;
-    R[4] = (int32_t)(units & 0xffffffffu); R[5] = (int32_t)(units >> 32); R[6] = nrow; R[7] = 0;
+    R[4] = (int32_t)units; R[5] = 0; R[6] = nrow; R[7] = 0;
     nslices[b] = slices;
 }
 
-// Exclusive prefix of the slice counts, one block: slice0[b], total at slice0[B].
+// Exclusive prefix of the slice counts, one block: slice0[b], total at
+// slice0[B].  4096 counts per pass, four consecutive ones per thread (their
+// loads in flight together); a shuffle scan of the thread sums per wave and
+// the 16 wave totals through LDS.
 __global__ void __launch_bounds__(1024) mis_scan_kernel(const int32_t *__restrict__ nslices, int64_t B,
                                                         int64_t *__restrict__ slice0)
 {
-    __shared__ int64_t part[1024];
-    const int t = threadIdx.x;
-    const int64_t per = (B + 1023) / 1024;
-    const int64_t lo = t * per, hi = (lo + per < B) ? lo + per : B;
-    int64_t sum = 0;
-    for (int64_t i = lo; i < hi; i++) sum += nslices[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int64_t v = t >= off ? part[t - off] : 0;
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int64_t carry = 0;
+    for (int64_t base = 0; base < B; base += 4096) {
+        const int64_t i0 = base + 4 * (int64_t)t;
+        int v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = i0 + q < B ? nslices[i0 + q] : 0;   // at most 64 each
+        const int own = v[0] + v[1] + v[2] + v[3];
+        int x = own;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
         __syncthreads();
-        part[t] += v;
-        __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int w = wsum[k];
+            if (k < wv) before += w;
+            total += w;
+        }
+        int64_t run = carry + before + x - own;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (i0 + q < B) slice0[i0 + q] = run;
+            run += v[q];
+        }
+        carry += total;
+        __syncthreads();   // wsum is rewritten by the next pass
     }
-    int64_t run = part[t] - sum;   // exclusive
-    for (int64_t i = lo; i < hi; i++) { slice0[i] = run; run += nslices[i]; }
-    if (t == 1023) slice0[B] = part[1023];
+    if (t == 0) slice0[B] = carry;
 }
 
 template <int N>
