@@ -457,11 +457,21 @@ __global__ void __launch_bounds__(1024) mis_scan_kernel(const int32_t *__restric
     if (t == 0) slice0[B] = carry;
 }
 
+// Owner of every slice, packed as proposal * 64 + slice (slices per
+// proposal <= 64), so a wave finds its slice's proposal with one load.
+__global__ void mis_owner_kernel(const int64_t *__restrict__ slice0, int64_t B, uint64_t *__restrict__ owner)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int64_t g0 = slice0[b], g1 = slice0[b + 1];
+    for (int64_t g = g0; g < g1; g++) owner[g] = ((uint64_t)b << 6) | (uint64_t)(g - g0);
+}
+
 template <int N>
 __global__ void __launch_bounds__(64) mis_slice_kernel(const cd *__restrict__ U, int m, const int32_t *__restrict__ cols,
                                                        const int32_t *__restrict__ rec, const int64_t *__restrict__ slice0,
-                                                       int64_t B, unsigned long long *__restrict__ cursor,
-                                                       cd *__restrict__ partial)
+                                                       const uint64_t *__restrict__ owner, int64_t B,
+                                                       unsigned long long *__restrict__ cursor, cd *__restrict__ partial)
 {
     __shared__ MispShared sh;
     __shared__ int32_t srec[MISP_REC];
@@ -474,14 +484,10 @@ __global__ void __launch_bounds__(64) mis_slice_kernel(const cd *__restrict__ U,
         gk = __shfl(gk, 0, 64);
         const int64_t g = (int64_t)gk;
         if (g >= total) break;
-        // proposal p: slice0[p] <= g < slice0[p+1] (slices are >= 1 per proposal)
-        int64_t lo = 0, hi = B - 1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (slice0[mid] <= g) lo = mid; else hi = mid - 1;
-        }
-        const int64_t pidx = lo;
-        const int s = (int)(g - slice0[pidx]);
+        // proposal p and slice s of global slice g, packed as p * 64 + s
+        const uint64_t own = owner[g];
+        const int64_t pidx = (int64_t)(own >> 6);
+        const int s = (int)(own & 63u);
         for (int e = lane; e < MISP_REC; e += 64) srec[e] = rec[pidx * MISP_REC + e];
         __syncthreads();
         const int nu = srec[0], ng = srec[1], c = srec[2], slices = srec[3], nrow = srec[6];
@@ -621,21 +627,21 @@ __global__ void mis_finalize_kernel(const cd *__restrict__ partial, const int64_
 
 template <int N>
 static int launch_mis_slices(int n, const cd *U, int m, const int32_t *cols, const int32_t *rec, const int64_t *slice0,
-                             int64_t B, unsigned long long *cursor, cd *partial, hipStream_t st)
+                             const uint64_t *owner, int64_t B, unsigned long long *cursor, cd *partial, hipStream_t st)
 {
     if constexpr (N > MISP_MAX_N) {
         set_error("photon number above 28");
         return ERR_ARG;
     } else {
-        if (n != N) return launch_mis_slices<N + 1>(n, U, m, cols, rec, slice0, B, cursor, partial, st);
+        if (n != N) return launch_mis_slices<N + 1>(n, U, m, cols, rec, slice0, owner, B, cursor, partial, st);
         int dev = 0, cus = 0, per_cu = 0;
         PCVL_HIP_CHECK(hipGetDevice(&dev));
         PCVL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         PCVL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&mis_slice_kernel<N>), 64, 0));
         if (per_cu < 1) per_cu = 1;
         const int grid = (int)std::min<int64_t>(B * MISP_MAX_SLICES, (int64_t)cus * per_cu);
-        hipLaunchKernelGGL((mis_slice_kernel<N>), dim3(grid), dim3(64), 0, st, U, m, cols, rec, slice0, B, cursor,
-                           partial);
+        hipLaunchKernelGGL((mis_slice_kernel<N>), dim3(grid), dim3(64), 0, st, U, m, cols, rec, slice0, owner, B,
+                           cursor, partial);
         PCVL_HIP_CHECK(hipGetLastError());
         return OK;
     }
@@ -651,20 +657,22 @@ extern "C" int pcvl_mis_permanents(const double *d_U, int m, const int32_t *d_co
     PCVL_ARG_CHECK(m >= 1 && n >= 1 && n <= MISP_MAX_N && B >= 0, "bad sizes (1 <= n <= 28)");
     if (B == 0) return OK;
     PCVL_ARG_CHECK(d_U && d_cols && d_seq && d_out, "null device buffer");
-    // stream-ordered workspace: records | slice counts | slice offsets | cursor | partials
+    // stream-ordered workspace: records | slice counts | slice offsets | cursor | partials | slice owners
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t b_rec = al(sizeof(int32_t) * (size_t)B * MISP_REC);
     const size_t b_ns = al(sizeof(int32_t) * (size_t)B);
     const size_t b_s0 = al(sizeof(int64_t) * ((size_t)B + 1));
     const size_t b_cur = al(sizeof(unsigned long long));
     const size_t b_part = al(sizeof(cd) * (size_t)B * MISP_MAX_SLICES);
+    const size_t b_own = al(sizeof(uint64_t) * (size_t)B * MISP_MAX_SLICES);
     char *blk = nullptr;
-    PCVL_HIP_CHECK(ws_alloc(reinterpret_cast<void **>(&blk), b_rec + b_ns + b_s0 + b_cur + b_part, stream));
+    PCVL_HIP_CHECK(ws_alloc(reinterpret_cast<void **>(&blk), b_rec + b_ns + b_s0 + b_cur + b_part + b_own, stream));
     int32_t *rec = reinterpret_cast<int32_t *>(blk);
     int32_t *ns = reinterpret_cast<int32_t *>(blk + b_rec);
     int64_t *s0 = reinterpret_cast<int64_t *>(blk + b_rec + b_ns);
     unsigned long long *cur = reinterpret_cast<unsigned long long *>(blk + b_rec + b_ns + b_s0);
     cd *part = reinterpret_cast<cd *>(blk + b_rec + b_ns + b_s0 + b_cur);
+    uint64_t *own = reinterpret_cast<uint64_t *>(blk + b_rec + b_ns + b_s0 + b_cur + b_part);
     int rc = OK;
     hipLaunchKernelGGL(mis_plan_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, stream, d_seq, B, n, m, rec, ns,
                        cur);
@@ -673,11 +681,15 @@ extern "C" int pcvl_mis_permanents(const double *d_U, int m, const int32_t *d_co
         hipLaunchKernelGGL(mis_scan_kernel, dim3(1), dim3(1024), 0, stream, ns, B, s0);
         e = hipGetLastError();
     }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mis_owner_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, stream, s0, B, own);
+        e = hipGetLastError();
+    }
     if (e != hipSuccess) {
         set_error(std::string("mis plan: ") + hipGetErrorString(e));
         rc = ERR_HIP;
     } else {
-        rc = launch_mis_slices<1>(n, reinterpret_cast<const cd *>(d_U), m, d_cols, rec, s0, B, cur, part, stream);
+        rc = launch_mis_slices<1>(n, reinterpret_cast<const cd *>(d_U), m, d_cols, rec, s0, own, B, cur, part, stream);
     }
     if (rc == OK) {
         hipLaunchKernelGGL(mis_finalize_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, stream, part, s0, B,
