@@ -221,6 +221,9 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     assert "workspace" in err(lib.pcvl_mlp_wgrad(out, 3, out, 3, 100, 3, 3, out, out, out, 0, None))
     assert "probe" in err(lib.pcvl_probe_fp64_fma(0, 10, out, ctypes.byref(f), None))
     assert "1 <= n <= 28" in err(lib.pcvl_mis_permanents(out, 10, out, 29, out, 5, out, None))
+    assert "2^36" in err(lib.pcvl_mis_chain(out, (1 << 36) + 1, 0.0, 1, 0, out, out, None))
+    assert "2^36" in err(lib.pcvl_mis_chain(out, -1, 0.0, 1, 0, out, out, None))
+    assert "null" in err(lib.pcvl_mis_chain(None, 10, 0.0, 1, 0, out, out, None))
 
 
 def test_output_submatrices_match_the_per_state_construction():
