@@ -281,7 +281,10 @@ int pcvl_cc_samples_multi(const double *d_U, int m, const int32_t *h_inputs, con
  *                  or h_const[8p] when h_param[p] = -1;
  *   h_kind[p] = 1: fixed 2x2 unitary on modes (h_mode[p], h_mode[p]+1),
  *                  h_const[8p..8p+7] row-major interleaved complex.
- * h_targets: ntargets rows of m int32.  m <= 32, 1..8 photons, <= 64 targets.
+ * h_targets: ntargets rows of m int32.  m <= 32, 1..8 photons, <= 512 targets
+ * (run in blocks of 64 whose cofactors sit in LDS); -4 when the program,
+ * index tables and target norms would take more than 160 KiB of LDS (a
+ * caller then splits the targets over several plans and adds the results).
  * Synchronises `stream`.
  * ------------------------------------------------------------------------- */
 int pcvl_mesh_adjoint(const int32_t *h_kind, const int32_t *h_mode, const int32_t *h_param,

@@ -29,7 +29,8 @@ namespace pcvl {
 
 constexpr int ADJ_MAX_M = 32;
 constexpr int ADJ_MAX_N = 8;
-constexpr int ADJ_MAX_TARGETS = 64;
+constexpr int ADJ_MAX_TARGETS = 512;     // targets of one plan (its LDS decides what fits)
+constexpr int ADJ_BLOCK_TARGETS = 64;    // targets whose cofactors are in LDS at once
 constexpr int ADJ_THREADS = 256;
 
 struct MeshOp {
@@ -154,8 +155,9 @@ __host__ __device__ inline AdjLds adj_lds(int nops, int nparams, int m, int n, i
     L.zph = o; o = adj_al16(o + sizeof(cd) * (size_t)(nparams > 0 ? nparams : 1));
     L.U = o; o = adj_al16(o + sizeof(cd) * (size_t)m * m);
     L.Z = o; o = adj_al16(o + sizeof(cd) * (size_t)m * m);
-    L.amp = o; o = adj_al16(o + sizeof(cd) * (size_t)ntargets);
-    L.cof = o; o = adj_al16(o + sizeof(cd) * (size_t)ntargets * n * n);
+    const int nb = ntargets < ADJ_BLOCK_TARGETS ? ntargets : ADJ_BLOCK_TARGETS;
+    L.amp = o; o = adj_al16(o + sizeof(cd) * (size_t)nb);
+    L.cof = o; o = adj_al16(o + sizeof(cd) * (size_t)nb * n * n);
     L.tn = o; o = adj_al16(o + sizeof(double) * (size_t)ntargets);
     L.tw = o; o = adj_al16(o + sizeof(double) * (size_t)ntargets);
     L.red = o; o = adj_al16(o + sizeof(double) * ADJ_THREADS);
@@ -262,43 +264,49 @@ __global__ void __launch_bounds__(ADJ_THREADS) mesh_adjoint_kernel(
             }
         }
         __syncthreads();
-        // 2. amplitudes and cofactors: thread (target s, row r)
-        for (int w = tid; w < ntargets * n; w += ADJ_THREADS) {
-            const int s = w / n, r = w - (w / n) * n;
-            // scaled by 1 / sqrt(prod s! prod t!)
-            dispatch_glynn<1>(n, U, m, tm + s * n, in_modes, r, tn[s], cof + (size_t)(s * n + r) * n, amp + s);
-        }
-        __syncthreads();
-        // loss and per-target probabilities
-        if (tid < ntargets) {
-            const cd a = amp[tid];
-            const double pr = a.r * a.r + a.i * a.i;
-            if (probs_out) probs_out[(size_t)set * ntargets + tid] = pr;
-            red[tid] = tw[tid] * pr;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double l = 0.0;
-            for (int s = 0; s < ntargets; s++) l += red[s];
-            loss_out[set] = l;
-        }
-        // G^T into Z: Z[j][i] = G[i][j] = sum over (s, r) with rows_s[r] = i (increasing s, r) of
+        // 2. per block of up to 64 targets: amplitudes and cofactors (thread (target s, row r)), the block's
+        // share of the loss, and its share of G^T, added into Z in block order:
+        // Z[j][i] = G[i][j] = sum over (s, r) with rows_s[r] = i (increasing s, r) of
         // w_s conj(A_s) sum over c with in[c] = j of cof[s][r][c]
-        for (int e = tid; e < m * m; e += ADJ_THREADS) {
-            const int j = e / m, i = e - (e / m) * m;
-            cd g = {0.0, 0.0};
-            const int c0 = cstart[j], c1 = cstart[j + 1];
-            if (c0 < c1)
-                for (int u = rstart[i]; u < rstart[i + 1]; u++) {
-                    const int sr = rlist[u];
-                    const cd a = amp[sr / n];
-                    const double ws = tw[sr / n];
-                    const cd ca = {ws * a.r, -(ws * a.i)};
-                    for (int v = c0; v < c1; v++) cfma_acc(g, ca, cof[(size_t)sr * n + clist[v]]);
-                }
-            Z[j * m + i] = g;
+        double lacc = 0.0;   // thread 0: the loss, in target order
+        for (int s0 = 0, blk = 0; s0 < ntargets; s0 += ADJ_BLOCK_TARGETS, blk++) {
+            const int nt = ntargets - s0 < ADJ_BLOCK_TARGETS ? ntargets - s0 : ADJ_BLOCK_TARGETS;
+            for (int w = tid; w < nt * n; w += ADJ_THREADS) {
+                const int s = w / n, r = w - (w / n) * n;
+                // scaled by 1 / sqrt(prod s! prod t!)
+                dispatch_glynn<1>(n, U, m, tm + (s0 + s) * n, in_modes, r, tn[s0 + s], cof + (size_t)(s * n + r) * n,
+                                  amp + s);
+            }
+            __syncthreads();
+            // loss and per-target probabilities
+            if (tid < nt) {
+                const cd a = amp[tid];
+                const double pr = a.r * a.r + a.i * a.i;
+                if (probs_out) probs_out[(size_t)set * ntargets + s0 + tid] = pr;
+                red[tid] = tw[s0 + tid] * pr;
+            }
+            __syncthreads();
+            if (tid == 0)
+                for (int s = 0; s < nt; s++) lacc += red[s];
+            const int *rs = rstart + blk * (m + 1);
+            for (int e = tid; e < m * m; e += ADJ_THREADS) {
+                const int j = e / m, i = e - (e / m) * m;
+                cd g = {0.0, 0.0};
+                const int c0 = cstart[j], c1 = cstart[j + 1];
+                if (c0 < c1)
+                    for (int u = rs[i]; u < rs[i + 1]; u++) {
+                        const int sr = rlist[u] - s0 * n;   // (s, r) within the block
+                        const cd a = amp[sr / n];
+                        const double ws = tw[s0 + sr / n];
+                        const cd ca = {ws * a.r, -(ws * a.i)};
+                        for (int v = c0; v < c1; v++) cfma_acc(g, ca, cof[(size_t)sr * n + clist[v]]);
+                    }
+                if (blk == 0) Z[j * m + i] = g;
+                else Z[j * m + i] = cadd(Z[j * m + i], g);
+            }
+            __syncthreads();   // amp, cof and red are rewritten by the next block
         }
-        __syncthreads();
+        if (tid == 0) loss_out[set] = lacc;
         // 3. backward sweep (wave 0), levels in reverse: F (in U) <- L_p^H F, grad, Z <- Z L_p
         if (w0) {
             for (int lv = X.nlev - 1; lv >= 0; lv--) {
@@ -377,7 +385,7 @@ extern "C" int pcvl_mesh_plan_create(const int32_t *h_kind, const int32_t *h_mod
     PCVL_ARG_CHECK(m >= 1 && m <= ADJ_MAX_M, "mode count outside [1, 32]");
     PCVL_ARG_CHECK(nops >= 0 && (nops == 0 || (h_kind && h_mode && h_param && h_const)), "bad mesh program");
     PCVL_ARG_CHECK(nparams >= 0, "negative sizes");
-    PCVL_ARG_CHECK(ntargets >= 1 && ntargets <= ADJ_MAX_TARGETS, "target count outside [1, 64]");
+    PCVL_ARG_CHECK(ntargets >= 1 && ntargets <= ADJ_MAX_TARGETS, "target count outside [1, 512]");
     PCVL_ARG_CHECK(h_in_occ && h_targets, "null input/target states");
     std::vector<MeshOp> ops(nops);
     for (int p = 0; p < nops; p++) {
@@ -440,15 +448,20 @@ extern "C" int pcvl_mesh_plan_create(const int32_t *h_kind, const int32_t *h_mod
     }
     X.lstart = (int)idx.size();
     idx.insert(idx.end(), lstart.begin(), lstart.end());
-    // per mode i the (s, r) whose target row is i, per mode j the c whose input column is j
+    // per block of 64 targets and mode i the (s, r) whose target row is i (rstart[blk * (m + 1) + i]),
+    // per mode j the c whose input column is j
     X.rstart = (int)idx.size();
-    std::vector<int32_t> rstart(m + 1, 0), rlist;
-    for (int i = 0; i < m; i++) {
-        rstart[i] = (int32_t)rlist.size();
-        for (int e = 0; e < ntargets * n; e++)
-            if (tmodes[(size_t)e] == i) rlist.push_back(e);
+    const int nblk = (ntargets + ADJ_BLOCK_TARGETS - 1) / ADJ_BLOCK_TARGETS;
+    std::vector<int32_t> rstart((size_t)nblk * (m + 1), 0), rlist;
+    for (int b = 0; b < nblk; b++) {
+        const int e0 = b * ADJ_BLOCK_TARGETS * n, e1 = std::min(ntargets, (b + 1) * ADJ_BLOCK_TARGETS) * n;
+        for (int i = 0; i < m; i++) {
+            rstart[(size_t)b * (m + 1) + i] = (int32_t)rlist.size();
+            for (int e = e0; e < e1; e++)
+                if (tmodes[(size_t)e] == i) rlist.push_back(e);
+        }
+        rstart[(size_t)b * (m + 1) + m] = (int32_t)rlist.size();
     }
-    rstart[m] = (int32_t)rlist.size();
     idx.insert(idx.end(), rstart.begin(), rstart.end());
     X.rlist = (int)idx.size();
     idx.insert(idx.end(), rlist.begin(), rlist.end());
@@ -472,7 +485,10 @@ extern "C" int pcvl_mesh_plan_create(const int32_t *h_kind, const int32_t *h_mod
     // it is read from global memory
     const bool stage_ops = adj_lds(nops, nparams, m, n, ntargets, X.total).bytes <= 64 * 1024;
     const AdjLds L = adj_lds(stage_ops ? nops : 0, nparams, m, n, ntargets, X.total);
-    PCVL_ARG_CHECK(L.bytes <= 160 * 1024, "too many phases or targets for the kernel's LDS");
+    if (L.bytes > 160 * 1024) {   // a caller may split the targets over several plans
+        set_error("too many phases or targets for the kernel's LDS (" + std::to_string(L.bytes) + " bytes)");
+        return ERR_RANGE;
+    }
     auto *plan = new (std::nothrow) pcvl_mesh_plan_s();
     if (!plan) { set_error("host allocation failed"); return ERR_ALLOC; }
     plan->m = m; plan->n = n; plan->nops = nops; plan->nparams = nparams; plan->ntargets = ntargets;

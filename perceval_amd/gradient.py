@@ -155,17 +155,19 @@ class MeshPlan:
             pass
 
 
-MAX_PLAN_TARGETS = 64   # targets of one kernel plan; more are split over several plans
+MAX_PLAN_TARGETS = 512   # targets of one kernel plan; more are split over several plans
 
 
 def mesh_adjoint(program: MeshProgram, phases, in_occ, targets, want_probs: bool = False, weights=None):
     """Run the kernel.  phases: (B, P) float64 CUDA tensor or numpy.
     Returns (loss (B,), grad (B, P)[, probs (B, T)]) as CUDA tensors, with
     loss = sum_s w_s P(s) over the targets (weights None: all w_s = 1), e.g.
-    the expectation of a diagonal observable over every output state.  More
-    than 64 targets are split over several plans, whose losses and gradients
-    add (the gradient is linear in the targets).  The program keeps one
-    MeshPlan per (input state, targets)."""
+    the expectation of a diagonal observable over every output state.  A
+    plan takes up to 512 targets (run in blocks of 64 inside the kernel);
+    more, or a set whose tables do not fit the kernel's LDS (-4), are split
+    over several plans, whose losses and gradients add (the gradient is
+    linear in the targets).  The program keeps one MeshPlan per (input
+    state, targets)."""
     import torch
     in_occ = tuple(int(v) for v in in_occ)
     targets = np.ascontiguousarray(np.asarray([list(t) for t in targets], dtype=np.int32))
@@ -175,13 +177,22 @@ def mesh_adjoint(program: MeshProgram, phases, in_occ, targets, want_probs: bool
     plans = program.__dict__.setdefault("_plans", {})
     loss = grad = None
     probs = []
-    for t0 in range(0, max(1, len(targets)), MAX_PLAN_TARGETS):
-        part = targets[t0:t0 + MAX_PLAN_TARGETS]
+    t0, size = 0, MAX_PLAN_TARGETS
+    while t0 < max(1, len(targets)):
+        part = targets[t0:t0 + size]
         key = (in_occ, part.shape, part.tobytes())
         plan = plans.get(key)
         if plan is None:
-            plan = plans[key] = MeshPlan(program, in_occ, part)
-        wp = None if w is None else w[t0:t0 + MAX_PLAN_TARGETS]
+            try:
+                plan = MeshPlan(program, in_occ, part)
+            except _native.NativeError as e:
+                if "failed (-4)" not in str(e) or size == 1:
+                    raise
+                size //= 2   # the part's tables do not fit the kernel's LDS: smaller parts
+                continue
+            plans[key] = plan
+        wp = None if w is None else w[t0:t0 + size]
+        t0 += size
         if not (wp is None and plan.weights is None) and not (wp is not None and plan.weights is not None
                                                               and np.array_equal(wp, plan.weights)):
             plan.set_weights(wp)

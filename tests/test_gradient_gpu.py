@@ -130,6 +130,35 @@ def test_limits_bunched_and_long_program(gpu, m, inp):
             assert abs(grad[b, k] - fd) < 1e-12 + 1e-6 * abs(fd), (b, k, grad[b, k], fd)
 
 
+def test_target_blocks_and_split_plans(gpu):
+    """512 bunched targets at 32 modes and 8 photons: their tables exceed the
+    kernel's LDS in one plan (-4), so mesh_adjoint runs two plans of 256, each
+    in 4 blocks of 64 targets.  Loss and gradient equal the sums over 8
+    single-block plans of 64 targets (to rounding), and the loss the oracle's."""
+    from perceval_amd.gradient import MeshPlan, MeshProgram, mesh_adjoint
+    m, inp = 32, [2, 1, 1, 1, 1, 1, 1] + [0] * 25
+    rng = np.random.RandomState(17)
+    prog = MeshProgram(clements_mesh(m))
+    phases = rng.uniform(0, 2 * math.pi, size=(2, prog.nparams))
+    targets = _random_targets(m, sum(inp), 512, rng)
+    loss, grad = mesh_adjoint(prog, phases, inp, targets)
+    loss, grad = loss.cpu().numpy(), grad.cpu().numpy()
+    assert sorted(k[1][0] for k in prog._plans) == [256, 256]
+    ref_l, ref_g = 0.0, 0.0
+    for t0 in range(0, 512, 64):
+        plan = MeshPlan(prog, inp, targets[t0:t0 + 64])
+        lp, gp = plan(phases)
+        ref_l, ref_g = ref_l + lp.cpu().numpy(), ref_g + gp.cpu().numpy()
+        plan.close()
+    assert np.max(np.abs(loss - ref_l)) < 1e-13 * max(1.0, np.max(np.abs(ref_l)))
+    assert np.max(np.abs(grad - ref_g)) < 1e-13 * max(1.0, np.max(np.abs(ref_g)))
+    for b in range(2):
+        U = prog.unitary(phases[b])
+        want = sum(abs(oracle.perm(oracle.submatrix(U, inp, t))) ** 2 /
+                   (oracle.prodnfact(t) * oracle.prodnfact(inp)) for t in targets)
+        assert abs(loss[b] - want) < 1e-12
+
+
 def test_one_shot_entry_matches_the_plan(gpu):
     """pcvl_mesh_adjoint (create, execute, synchronise, destroy in one call)
     against a MeshPlan executed twice on new phases: the same bits."""
@@ -162,8 +191,9 @@ def test_weighted_observable_over_every_output(gpu):
     """The objective of benchmark/benchmark_QML-DE-solver.py:61-84 on a phase
     mesh: 6 photons in 6 modes, every one of the 462 output states weighted by
     a seeded lambda (standardised and scaled by 200 as there), loss =
-    sum_s lambda_s P(s).  The 462 targets run as 8 plans of at most 64; the
-    loss matches the oracle's SLOS probabilities, the gradient central
+    sum_s lambda_s P(s).  The 462 targets run as one plan, in 8 blocks of at
+    most 64 inside the kernel; the loss matches the oracle's SLOS
+    probabilities, the gradient central
     differences of that loss (h = 1e-6), and the unweighted loss is the total
     probability 1."""
     import torch

@@ -224,6 +224,14 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     assert "2^36" in err(lib.pcvl_mis_chain(out, (1 << 36) + 1, 0.0, 1, 0, out, out, None))
     assert "2^36" in err(lib.pcvl_mis_chain(out, -1, 0.0, 1, 0, out, out, None))
     assert "null" in err(lib.pcvl_mis_chain(None, 10, 0.0, 1, 0, out, out, None))
+    # adjoint plans: at most 512 targets; tables beyond the kernel's LDS give -4 (the caller splits the targets)
+    one = (ctypes.c_int32 * 4)(1, 0, 0, 0)
+    tgt = (ctypes.c_int32 * (4 * 513))(*([1, 0, 0, 0] * 513))
+    plan = ctypes.c_void_p()
+    assert "[1, 512]" in err(lib.pcvl_mesh_plan_create(None, None, None, None, 0, 4, 0, one, tgt, 513,
+                                                       ctypes.byref(plan)))
+    rc = lib.pcvl_mesh_plan_create(None, None, None, None, 0, 4, 20000, one, tgt, 1, ctypes.byref(plan))
+    assert rc == -4 and "LDS" in lib.pcvl_last_error().decode() and not plan.value
 
 
 def test_output_submatrices_match_the_per_state_construction():
