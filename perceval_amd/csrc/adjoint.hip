@@ -31,7 +31,7 @@ constexpr int ADJ_MAX_M = 32;
 constexpr int ADJ_MAX_N = 8;
 constexpr int ADJ_MAX_TARGETS = 512;     // targets of one plan (its LDS decides what fits)
 constexpr int ADJ_BLOCK_TARGETS = 64;    // targets whose cofactors are in LDS at once
-constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_THREADS = 512;
 
 struct MeshOp {
     int32_t kind;    // 0: phase shifter on mode q; 1: fixed 2x2 block on (q, q+1)
