@@ -705,7 +705,7 @@ def bench_sampling(args, res):
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     log(f"[bench] proposal training: {train_s:.2f} s, mean -log q per epoch {['%.3f' % h for h in hist]}")
-    mis = LearnedMIS(U, inp, net, chunk=65536)
+    mis = LearnedMIS(U, inp, net, chunk=131072)
     x0 = occ_to_seq(cc_samples_device(dU, inp, 1, seed=7), n)[0]
     mis.run(4096, seed=3, x0_seq=x0)                              # warm-up
     with Timer() as t:

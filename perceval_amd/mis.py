@@ -226,9 +226,10 @@ class LearnedMIS:
     proposals scored per round of launches (the chain's draws do not depend
     on it; larger rounds leave fewer tails where the device drains: at
     m=60, n=20, rounds of 65,536 take about 5% less time than rounds of
-    16,384)."""
+    16,384, and one round of 131,072 for 10^5 samples 1.3% less than two of
+    65,536; a round's workspace is about 1.8 KB per proposal)."""
 
-    def __init__(self, U, in_occ, net: ProposalNet, chunk: int = 65536, multiset: bool = True):
+    def __init__(self, U, in_occ, net: ProposalNet, chunk: int = 131072, multiset: bool = True):
         import torch
         self.in_occ = list(in_occ)
         self.m = len(self.in_occ)
